@@ -1,0 +1,152 @@
+/*
+ * west_hip.h -- C ABI of the MI355X Wstat dielectric-response library
+ * (libwesthip.so).
+ *
+ * It replaces, behind the reference's own call surfaces, the Gamma-point
+ * hot path of WEST's Wstat:
+ *
+ *   davidson_diago_gamma  (Wstat/davidson_diago.f90:31-437)
+ *     -> apply_operator   (Wstat/apply_operator.f90:14-69)
+ *       -> dfpt           (DFPT_kernel/dfpt_module.f90:22-533)
+ *         -> linsolve_sternheimer_m_wfcts (DFPT_kernel/linsolve_sternheimer_m_wfcts.f90:14-353)
+ *           -> apply_sternheimerop_to_m_wfcs (DFPT_kernel/apply_sternheimerop_to_m_wfcs.f90:14-84)
+ *
+ * Conventions
+ *  - Arrays follow Fortran: column major, leading dimension first.  A
+ *    "complex" array is COMPLEX(DP), i.e. interleaved (re, im) doubles; the
+ *    prototypes use double* for it.  Integers are C int; scalars by value.
+ *  - Every array argument may be host memory or device memory of the
+ *    library's device; the library detects which.  Calls are synchronous:
+ *    outputs are complete when the call returns.
+ *  - The caller owns every array; registered ground-state data is copied at
+ *    wh_set_* and no caller pointer is retained.
+ *  - Return value: 0 = success; > 0 = number of unconverged Sternheimer
+ *    systems (a warning, like ierr in the reference); < 0 = error, with the
+ *    message in wh_last_error().
+ *  - One process drives one device.  Calls into the library from several
+ *    host threads at once are not supported.
+ *  - Gamma point only (iq must be 1), nspin = 1, integer occupations,
+ *    norm-conserving pseudopotentials with a diagonal D per projector.
+ */
+#ifndef WEST_HIP_H
+#define WEST_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- lifecycle ------------------------------------------------------- */
+
+/* Select the device and create the library's stream.  Replaces the device
+ * set-up of west_gpu_start (Modules/west_gpu.f90:115-140). */
+int wh_init(int device);
+/* Release all device memory.  Replaces west_gpu's deallocations
+ * (Modules/west_gpu.f90:194-227). */
+int wh_finalize(void);
+/* Message of the last failing call (errore in the reference). */
+const char* wh_last_error(void);
+
+/* ---- ground state: the module globals the reference reads ------------- */
+
+/* FFT grid and wave-function sphere: dffts%nr1..3, npw, npwx, gstart,
+ * dffts%nl / dffts%nlm (1-based, as in QE), omega, tpiba2, g2kin
+ * (DFPT_kernel/dfpt_module.f90:27-35, FFT_kernel/fft_at_gamma.f90:59-63,
+ *  wvfct%g2kin used by precondition_m_wfcts.f90:41 and set_eprec.f90:46). */
+int wh_set_grid(int nr1, int nr2, int nr3, int npw, int npwx, int gstart, const int* nl, const int* nlm,
+                double omega, double tpiba2, const double* g2kin);
+
+/* Local potential vrs(nr1*nr2*nr3) in Ry (scf%vrs used by QE vloc_psi). */
+int wh_set_potential(const double* vrs);
+
+/* Kleinman-Bylander projectors vkb(npwx, nkb) (complex) and their diagonal
+ * coefficients dvan(nkb) in Ry (uspp%vkb from init_us_2,
+ * DFPT_kernel/dfpt_module.f90:157-161; deeq used by QE add_vuspsi). */
+int wh_set_nonlocal(int nkb, const double* vkb, const double* dvan);
+
+/* Occupied states: evc(npwx, nbndval) (complex), et(nbndval), occ(nbndval),
+ * the k-point weight (k_grid%weight, dfpt_module.f90:486) and alpha_pv
+ * (westcom alphapv_dfpt, Tools/get_alpha_pv.f90:35-57). */
+int wh_set_states(int nbndval, const double* evc, const double* et, const double* occ, double weight,
+                  double alpha_pv);
+
+/* Coulomb factor sqvc(npwqx) (pot3D%sqvc, Coulomb_kernel/class_coulomb.f90:139-176). */
+int wh_set_coulomb(const double* sqvc);
+
+/* n_dfpt_maxiter (westcom, default 250) and l_kinetic_only
+ * (apply_sternheimerop_to_m_wfcs.f90:49-50). */
+int wh_set_params(int n_dfpt_maxiter, int l_kinetic_only);
+
+/* ---- the hot path --------------------------------------------------- */
+
+/* hpsi = H psi - e psi + alpha P_v psi for m bands (psi, hpsi: (npwx, m)
+ * complex; alpha: one complex scalar, by reference).
+ * Replaces apply_sternheimerop_to_m_wfcs (DFPT_kernel/apply_sternheimerop_to_m_wfcs.f90:14). */
+int wh_apply_sternheimerop_to_m_wfcs(int nbndval, const double* psi, double* hpsi, const double* e,
+                                     const double* alpha, int m);
+
+/* Preconditioned CG for (H - e_i + alpha_pv P_v) x_i = b_i, i = 1..m.
+ * b (npwx, m) complex is destroyed, x is the in/out guess; *ierr = number
+ * of unconverged columns after n_dfpt_maxiter iterations.
+ * Replaces linsolve_sternheimer_m_wfcts (DFPT_kernel/linsolve_sternheimer_m_wfcts.f90:14). */
+int wh_linsolve_sternheimer_m_wfcts(int nbndval, int m, double* b, double* x, const double* e, const double* eprec,
+                                    double tr2, int* ierr);
+
+/* dng = chi0 dvg for m perturbations (dvg, dng: (npwqx, m) complex); tr2 < 0
+ * selects the lite path (no Sternheimer solve).  ierr_per_pert (m, may be
+ * NULL) receives the unconverged count of each perturbation.
+ * Replaces dfpt (DFPT_kernel/dfpt_module.f90:22). */
+int wh_dfpt(int m, const double* dvg, double* dng, double tr2, int iq, int* ierr_per_pert);
+
+/* dng = sqvc * chi0 (sqvc * dvg).  Replaces apply_operator (Wstat/apply_operator.f90:14). */
+int wh_apply_operator(int m, const double* dvg, double* dng, double tr2, int iq);
+
+/* Davidson for the n_pdep_eigen lowest eigenpairs of sqvc chi0 sqvc.
+ * Inputs are the wstat_control values of westcom (Modules/westcom.f90:118-152);
+ * the start vectors are the library's seeded random vectors (do_randomize,
+ * Wstat/davidson_diago.f90:1121-1182).  Outputs: ev(n_pdep_eigen),
+ * conv(n_pdep_eigen) (1 = converged), dvg_out(npwqx, n_pdep_eigen) complex
+ * (may be NULL), and one log record per Davidson step (may be NULL).
+ * Replaces davidson_diago_gamma (Wstat/davidson_diago.f90:31). */
+typedef struct {
+  int n_pdep_eigen;
+  int n_pdep_times;
+  int n_pdep_maxiter;
+  double trev_pdep;
+  double trev_pdep_rel;
+  double tr2_dfpt;
+} wh_davidson_params;
+
+typedef struct {            /* exec.davitr[] of output_ev_and_time (davidson_diago.f90:1310-1329) */
+  int dav_iter;
+  int notcnv;
+  int dfpt_dim;
+  int diago_dim;
+  long sternop_ncalls;
+  double dfpt_tr2;
+  double time_elap;
+  double time_iter;
+} wh_davidson_step;
+
+int wh_davidson_diago(const wh_davidson_params* params, double* ev, int* conv, double* dvg_out,
+                      wh_davidson_step* log, int log_capacity, int* log_len);
+
+/* ---- counters -------------------------------------------------------- */
+
+/* Operator applications so far (the 'stern' clock count of the reference,
+ * test-suite/test001/ref/wstat.json:820-826) and CG iterations. */
+int wh_get_counters(long* stern_ncalls, long* cg_iters);
+
+/* Per-kernel device timing with HIP events on the library stream.
+ * wh_profile(1) enables and clears, wh_profile(0) disables.
+ * wh_profile_query(name, &ms, &count) returns the accumulated time (ms) and
+ * launch count of one kernel class ("zinv", "plane_vloc", "zfwd",
+ * "plane_density", "plane_to_real", "gemm_bra", "gemm_ket", "cg_prec",
+ * "cg_dir", "cg_update"). */
+int wh_profile(int enable);
+int wh_profile_query(const char* name, double* ms, long* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WEST_HIP_H */

@@ -1,0 +1,152 @@
+"""Parity of the HIP path with the FP64 oracle, through the C ABI.
+
+Tolerances (FP64 throughout):
+* operator applications (FFT + GEMM): relative max error <= 1e-12;
+* Sternheimer solutions and chi0 at a tight threshold: relative <= 1e-9
+  (two CG trajectories that differ only by rounding);
+* PDEP eigenvalues: relative <= 1e-8 (the bar of the plain FP64 path).
+"""
+import numpy as np
+import pytest
+
+from conftest import random_wfc
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def _register(gpu, gs, kinetic_only=False, maxiter=250):
+    from oracle.dfpt import sqvc_init
+    sq = sqvc_init(gs.basis, gs.npwx)
+    gpu.set_ground_state(gs, sqvc=sq, n_dfpt_maxiter=maxiter, l_kinetic_only=kinetic_only)
+    gs.l_kinetic_only = kinetic_only
+    return sq
+
+
+@pytest.mark.parametrize("key", ["h2", "3h2", "si8"])
+@pytest.mark.parametrize("m", [1, 2, 5])
+def test_sternheimer_operator(gpu, key, m, request):
+    from oracle.hamiltonian import apply_sternheimerop
+    gs = request.getfixturevalue(f"gs_{key}")
+    _register(gpu, gs)
+    print(f"case {key} m={m}", flush=True)
+    psi = random_wfc(gs, m, seed=11 + m)
+    e = np.linspace(-0.7, 0.3, m)
+    ref = apply_sternheimerop(gs, psi, e, gs.alpha_pv)
+    out = gpu.apply_sternheimerop_to_m_wfcs(gs.nbndval, psi, e, gs.alpha_pv)
+    assert _rel(out, ref) < 1e-12
+
+
+def test_kinetic_only_operator(gpu, gs_si8):
+    from oracle.hamiltonian import apply_sternheimerop
+    gs = gs_si8
+    _register(gpu, gs, kinetic_only=True)
+    try:
+        psi = random_wfc(gs, 4, seed=3)
+        e = np.array([-0.5, -0.2, 0.1, 0.3])
+        ref = apply_sternheimerop(gs, psi, e, gs.alpha_pv)
+        out = gpu.apply_sternheimerop_to_m_wfcs(gs.nbndval, psi, e, gs.alpha_pv)
+        assert _rel(out, ref) < 1e-13
+    finally:
+        gs.l_kinetic_only = False
+
+
+def test_h_psi_without_states(gpu, gs_si8):
+    """H psi with no registered states (the ground-state eigensolver's use)."""
+    from oracle.hamiltonian import h_psi
+    gs = gs_si8
+    h = gpu.gpu_h_factory(gs)
+    psi = random_wfc(gs, 3, seed=5)
+    assert _rel(h(psi), h_psi(gs, psi)) < 1e-12
+
+
+@pytest.mark.parametrize("key", ["3h2", "si8"])
+def test_linsolve(gpu, key, request):
+    from oracle.hamiltonian import precondition_m_wfcts, set_eprec
+    from oracle.linsolve import linsolve_sternheimer_m_wfcts
+    gs = request.getfixturevalue(f"gs_{key}")
+    _register(gpu, gs)
+    nv = gs.nbndval
+    b = random_wfc(gs, nv, seed=21)
+    eprec = set_eprec(gs, gs.evc[:nv])
+    x0 = precondition_m_wfcts(gs, b, eprec)
+    xr = x0.copy()
+    ierr_ref = linsolve_sternheimer_m_wfcts(gs, nv, b.copy(), xr, gs.et[:nv], eprec, 1e-24)
+    xg = x0.copy()
+    ierr = gpu.linsolve_sternheimer_m_wfcts(nv, nv, b.copy(), xg, gs.et[:nv], eprec, 1e-24)
+    assert ierr == ierr_ref == 0
+    assert _rel(xg, xr) < 1e-9
+
+
+def test_linsolve_nonconvergence_reported(gpu, gs_si8):
+    """n_dfpt_maxiter = 1 leaves systems unconverged: ierr > 0, not an error."""
+    from oracle.hamiltonian import precondition_m_wfcts, set_eprec
+    gs = gs_si8
+    _register(gpu, gs, maxiter=1)
+    nv = gs.nbndval
+    b = random_wfc(gs, nv, seed=4)
+    eprec = set_eprec(gs, gs.evc[:nv])
+    x = precondition_m_wfcts(gs, b, eprec)
+    ierr = gpu.linsolve_sternheimer_m_wfcts(nv, nv, b, x, gs.et[:nv], eprec, 1e-30)
+    assert ierr == nv
+    _register(gpu, gs)
+
+
+@pytest.mark.parametrize("key", ["h2", "3h2", "si8"])
+@pytest.mark.parametrize("tr2", [-1.0, 1e-22])
+def test_dfpt(gpu, key, tr2, request):
+    from oracle.dfpt import dfpt
+    gs = request.getfixturevalue(f"gs_{key}")
+    _register(gpu, gs)
+    print(f"case {key} tr2={tr2}", flush=True)
+    m = 3
+    dvg = random_wfc(gs, m, seed=31)
+    ref = dfpt(gs, dvg, tr2)
+    out = gpu.dfpt(m, dvg, tr2)
+    assert np.all(out[:, 0] == 0)
+    assert _rel(out, ref) < (1e-12 if tr2 < 0 else 1e-9)
+
+
+def test_apply_operator(gpu, gs_si8):
+    from oracle.dfpt import apply_operator
+    gs = gs_si8
+    sq = _register(gpu, gs)
+    dvg = random_wfc(gs, 4, seed=41)
+    ref = apply_operator(gs, sq, dvg, 1e-22)
+    out = gpu.apply_operator(4, dvg, 1e-22)
+    assert _rel(out, ref) < 1e-9
+
+
+def test_dfpt_device_arrays(gpu, gs_3h2):
+    """Device-resident input/output through the same entry point."""
+    import torch
+    from oracle.dfpt import dfpt
+    gs = gs_3h2
+    _register(gpu, gs)
+    dvg = random_wfc(gs, 2, seed=51)
+    ref = dfpt(gs, dvg, 1e-22)
+    d_in = torch.from_numpy(dvg).to("cuda")
+    d_out = torch.zeros_like(d_in)
+    gpu.dfpt(2, d_in, 1e-22, dng=d_out)
+    assert _rel(d_out.cpu().numpy(), ref) < 1e-9
+
+
+@pytest.mark.parametrize("key", ["3h2", "si8"])
+def test_davidson(gpu, key, request):
+    from oracle.davidson import davidson_diago_gamma
+    from oracle.dfpt import sqvc_init
+    gs = request.getfixturevalue(f"gs_{key}")
+    sq = _register(gpu, gs)
+    nvec = 4
+    log_ref = []
+    ev_ref, conv_ref, _ = davidson_diago_gamma(gs, sq, nvec, tr2_dfpt=1e-20, trev_pdep=1e-6, trev_pdep_rel=1e-4,
+                                               log=log_ref)
+    ev, conv, dvg, log = gpu.davidson_diago(nvec, tr2_dfpt=1e-20, trev_pdep=1e-6, trev_pdep_rel=1e-4)
+    print("ev", ev, "ref", ev_ref, flush=True)
+    assert np.all(conv == conv_ref)
+    assert np.max(np.abs(ev - ev_ref) / np.abs(ev_ref)) < 1e-8
+    assert [r["dfpt_dim"] for r in log] == [r["dfpt_dim"] for r in log_ref]
+    assert [r["diago_dim"] for r in log] == [r["diago_dim"] for r in log_ref]

@@ -1,0 +1,709 @@
+// C ABI of libwesthip: ground-state registration, the Sternheimer operator,
+// the CG solver and chi0 (dfpt / apply_operator) on one MI355X.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "../../include/west_hip.h"
+#include "wh_context.h"
+#include "wh_gemm.h"
+#include "wh_vec.h"
+
+namespace wh {
+
+// ---------------------------------------------------------------------------
+// error state
+// ---------------------------------------------------------------------------
+static std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+const char* last_error() { return g_last_error.c_str(); }
+
+static Context* g_ctx = nullptr;
+Context& ctx() {
+  if (!g_ctx) throw Error{-1, "wh_init has not been called"};
+  return *g_ctx;
+}
+
+void require_ready(bool need_states) {
+  Context& c = ctx();
+  WH_REQUIRE(c.have_grid, "wh_set_grid has not been called");
+  WH_REQUIRE(c.l_kinetic_only || c.have_pot, "wh_set_potential has not been called");
+  if (need_states) WH_REQUIRE(c.have_states, "wh_set_states has not been called");
+}
+
+int Context::items_capacity() const {
+  const long per = 2L * sticks_bytes();
+  long cap = static_cast<long>(sticks_budget) / std::max<long>(per, 1);
+  return static_cast<int>(std::max<long>(1, std::min<long>(cap, 1L << 16)));
+}
+
+// ---------------------------------------------------------------------------
+// profiler
+// ---------------------------------------------------------------------------
+hipEvent_t Profiler::take() {
+  if (pool.empty()) {
+    hipEvent_t e;
+    WH_HIP(hipEventCreate(&e));
+    return e;
+  }
+  hipEvent_t e = pool.back();
+  pool.pop_back();
+  return e;
+}
+
+void Profiler::flush() {
+  for (auto& p : pending) {
+    WH_HIP(hipEventSynchronize(p.b));
+    float ms = 0.f;
+    WH_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+    auto& slot = acc[p.name];
+    slot.first += ms;
+    slot.second += 1;
+    pool.push_back(p.a);
+    pool.push_back(p.b);
+  }
+  pending.clear();
+}
+
+void Profiler::clear() {
+  flush();
+  acc.clear();
+}
+
+// ---------------------------------------------------------------------------
+// host/device staging of caller arrays
+// ---------------------------------------------------------------------------
+static bool on_device(const void* p) {
+  if (!p) return false;
+  hipPointerAttribute_t attr;
+  hipError_t e = hipPointerGetAttributes(&attr, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+// device view of an input array (copied if it is host memory)
+template <typename T>
+static const T* dev_in(const T* p, size_t count, DevBuf& tmp) {
+  if (on_device(p)) return p;
+  T* d = tmp.as<T>(count);
+  WH_HIP(hipMemcpyAsync(d, p, sizeof(T) * count, hipMemcpyHostToDevice, ctx().stream));
+  return d;
+}
+
+// ---------------------------------------------------------------------------
+// projectors
+// ---------------------------------------------------------------------------
+static void rebuild_q() {
+  Context& c = ctx();
+  const int rows = c.nbndval + c.nkb;
+  if (c.d_q) WH_HIP(hipFree(c.d_q));
+  if (c.d_qscale) WH_HIP(hipFree(c.d_qscale));
+  c.d_q = nullptr;
+  c.d_qscale = nullptr;
+  c.q_rows = rows;
+  if (rows == 0) return;
+  std::vector<z64> q(static_cast<size_t>(rows) * c.npw);
+  if (c.nbndval) std::memcpy(q.data(), c.evc_s.data(), sizeof(z64) * c.nbndval * c.npw);
+  if (c.nkb) std::memcpy(q.data() + static_cast<size_t>(c.nbndval) * c.npw, c.vkb_s.data(), sizeof(z64) * c.nkb * c.npw);
+  WH_HIP(hipMalloc(&c.d_q, sizeof(z64) * q.size()));
+  WH_HIP(hipMemcpy(c.d_q, q.data(), sizeof(z64) * q.size(), hipMemcpyHostToDevice));
+  std::vector<double> sc(rows);
+  for (int i = 0; i < c.nbndval; ++i) sc[i] = c.alpha_pv;
+  for (int i = 0; i < c.nkb; ++i) sc[c.nbndval + i] = c.dvan[i];
+  c.qscale_alpha = c.alpha_pv;
+  WH_HIP(hipMalloc(&c.d_qscale, sizeof(double) * rows));
+  WH_HIP(hipMemcpy(c.d_qscale, sc.data(), sizeof(double) * rows, hipMemcpyHostToDevice));
+}
+
+static void set_qscale_alpha(double alpha) {
+  Context& c = ctx();
+  if (c.nbndval == 0 || alpha == c.qscale_alpha) return;
+  std::vector<double> sc(c.nbndval, alpha);
+  WH_HIP(hipMemcpy(c.d_qscale, sc.data(), sizeof(double) * c.nbndval, hipMemcpyHostToDevice));
+  c.qscale_alpha = alpha;
+}
+
+// cmat (nq x m, col major) = glbrak_gamma(Q[row0:row0+nq], f[0:m])
+//   = 2 Q_r^T f_r - Re Q(G=0) Re f(G=0)     (Hamiltonian_kernel/glbrak.f90:38-39)
+void glbrak_q(int row0, int nq, const z64* f, int m, double* cmat) {
+  Context& c = ctx();
+  const long k2 = 2L * c.npw;
+  const double* q = reinterpret_cast<const double*>(c.d_q) + static_cast<long>(row0) * k2;
+  const double* fr = reinterpret_cast<const double*>(f);
+  WH_TIMED("gemm_bra", dgemm(true, nq, m, static_cast<int>(k2), 2.0, q, k2, fr, k2, 0.0, cmat, nq, c.stream));
+  WH_TIMED("gemm_fix", g0_fix_scale(nq, m, c.gstart == 2 ? 1.0 : 0.0, q, k2, fr, k2, nullptr, cmat, nq, c.stream));
+}
+
+// ---------------------------------------------------------------------------
+// Sternheimer operator: t = H x - e x + alpha P_v x
+//   (DFPT_kernel/apply_sternheimerop_to_m_wfcs.f90:49-76; QE h_psi_ at Gamma:
+//    kinetic + vloc_psi + calbec/add_vuspsi)
+// ---------------------------------------------------------------------------
+void stern_apply(const z64* x, z64* t, const double* d_e, int m, double alpha) {
+  Context& c = ctx();
+  if (m <= 0) return;
+  hipStream_t s = c.stream;
+  const int npw = c.npw;
+  const long ld = npw;
+  if (c.l_kinetic_only) {
+    WH_TIMED("kinetic", kinetic_minus_e(m, npw, x, t, ld, c.d_g2kin, d_e, s));
+  } else {
+    const int npair = (m + 1) / 2;
+    const int cap = c.items_capacity();
+    FftItem* items = c.w_items.as<FftItem>(npair);
+    fill_pairs(items, m, 0, s);
+    FftItem* ident = c.w_items_id.as<FftItem>(std::min(cap, npair));
+    fill_ident(ident, std::min(cap, npair), s);
+    const int chunk = std::min(cap, npair);
+    z64* sa = c.w_sticks_a.as<z64>(static_cast<size_t>(chunk) * c.plan.sticks_elems());
+    z64* sb = c.w_sticks_b.as<z64>(static_cast<size_t>(chunk) * c.plan.sticks_elems());
+    ZfwdArgs za{ZFWD_PAIR_STERN, 1.0, c.d_g2kin, d_e, nullptr, 0};
+    for (int q0 = 0; q0 < npair; q0 += chunk) {
+      const int nq = std::min(chunk, npair - q0);
+      WH_TIMED("zinv", launch_zinv(c.plan, items + q0, nq, x, ld, sa, ZINV_PAIR, s));
+      WH_TIMED("plane_vloc", launch_plane_vloc(c.plan, ident, nq, sa, sb, c.d_vrs, 0, 1.0 / c.nnr, s));
+      WH_TIMED("zfwd", launch_zfwd(c.plan, items + q0, nq, sb, x, ld, t, ld, za, s));
+    }
+  }
+  // projector part: rows [evc ; vkb] (or evc only for the kinetic-only operator)
+  const int nq = c.l_kinetic_only ? c.nbndval : c.nbndval + c.nkb;
+  if (nq > 0) {
+    set_qscale_alpha(alpha);
+    const long k2 = 2L * npw;
+    double* cm = c.w_cmat.as<double>(static_cast<size_t>(nq) * m);
+    const double* q = reinterpret_cast<const double*>(c.d_q);
+    const double* xr = reinterpret_cast<const double*>(x);
+    WH_TIMED("gemm_bra", dgemm(true, nq, m, static_cast<int>(k2), 2.0, q, k2, xr, k2, 0.0, cm, nq, s));
+    WH_TIMED("gemm_fix", g0_fix_scale(nq, m, c.gstart == 2 ? 1.0 : 0.0, q, k2, xr, k2, c.d_qscale, cm, nq, s));
+    WH_TIMED("gemm_ket", dgemm(false, static_cast<int>(k2), m, nq, 1.0, q, k2, cm, nq, 1.0,
+                               reinterpret_cast<double*>(t), k2, s));
+  }
+  c.nstern += m;
+}
+
+// ---------------------------------------------------------------------------
+// Preconditioned CG  (DFPT_kernel/linsolve_sternheimer_m_wfcts.f90:107-332)
+// ---------------------------------------------------------------------------
+int linsolve_dev(int m, z64* b, z64* x, const std::vector<double>& e_col, const double* d_e_col,
+                 const double* d_eprec_col, double tr2, std::vector<int>* ierr_col) {
+  Context& c = ctx();
+  hipStream_t s = c.stream;
+  const int npw = c.npw;
+  const long ld = npw;
+  const size_t vec = static_cast<size_t>(m) * npw;
+  z64* g = c.w_g.as<z64>(vec);
+  z64* h = c.w_h.as<z64>(vec);
+  z64* hold = c.w_hold.as<z64>(vec);
+  z64* bc = c.w_bc.as<z64>(vec);
+  z64* tc = c.w_tc.as<z64>(vec);
+  int* d_l2i = c.w_l2i.as<int>(m);
+  double* d_beta = c.w_beta.as<double>(m);
+  double* d_eu = c.w_eu.as<double>(m);
+  double* d_rho = c.w_rho.as<double>(m);
+  // step 1: g = A x - b
+  stern_apply(x, g, d_e_col, m, c.alpha_pv);
+  axpby(static_cast<long>(vec), -1.0, b, 1.0, g, s);
+  std::vector<char> is_conv(m, 0);
+  std::vector<int> l2i(m);
+  std::iota(l2i.begin(), l2i.end(), 0);
+  std::vector<double> rho(m, 0.0), rhoold(m, 0.0), rho_c(m), beta(m), eu(m);
+  int nc = m;
+  for (int iter = 1; iter <= c.n_dfpt_maxiter; ++iter) {
+    WH_HIP(hipMemcpyAsync(d_l2i, l2i.data(), sizeof(int) * nc, hipMemcpyHostToDevice, s));
+    WH_TIMED("cg_prec", prec_rho(nc, npw, c.gstart, d_l2i, g, h, ld, c.d_g2kin, d_eprec_col, d_rho, s));
+    WH_HIP(hipMemcpyAsync(rho_c.data(), d_rho, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
+    WH_HIP(hipStreamSynchronize(s));
+    ++c.cg_iters;
+    for (int lb = 0; lb < nc; ++lb) {
+      const int ib = l2i[lb];
+      rho[ib] = rho_c[lb];
+      if (rho[ib] < tr2) is_conv[ib] = 1;
+    }
+    int n2 = 0;
+    for (int ib = 0; ib < m; ++ib)
+      if (!is_conv[ib]) l2i[n2++] = ib;
+    nc = n2;
+    if (nc == 0) break;
+    for (int lb = 0; lb < nc; ++lb) {
+      const int ib = l2i[lb];
+      beta[lb] = (iter != 1) ? rho[ib] / rhoold[ib] : 0.0;
+      eu[lb] = e_col[ib];
+    }
+    WH_HIP(hipMemcpyAsync(d_l2i, l2i.data(), sizeof(int) * nc, hipMemcpyHostToDevice, s));
+    WH_HIP(hipMemcpyAsync(d_beta, beta.data(), sizeof(double) * nc, hipMemcpyHostToDevice, s));
+    WH_HIP(hipMemcpyAsync(d_eu, eu.data(), sizeof(double) * nc, hipMemcpyHostToDevice, s));
+    WH_TIMED("cg_dir", direction(nc, npw, d_l2i, d_beta, h, hold, bc, ld, s));
+    stern_apply(bc, tc, d_eu, nc, c.alpha_pv);
+    WH_TIMED("cg_update", cg_update(nc, npw, c.gstart, d_l2i, h, g, tc, x, ld, nullptr, s));
+    std::swap(h, hold);   // hold <- h  (linsolve 319)
+    for (int lb = 0; lb < nc; ++lb) rhoold[l2i[lb]] = rho[l2i[lb]];
+    // keep the staging vectors alive until the copies above have been consumed
+    WH_HIP(hipStreamSynchronize(s));
+  }
+  if (ierr_col) {
+    ierr_col->assign(m, 0);
+    for (int lb = 0; lb < nc; ++lb) (*ierr_col)[l2i[lb]] = 1;
+  }
+  return nc;
+}
+
+// ---------------------------------------------------------------------------
+// chi0 at Gamma  (DFPT_kernel/dfpt_module.f90:139-503, Gamma branches)
+//   dvg: m perturbations (rows, stick order); dng: m rows out.
+//   out_weight: per-G factor applied to dng (sqvc for apply_operator) or null.
+// ---------------------------------------------------------------------------
+int dfpt_dev(int m, const z64* dvg, z64* dng, double tr2, const double* out_weight, std::vector<int>* ierr_pert) {
+  Context& c = ctx();
+  require_ready(true);
+  hipStream_t s = c.stream;
+  const int npw = c.npw;
+  const long ld = npw;
+  const int nv = c.nbndval;
+  const int nevp = (nv + 1) / 2;
+  const long sl = c.plan.sticks_elems();
+  const int cap = c.items_capacity();
+  if (ierr_pert) ierr_pert->assign(m, 0);
+  if (m <= 0) return 0;
+
+  // occupied-pair sticks, once per ground state
+  if (!c.evc_sticks_valid) {
+    z64* es = c.evc_sticks.as<z64>(static_cast<size_t>(nevp) * sl);
+    FftItem* it = c.w_items2.as<FftItem>(nevp);
+    fill_pairs(it, nv, 0, s);
+    WH_TIMED("zinv", launch_zinv(c.plan, it, nevp, c.d_q, ld, es, ZINV_PAIR, s));
+    c.evc_sticks_valid = true;
+  }
+  // 1. dv(r) of every perturbation (single_invfft_gamma, dfpt_module.f90:229-235)
+  double* dvr = c.w_dvr.as<double>(static_cast<size_t>(m) * c.nnr);
+  {
+    const int npp = (m + 1) / 2;
+    FftItem* it = c.w_items.as<FftItem>(npp);
+    fill_pairs(it, m, 0, s);
+    const int chunk = std::min(cap, npp);
+    z64* sa = c.w_sticks_a.as<z64>(static_cast<size_t>(chunk) * sl);
+    for (int q0 = 0; q0 < npp; q0 += chunk) {
+      const int nq = std::min(chunk, npp - q0);
+      WH_TIMED("zinv", launch_zinv(c.plan, it + q0, nq, dvg, ld, sa, ZINV_PAIR_REALG0, s));
+      WH_TIMED("plane_to_real", launch_plane_to_real(c.plan, it + q0, nq, sa, dvr, c.nnr, s));
+    }
+  }
+  // 2. right-hand side dvpsi = P[psi_v dv]  (dfpt_module.f90:245-278)
+  const long M = static_cast<long>(m) * nv;
+  z64* dvpsi = c.w_b.as<z64>(static_cast<size_t>(M) * npw);
+  z64* dpsi = c.w_x.as<z64>(static_cast<size_t>(M) * npw);
+  {
+    const long nitem = static_cast<long>(m) * nevp;
+    std::vector<FftItem> hp(nitem), hz(nitem);
+    for (int p = 0; p < m; ++p)
+      for (int q = 0; q < nevp; ++q) {
+        const long i = static_cast<long>(p) * nevp + q;
+        hp[i] = FftItem{q, -1, p, 0};
+        hz[i] = FftItem{p * nv + 2 * q, (2 * q + 1 < nv) ? p * nv + 2 * q + 1 : -1, -1, 0};
+      }
+    FftItem* dp = c.w_items.as<FftItem>(2 * nitem);
+    FftItem* dz = dp + nitem;
+    WH_HIP(hipMemcpyAsync(dp, hp.data(), sizeof(FftItem) * nitem, hipMemcpyHostToDevice, s));
+    WH_HIP(hipMemcpyAsync(dz, hz.data(), sizeof(FftItem) * nitem, hipMemcpyHostToDevice, s));
+    const int chunk = static_cast<int>(std::min<long>(cap, nitem));
+    z64* sb = c.w_sticks_b.as<z64>(static_cast<size_t>(chunk) * sl);
+    ZfwdArgs za{ZFWD_PAIR_STORE, 1.0, c.d_g2kin, nullptr, nullptr, 0};
+    for (long i0 = 0; i0 < nitem; i0 += chunk) {
+      const int ni = static_cast<int>(std::min<long>(chunk, nitem - i0));
+      WH_TIMED("plane_vloc", launch_plane_vloc(c.plan, dp + i0, ni, static_cast<const z64*>(c.evc_sticks.p), sb,
+                                               dvr, c.nnr, 1.0 / c.nnr, s));
+      WH_TIMED("zfwd", launch_zfwd(c.plan, dz + i0, ni, sb, nullptr, 0, dvpsi, ld, za, s));
+    }
+    WH_HIP(hipStreamSynchronize(s));   // host item vectors go out of scope
+  }
+  // 3. dvpsi <- -P_c dvpsi = -dvpsi + evc <evc|dvpsi>  (apply_alpha_pc, alpha = -1)
+  if (nv > 0) {
+    double* cm = c.w_cmat.as<double>(static_cast<size_t>(nv) * M);
+    glbrak_q(0, nv, dvpsi, static_cast<int>(M), cm);
+    const long k2 = 2L * npw;
+    WH_TIMED("gemm_ket", dgemm(false, static_cast<int>(k2), static_cast<int>(M), nv, 1.0,
+                               reinterpret_cast<const double*>(c.d_q), k2, cm, nv, -1.0,
+                               reinterpret_cast<double*>(dvpsi), k2, s));
+  }
+  // 4. dpsi = M^-1 dvpsi (initial guess, dfpt_module.f90:348)
+  WH_TIMED("precondition", precondition(static_cast<int>(M), npw, dvpsi, dpsi, ld, c.d_g2kin, c.d_eprec, nv, s));
+  // 5. Sternheimer solve (dfpt_module.f90:350-361)
+  int nonconv = 0;
+  if (tr2 >= 0.0) {
+    std::vector<double> e_col(M), ep_col(M);
+    for (long i = 0; i < M; ++i) {
+      e_col[i] = c.et[i % nv];
+      ep_col[i] = c.eprec[i % nv];
+    }
+    double* de = c.w_ecol.as<double>(M);
+    double* dep = c.w_epcol.as<double>(M);
+    WH_HIP(hipMemcpyAsync(de, e_col.data(), sizeof(double) * M, hipMemcpyHostToDevice, s));
+    WH_HIP(hipMemcpyAsync(dep, ep_col.data(), sizeof(double) * M, hipMemcpyHostToDevice, s));
+    std::vector<int> ierr_col;
+    nonconv = linsolve_dev(static_cast<int>(M), dvpsi, dpsi, e_col, de, dep, tr2, &ierr_col);
+    if (ierr_pert)
+      for (long i = 0; i < M; ++i) (*ierr_pert)[i / nv] += ierr_col[i];
+  }
+  // 6. response density and dng (dfpt_module.f90:403-503)
+  {
+    const int pc = std::max(1, std::min(m, (2 * cap) / std::max(nv, 1)));
+    z64* sa = c.w_sticks_a.as<z64>(static_cast<size_t>(pc) * nv * sl);
+    z64* sb = c.w_sticks_b.as<z64>(static_cast<size_t>(pc) * sl);
+    std::vector<FftItem> hi(static_cast<size_t>(pc) * nv), hz(pc);
+    FftItem* di = c.w_items.as<FftItem>(static_cast<size_t>(pc) * nv + pc);
+    FftItem* dz = di + static_cast<size_t>(pc) * nv;
+    // fwfft normalisation 1/N times the 2 w_k / omega of dfpt_module.f90:486
+    ZfwdArgs za{ZFWD_SINGLE, 2.0 * c.weight / (c.omega * c.nnr), c.d_g2kin, nullptr, out_weight,
+                c.gstart == 2 ? 1 : 0};
+    for (int p0 = 0; p0 < m; p0 += pc) {
+      const int np_ = std::min(pc, m - p0);
+      for (int p = 0; p < np_; ++p) {
+        for (int v = 0; v < nv; ++v) hi[static_cast<size_t>(p) * nv + v] = FftItem{v, (p0 + p) * nv + v, -1, 0};
+        hz[p] = FftItem{p0 + p, -1, -1, 0};
+      }
+      WH_HIP(hipMemcpyAsync(di, hi.data(), sizeof(FftItem) * np_ * nv, hipMemcpyHostToDevice, s));
+      WH_HIP(hipMemcpyAsync(dz, hz.data(), sizeof(FftItem) * np_, hipMemcpyHostToDevice, s));
+      WH_TIMED("zinv", launch_zinv(c.plan, di, np_ * nv, c.d_q, ld, dpsi, ld, sa, ZINV_PAIR, s));
+      WH_TIMED("plane_density", launch_plane_density(c.plan, np_, nv, sa, c.d_occ, sb, s));
+      WH_TIMED("zfwd", launch_zfwd(c.plan, dz, np_, sb, nullptr, 0, dng, ld, za, s));
+      WH_HIP(hipStreamSynchronize(s));
+    }
+  }
+  return nonconv;
+}
+
+}  // namespace wh
+
+// ===========================================================================
+// extern "C" entry points
+// ===========================================================================
+using namespace wh;
+
+#define WH_API_BEGIN try {
+#define WH_API_END                                                  \
+  }                                                                 \
+  catch (const ::wh::Error& e) {                                    \
+    ::wh::set_error(e.msg);                                         \
+    return e.code < 0 ? e.code : -1;                                \
+  }                                                                 \
+  catch (const std::exception& e) {                                 \
+    ::wh::set_error(e.what());                                      \
+    return -3;                                                      \
+  }
+
+static void finish_call() {
+  Context& c = ctx();
+  WH_HIP(hipStreamSynchronize(c.stream));
+  if (c.prof.on) c.prof.flush();
+}
+
+extern "C" {
+
+const char* wh_last_error(void) { return last_error(); }
+
+int wh_init(int device) {
+  WH_API_BEGIN
+  int n = 0;
+  WH_HIP(hipGetDeviceCount(&n));
+  WH_REQUIRE(device >= 0 && device < n, "device index out of range");
+  if (g_ctx) {
+    WH_REQUIRE(g_ctx->device == device, "the library is already initialised on another device");
+    return 0;
+  }
+  WH_HIP(hipSetDevice(device));
+  g_ctx = new Context();
+  g_ctx->device = device;
+  WH_HIP(hipStreamCreateWithFlags(&g_ctx->stream, hipStreamNonBlocking));
+  return 0;
+  WH_API_END
+}
+
+int wh_finalize(void) {
+  WH_API_BEGIN
+  if (!g_ctx) return 0;
+  Context& c = *g_ctx;
+  (void)hipStreamSynchronize(c.stream);
+  c.prof.flush();
+  for (auto e : c.prof.pool) (void)hipEventDestroy(e);
+  c.plan.release();
+  for (void* p : {static_cast<void*>(c.d_perm), static_cast<void*>(c.d_g2kin), static_cast<void*>(c.d_vrs),
+                  static_cast<void*>(c.d_q), static_cast<void*>(c.d_qscale), static_cast<void*>(c.d_occ),
+                  static_cast<void*>(c.d_eprec), static_cast<void*>(c.d_sqvc)})
+    if (p) (void)hipFree(p);
+  for (DevBuf* b : {&c.evc_sticks, &c.w_items, &c.w_items2, &c.w_items_id, &c.w_sticks_a, &c.w_sticks_b,
+                    &c.w_cmat, &c.w_x, &c.w_t, &c.w_g, &c.w_h, &c.w_hold, &c.w_bc, &c.w_tc, &c.w_b, &c.w_dvr,
+                    &c.w_in, &c.w_out, &c.w_dvin, &c.w_dng, &c.w_l2i, &c.w_beta, &c.w_eu, &c.w_rho, &c.w_ecol, &c.w_epcol})
+    b->release();
+  (void)hipStreamDestroy(c.stream);
+  delete g_ctx;
+  g_ctx = nullptr;
+  return 0;
+  WH_API_END
+}
+
+int wh_set_grid(int nr1, int nr2, int nr3, int npw, int npwx, int gstart, const int* nl, const int* nlm,
+                double omega, double tpiba2, const double* g2kin) {
+  WH_API_BEGIN
+  Context& c = ctx();
+  WH_REQUIRE(nr1 > 0 && nr2 > 0 && nr3 > 0, "FFT dimensions must be positive");
+  WH_REQUIRE(npw > 0 && npwx >= npw, "need 0 < npw <= npwx");
+  WH_REQUIRE(gstart == 2, "Gamma point: gstart must be 2 (G = 0 is the first plane wave)");
+  WH_REQUIRE(nl && nlm && g2kin, "nl, nlm and g2kin are required");
+  std::vector<int> hnl(npw), hnlm(npw);
+  std::vector<double> hg2(npw);
+  WH_HIP(hipMemcpy(hnl.data(), nl, sizeof(int) * npw, hipMemcpyDefault));
+  WH_HIP(hipMemcpy(hnlm.data(), nlm, sizeof(int) * npw, hipMemcpyDefault));
+  WH_HIP(hipMemcpy(hg2.data(), g2kin, sizeof(double) * npw, hipMemcpyDefault));
+  c.plan.build(nr1, nr2, nr3, npw, hnl.data(), hnlm.data());
+  WH_REQUIRE(c.plan.perm[0] == 0, "G = 0 must be the first plane wave (gstart = 2)");
+  c.nr1 = nr1; c.nr2 = nr2; c.nr3 = nr3;
+  c.nnr = nr1 * nr2 * nr3;
+  c.npw = npw; c.npwx = npwx; c.gstart = gstart;
+  c.omega = omega; c.tpiba2 = tpiba2;
+  c.g2kin_s.resize(npw);
+  for (int s = 0; s < npw; ++s) c.g2kin_s[s] = hg2[c.plan.perm[s]];
+  if (c.d_perm) WH_HIP(hipFree(c.d_perm));
+  if (c.d_g2kin) WH_HIP(hipFree(c.d_g2kin));
+  WH_HIP(hipMalloc(&c.d_perm, sizeof(int) * npw));
+  WH_HIP(hipMalloc(&c.d_g2kin, sizeof(double) * npw));
+  WH_HIP(hipMemcpy(c.d_perm, c.plan.perm.data(), sizeof(int) * npw, hipMemcpyHostToDevice));
+  WH_HIP(hipMemcpy(c.d_g2kin, c.g2kin_s.data(), sizeof(double) * npw, hipMemcpyHostToDevice));
+  c.have_grid = true;
+  c.have_pot = false;
+  c.have_states = false;
+  c.have_sqvc = false;
+  c.nkb = 0;
+  c.nbndval = 0;
+  c.evc_sticks_valid = false;
+  return 0;
+  WH_API_END
+}
+
+int wh_set_potential(const double* vrs) {
+  WH_API_BEGIN
+  Context& c = ctx();
+  WH_REQUIRE(c.have_grid, "wh_set_grid first");
+  if (c.d_vrs) WH_HIP(hipFree(c.d_vrs));
+  WH_HIP(hipMalloc(&c.d_vrs, sizeof(double) * c.nnr));
+  WH_HIP(hipMemcpy(c.d_vrs, vrs, sizeof(double) * c.nnr, hipMemcpyDefault));
+  c.have_pot = true;
+  return 0;
+  WH_API_END
+}
+
+static std::vector<z64> to_sticks(const double* a, int ncol, int npwx) {
+  Context& c = ctx();
+  std::vector<z64> host(static_cast<size_t>(ncol) * npwx);
+  if (ncol) WH_HIP(hipMemcpy(host.data(), a, sizeof(z64) * host.size(), hipMemcpyDefault));
+  std::vector<z64> out(static_cast<size_t>(ncol) * c.npw);
+  for (int j = 0; j < ncol; ++j)
+    for (int s = 0; s < c.npw; ++s) out[static_cast<size_t>(j) * c.npw + s] = host[static_cast<size_t>(j) * npwx + c.plan.perm[s]];
+  return out;
+}
+
+int wh_set_nonlocal(int nkb, const double* vkb, const double* dvan) {
+  WH_API_BEGIN
+  Context& c = ctx();
+  WH_REQUIRE(c.have_grid, "wh_set_grid first");
+  WH_REQUIRE(nkb >= 0, "nkb must be >= 0");
+  c.nkb = nkb;
+  c.vkb_s = to_sticks(vkb, nkb, c.npwx);
+  // a real projector has a real G = 0 coefficient; the Gamma algebra relies on it
+  for (int j = 0; j < nkb; ++j) c.vkb_s[static_cast<size_t>(j) * c.npw].y = 0.0;
+  c.dvan.assign(nkb, 0.0);
+  if (nkb) WH_HIP(hipMemcpy(c.dvan.data(), dvan, sizeof(double) * nkb, hipMemcpyDefault));
+  rebuild_q();
+  return 0;
+  WH_API_END
+}
+
+int wh_set_states(int nbndval, const double* evc, const double* et, const double* occ, double weight,
+                  double alpha_pv) {
+  WH_API_BEGIN
+  Context& c = ctx();
+  WH_REQUIRE(c.have_grid, "wh_set_grid first");
+  WH_REQUIRE(nbndval > 0, "nbndval must be positive");
+  c.nbndval = nbndval;
+  c.evc_s = to_sticks(evc, nbndval, c.npwx);
+  for (int j = 0; j < nbndval; ++j) c.evc_s[static_cast<size_t>(j) * c.npw].y = 0.0;
+  c.et.assign(nbndval, 0.0);
+  c.occ.assign(nbndval, 1.0);
+  WH_HIP(hipMemcpy(c.et.data(), et, sizeof(double) * nbndval, hipMemcpyDefault));
+  if (occ) WH_HIP(hipMemcpy(c.occ.data(), occ, sizeof(double) * nbndval, hipMemcpyDefault));
+  c.weight = weight;
+  c.alpha_pv = alpha_pv;
+  rebuild_q();
+  if (c.d_occ) WH_HIP(hipFree(c.d_occ));
+  if (c.d_eprec) WH_HIP(hipFree(c.d_eprec));
+  WH_HIP(hipMalloc(&c.d_occ, sizeof(double) * nbndval));
+  WH_HIP(hipMalloc(&c.d_eprec, sizeof(double) * nbndval));
+  WH_HIP(hipMemcpy(c.d_occ, c.occ.data(), sizeof(double) * nbndval, hipMemcpyHostToDevice));
+  eprec_of(nbndval, c.npw, c.d_q, c.npw, c.d_g2kin, c.d_eprec, c.stream);   // set_eprec (dfpt_module.f90:211)
+  c.eprec.assign(nbndval, 0.0);
+  WH_HIP(hipMemcpyAsync(c.eprec.data(), c.d_eprec, sizeof(double) * nbndval, hipMemcpyDeviceToHost, c.stream));
+  WH_HIP(hipStreamSynchronize(c.stream));
+  c.have_states = true;
+  c.evc_sticks_valid = false;
+  return 0;
+  WH_API_END
+}
+
+int wh_set_coulomb(const double* sqvc) {
+  WH_API_BEGIN
+  Context& c = ctx();
+  WH_REQUIRE(c.have_grid, "wh_set_grid first");
+  std::vector<double> h(c.npwx);
+  WH_HIP(hipMemcpy(h.data(), sqvc, sizeof(double) * c.npwx, hipMemcpyDefault));
+  c.sqvc_s.resize(c.npw);
+  for (int s = 0; s < c.npw; ++s) c.sqvc_s[s] = h[c.plan.perm[s]];
+  if (c.d_sqvc) WH_HIP(hipFree(c.d_sqvc));
+  WH_HIP(hipMalloc(&c.d_sqvc, sizeof(double) * c.npw));
+  WH_HIP(hipMemcpy(c.d_sqvc, c.sqvc_s.data(), sizeof(double) * c.npw, hipMemcpyHostToDevice));
+  c.have_sqvc = true;
+  return 0;
+  WH_API_END
+}
+
+int wh_set_params(int n_dfpt_maxiter, int l_kinetic_only) {
+  WH_API_BEGIN
+  Context& c = ctx();
+  WH_REQUIRE(n_dfpt_maxiter >= 1, "n_dfpt_maxiter must be >= 1");
+  c.n_dfpt_maxiter = n_dfpt_maxiter;
+  c.l_kinetic_only = l_kinetic_only ? 1 : 0;
+  return 0;
+  WH_API_END
+}
+
+int wh_apply_sternheimerop_to_m_wfcs(int nbndval, const double* psi, double* hpsi, const double* e,
+                                     const double* alpha, int m) {
+  WH_API_BEGIN
+  Context& c = ctx();
+  // without registered states (nbndval = 0) this is H - e, used to find the states themselves
+  require_ready(false);
+  WH_REQUIRE(nbndval == c.nbndval, "nbndval differs from the registered states");
+  WH_REQUIRE(m >= 0, "m must be >= 0");
+  if (m == 0) return 0;
+  hipStream_t s = c.stream;
+  double a[2] = {0.0, 0.0};
+  WH_HIP(hipMemcpy(a, alpha, sizeof(double) * 2, hipMemcpyDefault));
+  const size_t cin = static_cast<size_t>(m) * c.npwx;
+  const z64* pin = dev_in(reinterpret_cast<const z64*>(psi), cin, c.w_in);
+  const double* pe = dev_in(e, m, c.w_ecol);
+  z64* x = c.w_x.as<z64>(static_cast<size_t>(m) * c.npw);
+  z64* t = c.w_t.as<z64>(static_cast<size_t>(m) * c.npw);
+  gather_sticks(m, c.npw, c.d_perm, pin, c.npwx, x, c.npw, nullptr, s);
+  stern_apply(x, t, pe, m, a[0]);
+  const bool dout = on_device(hpsi);
+  z64* o = dout ? reinterpret_cast<z64*>(hpsi) : c.w_out.as<z64>(cin);
+  scatter_caller(m, c.npw, c.npwx, c.d_perm, t, c.npw, o, c.npwx, s);
+  if (!dout) WH_HIP(hipMemcpyAsync(hpsi, o, sizeof(z64) * cin, hipMemcpyDeviceToHost, s));
+  finish_call();
+  return 0;
+  WH_API_END
+}
+
+int wh_linsolve_sternheimer_m_wfcts(int nbndval, int m, double* b, double* x, const double* e, const double* eprec,
+                                    double tr2, int* ierr) {
+  WH_API_BEGIN
+  Context& c = ctx();
+  require_ready(true);
+  WH_REQUIRE(nbndval == c.nbndval, "nbndval differs from the registered states");
+  WH_REQUIRE(m >= 0, "m must be >= 0");
+  if (ierr) *ierr = 0;
+  if (m == 0) return 0;
+  hipStream_t s = c.stream;
+  const size_t cin = static_cast<size_t>(m) * c.npwx;
+  std::vector<double> e_col(m), ep(m);
+  WH_HIP(hipMemcpy(e_col.data(), e, sizeof(double) * m, hipMemcpyDefault));
+  WH_HIP(hipMemcpy(ep.data(), eprec, sizeof(double) * m, hipMemcpyDefault));
+  double* de = c.w_ecol.as<double>(m);
+  double* dep = c.w_epcol.as<double>(m);
+  WH_HIP(hipMemcpyAsync(de, e_col.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
+  WH_HIP(hipMemcpyAsync(dep, ep.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
+  z64* db = c.w_b.as<z64>(static_cast<size_t>(m) * c.npw);
+  z64* dx = c.w_x.as<z64>(static_cast<size_t>(m) * c.npw);
+  const z64* bin = dev_in(reinterpret_cast<const z64*>(b), cin, c.w_in);
+  gather_sticks(m, c.npw, c.d_perm, bin, c.npwx, db, c.npw, nullptr, s);
+  const z64* xin = dev_in(reinterpret_cast<const z64*>(x), cin, c.w_out);
+  gather_sticks(m, c.npw, c.d_perm, xin, c.npwx, dx, c.npw, nullptr, s);
+  const int nc = linsolve_dev(m, db, dx, e_col, de, dep, tr2, nullptr);
+  const bool dout = on_device(x);
+  z64* o = dout ? reinterpret_cast<z64*>(x) : c.w_out.as<z64>(cin);
+  scatter_caller(m, c.npw, c.npwx, c.d_perm, dx, c.npw, o, c.npwx, s);
+  if (!dout) WH_HIP(hipMemcpyAsync(x, o, sizeof(z64) * cin, hipMemcpyDeviceToHost, s));
+  finish_call();
+  if (ierr) *ierr = nc;
+  return nc;
+  WH_API_END
+}
+
+static int dfpt_entry(int m, const double* dvg, double* dng, double tr2, int iq, int* ierr_pert, bool coulomb) {
+  Context& c = ctx();
+  require_ready(true);
+  WH_REQUIRE(iq == 1, "only the Gamma point (iq = 1) is supported");
+  WH_REQUIRE(m >= 0, "m must be >= 0");
+  if (coulomb) WH_REQUIRE(c.have_sqvc, "wh_set_coulomb has not been called");
+  hipStream_t s = c.stream;
+  const size_t cin = static_cast<size_t>(m) * c.npwx;
+  if (m == 0) return 0;
+  const z64* din = dev_in(reinterpret_cast<const z64*>(dvg), cin, c.w_in);
+  z64* dvk = c.w_dvin.as<z64>(static_cast<size_t>(m) * c.npw);
+  gather_sticks(m, c.npw, c.d_perm, din, c.npwx, dvk, c.npw, coulomb ? c.d_sqvc : nullptr, s);
+  z64* dn = c.w_dng.as<z64>(static_cast<size_t>(m) * c.npw);
+  std::vector<int> ierr;
+  const int nonconv = dfpt_dev(m, dvk, dn, tr2, coulomb ? c.d_sqvc : nullptr, &ierr);
+  const bool dout = on_device(dng);
+  z64* o = dout ? reinterpret_cast<z64*>(dng) : c.w_out.as<z64>(cin);
+  scatter_caller(m, c.npw, c.npwx, c.d_perm, dn, c.npw, o, c.npwx, s);
+  if (!dout) WH_HIP(hipMemcpyAsync(dng, o, sizeof(z64) * cin, hipMemcpyDeviceToHost, s));
+  finish_call();
+  if (ierr_pert) std::copy(ierr.begin(), ierr.end(), ierr_pert);
+  return nonconv;
+}
+
+int wh_dfpt(int m, const double* dvg, double* dng, double tr2, int iq, int* ierr_per_pert) {
+  WH_API_BEGIN
+  return dfpt_entry(m, dvg, dng, tr2, iq, ierr_per_pert, false);
+  WH_API_END
+}
+
+int wh_apply_operator(int m, const double* dvg, double* dng, double tr2, int iq) {
+  WH_API_BEGIN
+  return dfpt_entry(m, dvg, dng, tr2, iq, nullptr, true);
+  WH_API_END
+}
+
+int wh_get_counters(long* stern_ncalls, long* cg_iters) {
+  WH_API_BEGIN
+  Context& c = ctx();
+  if (stern_ncalls) *stern_ncalls = c.nstern;
+  if (cg_iters) *cg_iters = c.cg_iters;
+  return 0;
+  WH_API_END
+}
+
+int wh_profile(int enable) {
+  WH_API_BEGIN
+  Context& c = ctx();
+  WH_HIP(hipStreamSynchronize(c.stream));
+  c.prof.clear();
+  c.prof.on = enable != 0;
+  return 0;
+  WH_API_END
+}
+
+int wh_profile_query(const char* name, double* ms, long* count) {
+  WH_API_BEGIN
+  Context& c = ctx();
+  c.prof.flush();
+  auto it = c.prof.acc.find(name ? name : "");
+  if (ms) *ms = it == c.prof.acc.end() ? 0.0 : it->second.first;
+  if (count) *count = it == c.prof.acc.end() ? 0 : it->second.second;
+  return 0;
+  WH_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,83 @@
+// Common definitions for the MI355X Wstat library (gfx950, HIP).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+namespace wh {
+
+// ---------------------------------------------------------------------------
+// error handling: every entry point returns 0 / >0 (warning count) / <0 and
+// records a message retrievable through wh_last_error().
+// ---------------------------------------------------------------------------
+struct Error {
+  int code;
+  std::string msg;
+};
+
+void set_error(const std::string& msg);
+const char* last_error();
+
+#define WH_HIP(expr)                                                                \
+  do {                                                                              \
+    hipError_t e_ = (expr);                                                         \
+    if (e_ != hipSuccess) {                                                         \
+      throw ::wh::Error{-2, std::string(#expr) + ": " + hipGetErrorString(e_)};     \
+    }                                                                               \
+  } while (0)
+
+#define WH_REQUIRE(cond, text)                                                      \
+  do {                                                                              \
+    if (!(cond)) throw ::wh::Error{-1, std::string(text)};                          \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// complex arithmetic on plain structs (interleaved re/im, like Fortran COMPLEX)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct alignas(2 * sizeof(T)) cplx {
+  T x, y;
+};
+
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> mk(T a, T b) {
+  cplx<T> r;
+  r.x = a;
+  r.y = b;
+  return r;
+}
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> operator+(cplx<T> a, cplx<T> b) {
+  return mk<T>(a.x + b.x, a.y + b.y);
+}
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> operator-(cplx<T> a, cplx<T> b) {
+  return mk<T>(a.x - b.x, a.y - b.y);
+}
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+  return mk<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> cscale(cplx<T> a, T s) {
+  return mk<T>(a.x * s, a.y * s);
+}
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> conjg(cplx<T> a) {
+  return mk<T>(a.x, -a.y);
+}
+// multiply by +i (dir = +1) or -i (dir = -1)
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> mul_i(cplx<T> a, int dir) {
+  return dir > 0 ? mk<T>(-a.y, a.x) : mk<T>(a.y, -a.x);
+}
+
+using z64 = cplx<double>;
+
+inline int ceil_div(long a, long b) { return static_cast<int>((a + b - 1) / b); }
+
+}  // namespace wh

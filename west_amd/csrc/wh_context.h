@@ -1,0 +1,131 @@
+// Library state: the ground state registered through the C ABI, device
+// workspaces, the stream, counters and the optional per-kernel timer.
+#pragma once
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "wh_common.h"
+#include "wh_fft.h"
+
+namespace wh {
+
+// Growable device buffer (allocated once, reused across calls).
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void* get(size_t n) {
+    if (n > bytes) {
+      // work queued on the device may still use the old block
+      if (p) {
+        WH_HIP(hipDeviceSynchronize());
+        WH_HIP(hipFree(p));
+      }
+      p = nullptr;
+      bytes = 0;
+      WH_HIP(hipMalloc(&p, n));
+      bytes = n;
+    }
+    return p;
+  }
+  template <typename T>
+  T* as(size_t count) {
+    return static_cast<T*>(get(sizeof(T) * std::max<size_t>(count, 1)));
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+struct Profiler {
+  struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  std::vector<Pending> pending;
+  std::map<std::string, std::pair<double, long>> acc;
+  hipEvent_t take();
+  void flush();
+  void clear();
+};
+
+struct Context {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  // grid / sphere
+  bool have_grid = false;
+  int nr1 = 0, nr2 = 0, nr3 = 0, nnr = 0, npw = 0, npwx = 0, gstart = 2;
+  double omega = 0.0, tpiba2 = 0.0;
+  StickPlan plan;
+  int* d_perm = nullptr;
+  double* d_g2kin = nullptr;
+  std::vector<double> g2kin_s;   // stick order (host)
+  // potential
+  bool have_pot = false;
+  double* d_vrs = nullptr;
+  // states and projectors (stick order, rows of npw complex)
+  bool have_states = false;
+  int nbndval = 0, nkb = 0;
+  std::vector<z64> evc_s, vkb_s;
+  std::vector<double> et, occ, dvan, eprec;
+  double weight = 2.0, alpha_pv = 0.0;
+  z64* d_q = nullptr;            // [evc ; vkb], (nbndval + nkb) rows
+  int q_rows = 0;
+  double* d_qscale = nullptr;    // per Q row: alpha (evc rows) or dvan (vkb rows)
+  double qscale_alpha = 0.0;     // alpha currently stored in d_qscale
+  double* d_occ = nullptr;
+  double* d_eprec = nullptr;
+  bool have_sqvc = false;
+  double* d_sqvc = nullptr;      // stick order
+  std::vector<double> sqvc_s;
+  // precomputed sticks of the occupied pairs (dfpt right-hand side)
+  DevBuf evc_sticks;
+  bool evc_sticks_valid = false;
+  // params
+  int n_dfpt_maxiter = 250;
+  int l_kinetic_only = 0;
+  size_t sticks_budget = size_t(4) << 30;   // bytes for transform intermediates
+  // workspaces
+  DevBuf w_items, w_items2, w_items_id, w_sticks_a, w_sticks_b, w_cmat;
+  DevBuf w_x, w_t, w_g, w_h, w_hold, w_bc, w_tc, w_b, w_dvr, w_in, w_out, w_dvin, w_dng;
+  DevBuf w_l2i, w_beta, w_eu, w_rho, w_ecol, w_epcol;
+  // counters
+  long nstern = 0;
+  long cg_iters = 0;
+  Profiler prof;
+
+  long sticks_bytes() const { return plan.sticks_elems() * static_cast<long>(sizeof(z64)); }
+  int items_capacity() const;   // transform items per launch chunk
+};
+
+Context& ctx();
+void require_ready(bool need_states);
+
+// timed launch: the statement runs between two events when profiling is on
+#define WH_TIMED(NAME, STMT)                                                       \
+  do {                                                                             \
+    ::wh::Context& c_ = ::wh::ctx();                                               \
+    if (c_.prof.on) {                                                              \
+      hipEvent_t a_ = c_.prof.take(), b_ = c_.prof.take();                         \
+      WH_HIP(hipEventRecord(a_, c_.stream));                                       \
+      STMT;                                                                        \
+      WH_HIP(hipEventRecord(b_, c_.stream));                                       \
+      c_.prof.pending.push_back({NAME, a_, b_});                                   \
+    } else {                                                                       \
+      STMT;                                                                        \
+    }                                                                              \
+  } while (0)
+
+// operations on device data in stick order (rows of npw complex, ld = npw)
+void stern_apply(const z64* x, z64* t, const double* d_e, int m, double alpha);
+int linsolve_dev(int m, z64* b, z64* x, const std::vector<double>& e_col, const double* d_e_col,
+                 const double* d_eprec_col, double tr2, std::vector<int>* ierr_col);
+int dfpt_dev(int m, const z64* dvg, z64* dng, double tr2, const double* out_weight, std::vector<int>* ierr_pert);
+void glbrak_q(int row0, int nq, const z64* f, int m, double* cmat);   // cmat (nq x m) = <Q_row|f_col>
+
+}  // namespace wh

@@ -1,0 +1,371 @@
+// Wstat Davidson driver at Gamma on the device (FP64).
+//
+// Follows Wstat/davidson_diago.f90:31-437 (davidson_diago_gamma) with the real
+// variants of Wstat/wstat_tools.f90 (build_hr 345-468, serial_diagox_dsy
+// 208-273, redistribute_vr_distr 591-765, update_with_vr_distr 945-1092,
+// refresh_with_vr_distr 1245-1395) on one process.  The basis dvg and its
+// images dng stay resident on the device in stick order; the subspace
+// products are matrix-core GEMMs; the dense eigensolve (matdiago_dsy,
+// Tools/linear_algebra_kernel.f90:84-181) is rocSOLVER dsyevd in FP64.
+//
+// Orthonormalisation: do_mgs (davidson_diago.f90:909-1117) projects each old
+// basis vector out of the new ones, one at a time, then runs Gram-Schmidt
+// inside the new block.  Here the old-basis part is two classical
+// Gram-Schmidt passes done as GEMMs (the same span, reorthogonalised), and
+// the in-block part is the same sequential modified Gram-Schmidt.
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "../../include/west_hip.h"
+#include "wh_context.h"
+#include "wh_gemm.h"
+#include "wh_vec.h"
+
+namespace wh {
+
+namespace {
+
+constexpr int NT = 256;
+constexpr unsigned long long kRngSeed = 0x5EED0F1E57ULL;
+
+__device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
+  x += 0x9E3779B97F4A7C15ULL;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+  return x ^ (x >> 31);
+}
+
+// do_randomize (davidson_diago.f90:1156-1177): vector ivec (1-based), plane
+// wave ig (1-based, caller order): amplitude u0, phase 2 pi u1, divided by
+// |G|^2 + 1 in (2 pi / alat)^2 units; G = 0 left at zero.
+__global__ void k_randomize(int npw, int ivec0, int nvec, const int* perm, const double* g2kin, double inv_tpiba2,
+                            z64* amat, long ld) {
+  const int r = blockIdx.y;
+  const int s = blockIdx.x * NT + threadIdx.x;
+  if (s >= npw || r >= nvec) return;
+  const unsigned long long ivec = static_cast<unsigned long long>(ivec0 + r);
+  const unsigned long long ig = static_cast<unsigned long long>(perm[s]) + 1ULL;
+  const unsigned long long base = kRngSeed ^ (ivec << 32);
+  const double u0 = static_cast<double>(mix64(base ^ (2ULL * ig)) >> 11) * (1.0 / 9007199254740992.0);
+  const double u1 = static_cast<double>(mix64(base ^ (2ULL * ig + 1ULL)) >> 11) * (1.0 / 9007199254740992.0);
+  z64 v = mk(0.0, 0.0);
+  if (s != 0) {
+    const double arg = 2.0 * 3.14159265358979323846 * u1;
+    const double den = g2kin[s] * inv_tpiba2 + 1.0;
+    v = mk(u0 * cos(arg) / den, u0 * sin(arg) / den);
+  }
+  amat[static_cast<long>(r) * ld + s] = v;
+}
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int i = 0; i < NT / 64; ++i) s += red[i];
+  return s;
+}
+
+// v <- v / sqrt(<v|v>)   (davidson_diago.f90:1005-1025, Gamma norm)
+__global__ __launch_bounds__(NT) void k_normalize(int npw, int gstart, z64* v) {
+  __shared__ double red[NT / 64];
+  double s = 0.0;
+  for (int q = threadIdx.x; q < npw; q += NT) s += v[q].x * v[q].x + v[q].y * v[q].y;
+  s = block_sum(s, red);
+  double an = 2.0 * s;
+  if (gstart == 2) an -= v[0].x * v[0].x;
+  __syncthreads();
+  const double f = 1.0 / sqrt(an);
+  for (int q = threadIdx.x; q < npw; q += NT) v[q] = cscale(v[q], f);
+}
+
+__global__ void k_scale_rows(int m, int npw, const double* w, const z64* in, z64* out, long ld) {
+  const int r = blockIdx.y;
+  const int s = blockIdx.x * NT + threadIdx.x;
+  if (s >= npw || r >= m) return;
+  out[static_cast<long>(r) * ld + s] = cscale(in[static_cast<long>(r) * ld + s], w[s]);
+}
+
+struct Solver {
+  rocblas_handle h = nullptr;
+  ~Solver() {
+    if (h) rocblas_destroy_handle(h);
+  }
+};
+
+Solver& solver() {
+  static Solver s;
+  if (!s.h) {
+    WH_REQUIRE(rocblas_create_handle(&s.h) == rocblas_status_success, "rocblas_create_handle failed");
+  }
+  WH_REQUIRE(rocblas_set_stream(s.h, ctx().stream) == rocblas_status_success, "rocblas_set_stream failed");
+  return s;
+}
+
+// glbrak of rows: C (na x nb, col major ldc) = <A_i|B_j>
+void glbrak_rows(const z64* A, int na, const z64* B, int nb, double* Cm, int ldc) {
+  Context& c = ctx();
+  const long k2 = 2L * c.npw;
+  const double* ar = reinterpret_cast<const double*>(A);
+  const double* br = reinterpret_cast<const double*>(B);
+  WH_TIMED("gemm_bra", dgemm(true, na, nb, static_cast<int>(k2), 2.0, ar, k2, br, k2, 0.0, Cm, ldc, c.stream));
+  WH_TIMED("gemm_fix", g0_fix_scale(na, nb, c.gstart == 2 ? 1.0 : 0.0, ar, k2, br, k2, nullptr, Cm, ldc, c.stream));
+}
+
+// Y (nb rows) = alpha * X (na rows)^T-combination: Y_j = alpha sum_i W(i,j) X_i + beta Y_j
+void combine_rows(const z64* X, int na, const double* W, int ldw, z64* Y, int nb, double alpha, double beta) {
+  Context& c = ctx();
+  const long k2 = 2L * c.npw;
+  WH_TIMED("gemm_ket", dgemm(false, static_cast<int>(k2), nb, na, alpha, reinterpret_cast<const double*>(X), k2, W,
+                             ldw, beta, reinterpret_cast<double*>(Y), k2, c.stream));
+}
+
+// orthonormalise rows [ms, me) of amat against [0, ms) and among themselves
+void do_mgs(z64* amat, int ms, int me) {
+  Context& c = ctx();
+  hipStream_t s = c.stream;
+  const int npw = c.npw;
+  const int nnew = me - ms;
+  if (nnew <= 0) return;
+  if (ms > 0) {
+    double* cm = c.w_cmat.as<double>(static_cast<size_t>(ms) * nnew);
+    for (int pass = 0; pass < 2; ++pass) {
+      glbrak_rows(amat, ms, amat + static_cast<long>(ms) * npw, nnew, cm, ms);
+      combine_rows(amat, ms, cm, ms, amat + static_cast<long>(ms) * npw, nnew, -1.0, 1.0);
+    }
+  }
+  double* cm = c.w_cmat.as<double>(static_cast<size_t>(std::max(nnew, 1)));
+  for (int k = ms; k < me; ++k) {
+    z64* vk = amat + static_cast<long>(k) * npw;
+    k_normalize<<<1, NT, 0, s>>>(npw, c.gstart, vk);
+    WH_HIP(hipGetLastError());
+    const int nr = me - k - 1;
+    if (nr <= 0) continue;
+    glbrak_rows(vk, 1, vk + npw, nr, cm, 1);
+    combine_rows(vk, 1, cm, 1, vk + npw, nr, -1.0, 1.0);
+  }
+}
+
+double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+}  // namespace wh
+
+using namespace wh;
+
+extern "C" int wh_davidson_diago(const wh_davidson_params* prm, double* ev_out, int* conv_out, double* dvg_out,
+                                 wh_davidson_step* log, int log_capacity, int* log_len) {
+  try {
+    Context& c = ctx();
+    require_ready(true);
+    WH_REQUIRE(prm != nullptr, "params required");
+    WH_REQUIRE(c.have_sqvc, "wh_set_coulomb has not been called");
+    hipStream_t s = c.stream;
+    const int nvec = prm->n_pdep_eigen;
+    const int nvecx = prm->n_pdep_eigen * prm->n_pdep_times;
+    WH_REQUIRE(nvec > 0, "n_pdep_eigen must be positive");
+    WH_REQUIRE(nvec <= nvecx / 2, "nvecx is too small");   // davidson_diago.f90:108
+    const int npw = c.npw;
+    const long ld = npw;
+    if (log_len) *log_len = 0;
+    // device basis
+    static DevBuf b_dvg, b_dng, b_tmp, b_hr, b_w, b_e, b_info, b_vr, b_sq;
+    z64* dvg = b_dvg.as<z64>(static_cast<size_t>(nvecx) * npw);
+    z64* dng = b_dng.as<z64>(static_cast<size_t>(nvecx) * npw);
+    z64* tmp = b_tmp.as<z64>(static_cast<size_t>(nvecx) * npw);
+    WH_HIP(hipMemsetAsync(dvg, 0, sizeof(z64) * nvecx * npw, s));
+    WH_HIP(hipMemsetAsync(dng, 0, sizeof(z64) * nvecx * npw, s));
+    std::vector<double> hr(static_cast<size_t>(nvecx) * nvecx, 0.0), vr(static_cast<size_t>(nvecx) * nvecx, 0.0);
+    std::vector<double> ew(nvecx, 0.0), ev(nvec, 0.0);
+    std::vector<char> conv(nvec, 0);
+    int nbase = nvec, notcnv = nvec;
+    const double t0 = now_s();
+    double tit = t0;
+    long ns0 = c.nstern;
+    int nlog = 0;
+
+    // chi~ applied to rows [r0, r0+n): dng = sqvc chi0 (sqvc dvg)
+    auto apply_op = [&](int r0, int n, double tr2) {
+      z64* in = tmp;   // sqvc * dvg rows
+      k_scale_rows<<<dim3(ceil_div(npw, NT), n), NT, 0, s>>>(n, npw, c.d_sqvc, dvg + static_cast<long>(r0) * ld, in, ld);
+      WH_HIP(hipGetLastError());
+      dfpt_dev(n, in, dng + static_cast<long>(r0) * ld, tr2, c.d_sqvc, nullptr);
+    };
+    // hr(0:g_e, cols) = <dvg_i|dng_j>, rows beyond g_e zeroed
+    auto build_hr = [&](int l2s, int l2e, int g_e) {
+      const int nb = l2e - l2s;
+      double* d = b_hr.as<double>(static_cast<size_t>(g_e) * nb);
+      glbrak_rows(dvg, g_e, dng + static_cast<long>(l2s) * ld, nb, d, g_e);
+      std::vector<double> blk(static_cast<size_t>(g_e) * nb);
+      WH_HIP(hipMemcpyAsync(blk.data(), d, sizeof(double) * blk.size(), hipMemcpyDeviceToHost, s));
+      WH_HIP(hipStreamSynchronize(s));
+      for (int j = 0; j < nb; ++j) {
+        double* col = &hr[static_cast<size_t>(l2s + j) * nvecx];
+        std::fill(col, col + nvecx, 0.0);
+        std::memcpy(col, &blk[static_cast<size_t>(j) * g_e], sizeof(double) * g_e);
+      }
+    };
+    // serial_diagox_dsy: eigenpairs of hr(0:n,0:n) upper; nvec lowest kept
+    auto diagox = [&](int n) {
+      double* A = b_hr.as<double>(static_cast<size_t>(n) * n);
+      double* D = b_w.as<double>(n);
+      double* E = b_e.as<double>(n);
+      rocblas_int* info = b_info.as<rocblas_int>(1);
+      std::vector<double> a(static_cast<size_t>(n) * n);
+      for (int j = 0; j < n; ++j) std::memcpy(&a[static_cast<size_t>(j) * n], &hr[static_cast<size_t>(j) * nvecx], sizeof(double) * n);
+      WH_HIP(hipMemcpyAsync(A, a.data(), sizeof(double) * a.size(), hipMemcpyHostToDevice, s));
+      WH_REQUIRE(rocsolver_dsyevd(solver().h, rocblas_evect_original, rocblas_fill_upper, n, A, n, D, E, info) ==
+                     rocblas_status_success, "rocsolver_dsyevd failed");
+      std::vector<double> w(n);
+      rocblas_int hinfo = 0;
+      WH_HIP(hipMemcpyAsync(a.data(), A, sizeof(double) * a.size(), hipMemcpyDeviceToHost, s));
+      WH_HIP(hipMemcpyAsync(w.data(), D, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+      WH_HIP(hipMemcpyAsync(&hinfo, info, sizeof(rocblas_int), hipMemcpyDeviceToHost, s));
+      WH_HIP(hipStreamSynchronize(s));
+      WH_REQUIRE(hinfo == 0, "dsyevd did not converge");
+      for (int j = 0; j < nvec; ++j) {
+        ew[j] = w[j];
+        std::memcpy(&vr[static_cast<size_t>(j) * nvecx], &a[static_cast<size_t>(j) * n], sizeof(double) * n);
+      }
+    };
+    auto record = [&](int dav_iter, double tr2, int dfpt_dim, int diago_dim) {
+      if (log && nlog < log_capacity) {
+        wh_davidson_step& r = log[nlog];
+        r.dav_iter = dav_iter;
+        r.notcnv = notcnv;
+        r.dfpt_dim = dfpt_dim;
+        r.diago_dim = diago_dim;
+        r.sternop_ncalls = c.nstern - ns0;
+        r.dfpt_tr2 = tr2;
+        r.time_elap = now_s() - t0;
+        r.time_iter = now_s() - tit;
+      }
+      ++nlog;
+    };
+    // rows [0, ncol) of X <- X * VR(0:n, 0:ncol) (refresh); uses tmp
+    auto refresh = [&](z64* X, int n, int ncol) {
+      double* W = b_vr.as<double>(static_cast<size_t>(n) * ncol);
+      std::vector<double> w(static_cast<size_t>(n) * ncol);
+      for (int j = 0; j < ncol; ++j) std::memcpy(&w[static_cast<size_t>(j) * n], &vr[static_cast<size_t>(j) * nvecx], sizeof(double) * n);
+      WH_HIP(hipMemcpyAsync(W, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice, s));
+      combine_rows(X, n, W, n, tmp, ncol, 1.0, 0.0);
+      WH_HIP(hipMemcpyAsync(X, tmp, sizeof(z64) * ncol * npw, hipMemcpyDeviceToDevice, s));
+      if (nvecx > ncol) WH_HIP(hipMemsetAsync(X + static_cast<long>(ncol) * ld, 0, sizeof(z64) * (nvecx - ncol) * npw, s));
+      WH_HIP(hipStreamSynchronize(s));
+    };
+
+    // ---- from scratch (davidson_diago.f90:167-196)
+    k_randomize<<<dim3(ceil_div(npw, NT), nvec), NT, 0, s>>>(npw, 1, nvec, c.d_perm, c.d_g2kin, 1.0 / c.tpiba2, dvg, ld);
+    WH_HIP(hipGetLastError());
+    do_mgs(dvg, 0, nvec);
+    apply_op(0, nvec, -1.0);
+    // ---- extra step (205-249)
+    tit = now_s();
+    ns0 = c.nstern;
+    WH_HIP(hipMemcpyAsync(dvg, dng, sizeof(z64) * nvec * npw, hipMemcpyDeviceToDevice, s));
+    do_mgs(dvg, 0, nvec);
+    double tr2 = std::min(1.0e-4, 1.0e8 * prm->tr2_dfpt);
+    apply_op(0, nvec, tr2);
+    build_hr(0, nvec, nvec);
+    diagox(nbase);
+    for (int i = 0; i < nvec; ++i) ev[i] = ew[i];
+    record(0, tr2, nbase, nbase);
+    int dav_iter = 0;
+    bool converged = false;
+    for (int kter = 1; kter <= prm->n_pdep_maxiter; ++kter) {   // (259-424)
+      tit = now_s();
+      ns0 = c.nstern;
+      ++dav_iter;
+      const int dfpt_dim = notcnv;
+      const int diago_dim = nbase + notcnv;
+      int np = 0;
+      for (int n = 0; n < nvec; ++n) {
+        if (!conv[n]) {
+          std::memcpy(&vr[static_cast<size_t>(nbase + np) * nvecx], &vr[static_cast<size_t>(n) * nvecx], sizeof(double) * nvecx);
+          ew[nbase + np] = ev[n];
+          ++np;
+        }
+      }
+      // update_with_vr_distr: new_j = sum_i vr(i,j) (dng_i - ew_j dvg_i)
+      {
+        double* W = b_vr.as<double>(static_cast<size_t>(nbase) * 2 * notcnv);
+        std::vector<double> w(static_cast<size_t>(nbase) * 2 * notcnv);
+        for (int j = 0; j < notcnv; ++j)
+          for (int i = 0; i < nbase; ++i) {
+            const double v = vr[static_cast<size_t>(nbase + j) * nvecx + i];
+            w[static_cast<size_t>(j) * nbase + i] = v;
+            w[static_cast<size_t>(notcnv + j) * nbase + i] = -ew[nbase + j] * v;
+          }
+        WH_HIP(hipMemcpyAsync(W, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice, s));
+        z64* newv = dvg + static_cast<long>(nbase) * ld;
+        combine_rows(dng, nbase, W, nbase, newv, notcnv, 1.0, 0.0);
+        combine_rows(dvg, nbase, W + static_cast<size_t>(notcnv) * nbase, nbase, newv, notcnv, 1.0, 1.0);
+        WH_HIP(hipStreamSynchronize(s));
+      }
+      do_mgs(dvg, nbase, nbase + notcnv);
+      tr2 = prm->tr2_dfpt;
+      apply_op(nbase, notcnv, tr2);
+      build_hr(nbase, nbase + notcnv, nbase + notcnv);
+      nbase += notcnv;
+      diagox(nbase);
+      notcnv = 0;
+      for (int n = 0; n < nvec; ++n) {
+        conv[n] = (std::fabs((ew[n] - ev[n]) / ev[n]) < prm->trev_pdep_rel) && (std::fabs(ew[n] - ev[n]) < prm->trev_pdep);
+        if (!conv[n]) ++notcnv;
+      }
+      for (int n = 0; n < nvec; ++n) ev[n] = ew[n];
+      record(dav_iter, tr2, dfpt_dim, diago_dim);
+      if (notcnv == 0 || nbase + notcnv > nvecx || kter == prm->n_pdep_maxiter) {
+        if (notcnv == 0) {
+          refresh(dvg, nbase, nvec);
+          converged = true;
+          break;
+        }
+        if (kter == prm->n_pdep_maxiter) break;
+        refresh(dvg, nbase, nvec);
+        refresh(dng, nbase, nvec);
+        nbase = nvec;
+        std::fill(hr.begin(), hr.end(), 0.0);
+        std::fill(vr.begin(), vr.end(), 0.0);
+        for (int i = 0; i < nbase; ++i) {
+          hr[static_cast<size_t>(i) * nvecx + i] = ev[i];
+          vr[static_cast<size_t>(i) * nvecx + i] = 1.0;
+        }
+      }
+    }
+    (void)converged;
+    std::memcpy(ev_out, ev.data(), sizeof(double) * nvec);
+    if (conv_out)
+      for (int n = 0; n < nvec; ++n) conv_out[n] = conv[n] ? 1 : 0;
+    if (dvg_out) {
+      const size_t cin = static_cast<size_t>(nvec) * c.npwx;
+      std::vector<z64> h(cin);
+      z64* o = c.w_out.as<z64>(cin);
+      scatter_caller(nvec, npw, c.npwx, c.d_perm, dvg, ld, o, c.npwx, s);
+      WH_HIP(hipMemcpyAsync(h.data(), o, sizeof(z64) * cin, hipMemcpyDeviceToHost, s));
+      WH_HIP(hipStreamSynchronize(s));
+      WH_HIP(hipMemcpy(dvg_out, h.data(), sizeof(z64) * cin, hipMemcpyDefault));
+    }
+    WH_HIP(hipStreamSynchronize(s));
+    if (c.prof.on) c.prof.flush();
+    if (log_len) *log_len = std::min(nlog, log_capacity);
+    return notcnv;
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return e.code < 0 ? e.code : -1;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return -3;
+  }
+}

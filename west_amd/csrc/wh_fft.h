@@ -1,0 +1,101 @@
+// Gamma-point 3D FFTs on the G-sphere "stick" layout (declarations).
+//
+// The wave-function sphere is stored in stick order: columns of constant
+// Miller (i, j) in the Gamma half plane (i > 0, or i == 0 and j >= 0), column
+// (0,0) first, k ascending inside a column, so that G = 0 is element 0.
+// Every column c has a mirror column (-i, -j) that is never stored; it is
+// generated from the stored one by Hermitian symmetry.
+//
+// A transform pair (inverse, pointwise real multiply, forward) of two real
+// bands packed as a1 + i a2 runs as three kernels:
+//   zinv  : sphere -> sticks    (1D inverse FFT along a3 for stored + mirror columns)
+//   plane : sticks -> sticks    (per a3-plane: inverse FFT along a2 on |i| <= imax rows,
+//                                inverse FFT along a1, pointwise op, forward FFT along a1
+//                                keeping |i| <= imax, forward FFT along a2 keeping columns)
+//   zfwd  : sticks -> sphere    (1D forward FFT along a3, fp/fm unpacking, epilogue)
+// so the full real-space grid never goes to HBM for the local-potential term.
+#pragma once
+
+#include "wh_common.h"
+
+namespace wh {
+
+constexpr int kMaxStages = 8;
+
+struct FftAxis {
+  int n;
+  int nstage;
+  int radix[kMaxStages];
+};
+
+struct StickPlanDev {
+  int n1, n2, n3;
+  int npw;        // points in the stored half sphere
+  int ncs;        // stored columns
+  int nct;        // all columns (2*ncs - 1)
+  int imax;       // max |i| over columns
+  int nrows;      // 2*imax + 1
+  const int* col_off;   // (ncs+1)
+  const int* col_kmin;  // (ncs)
+  const int* col_pos;   // (nct) (i+imax)*n2 + (j mod n2)
+  const int* pt_col;    // (npw) stored column of each point
+  const int* pt_k;      // (npw) Miller k of each point
+  FftAxis ax1, ax2, ax3;
+  const double* tw1;    // (2*n1) forward roots exp(-2 pi i m/n1), interleaved
+  const double* tw2;
+  const double* tw3;
+};
+
+struct StickPlan {
+  StickPlanDev dev{};
+  std::vector<int> col_off, col_kmin, col_pos, pt_col, pt_k;
+  std::vector<int> col_i, col_j;   // stored columns' Miller (i, j)
+  std::vector<int> perm;           // stick position -> caller index (0-based)
+  std::vector<void*> allocations;
+  void build(int nr1, int nr2, int nr3, int npw, const int* nl_1based, const int* nlm_1based);
+  void release();
+  long sticks_elems() const { return static_cast<long>(dev.n3) * dev.nct; }
+};
+
+// One transform item: bands a and b (row indices into a vector block; b < 0 -> zero),
+// pot = index of the real-space potential (per-item multiply) or -1 for the shared one.
+struct FftItem {
+  int a, b, pot, pad;
+};
+
+enum ZinvMode : int {
+  ZINV_PAIR = 0,      // b(nl) = a1 + i a2, b(nlm) = conj(a1 - i a2) (nlm written last)
+  ZINV_PAIR_REALG0 = 1  // same, but G = 0 takes (Re a1, Re a2): exact real parts of two single transforms
+};
+
+enum ZfwdMode : int {
+  ZFWD_PAIR_STORE = 0,   // out_a = scale*b1, out_b = scale*b2
+  ZFWD_PAIR_STERN = 1,   // out = b1 + (g2kin - e) * in   (local potential + kinetic - e)
+  ZFWD_SINGLE = 2        // out_a = scale * F(nl) * w(G), w = sqvc or 1 ; G = 0 set to zero if requested
+};
+
+struct ZfwdArgs {
+  int mode;
+  double scale;
+  const double* g2kin;    // stick order
+  const double* e;        // per-row eigenvalue (STERN), indexed by the out row
+  const double* weight;   // per-G weight (SINGLE), may be null
+  int zero_g0;
+};
+
+// kernel launchers (double precision)
+void launch_zinv(const StickPlan& p, const FftItem* d_items, int nitems, const z64* psi, long ldpsi,
+                 z64* sticks, int mode, hipStream_t s);
+// bands a from psiA, bands b from psiB
+void launch_zinv(const StickPlan& p, const FftItem* d_items, int nitems, const z64* psiA, long ldA,
+                 const z64* psiB, long ldB, z64* sticks, int mode, hipStream_t s);
+void launch_plane_vloc(const StickPlan& p, const FftItem* d_items, int nitems, const z64* sin, z64* sout,
+                       const double* vpot, long ldpot, double scale, hipStream_t s);
+void launch_plane_to_real(const StickPlan& p, const FftItem* d_items, int nitems, const z64* sin,
+                          double* out, long ldout, hipStream_t s);
+void launch_plane_density(const StickPlan& p, int npert, int nband, const z64* sin, const double* occ,
+                          z64* sout, hipStream_t s);
+void launch_zfwd(const StickPlan& p, const FftItem* d_items, int nitems, const z64* sticks,
+                 const z64* in, long ldin, z64* out, long ldout, const ZfwdArgs& args, hipStream_t s);
+
+}  // namespace wh

@@ -1,0 +1,16 @@
+// FP64 matrix-core GEMMs (declarations).
+#pragma once
+
+#include "wh_common.h"
+
+namespace wh {
+
+// C (M x N) = alpha * op(A) * B + beta * C, column major; op(A) = A^T if transa.
+void dgemm(bool transa, int M, int N, int K, double alpha, const double* A, long lda, const double* B, long ldb,
+           double beta, double* C, long ldc, hipStream_t s);
+
+// C(i, j) = scale_i * (C(i, j) - g0 * A[i*lda] * B[j*ldb])   (scale may be null)
+void g0_fix_scale(int M, int N, double g0, const double* A, long lda, const double* B, long ldb,
+                  const double* scale, double* C, long ldc, hipStream_t s);
+
+}  // namespace wh

@@ -1,0 +1,276 @@
+// Vector kernels of the Sternheimer CG and the dfpt data movement (FP64).
+//
+// Vectors are rows of npw complex numbers in stick order (G = 0 first).
+// Gamma inner products follow the reference: 2 Re sum a* b - Re a(G=0) Re b(G=0)
+// (DFPT_kernel/linsolve_sternheimer_m_wfcts.f90:162-181, 253-276).
+#include "wh_vec.h"
+
+namespace wh {
+
+namespace {
+
+constexpr int NT = 256;
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  // wavefront reduction then across the 4 wavefronts of the block
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int i = 0; i < NT / 64; ++i) s += red[i];
+  return s;
+}
+
+__device__ __forceinline__ double gamma_dot_part(const z64* a, const z64* b, int npw) {
+  double s = 0.0;
+  for (int g = threadIdx.x; g < npw; g += NT) s += a[g].x * b[g].x + a[g].y * b[g].y;
+  return s;
+}
+
+// h[ib] = g[ib] / max(1, g2kin/eprec[ib]);  rho[lb] = <h|g>   (linsolve 150-181)
+__global__ __launch_bounds__(NT) void k_prec_rho(int npw, int gstart, const int* l2i, const z64* g, z64* h,
+                                                 long ld, const double* g2kin, const double* eprec, double* rho) {
+  __shared__ double red[NT / 64];
+  const int lb = blockIdx.x;
+  const int ib = l2i[lb];
+  const z64* gi = g + static_cast<long>(ib) * ld;
+  z64* hi = h + static_cast<long>(ib) * ld;
+  const double ep = eprec[ib];
+  double s = 0.0;
+  for (int q = threadIdx.x; q < npw; q += NT) {
+    const double f = 1.0 / fmax(1.0, g2kin[q] / ep);
+    const z64 gv = gi[q];
+    const z64 hv = mk(gv.x * f, gv.y * f);
+    hi[q] = hv;
+    s += hv.x * gv.x + hv.y * gv.y;
+  }
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) {
+    double r = 2.0 * s;
+    if (gstart == 2) r -= hi[0].x * gi[0].x;
+    rho[lb] = r;
+  }
+}
+
+// h[ib] = -h[ib] + beta[lb] hold[ib];  bc[lb] = h[ib]   (linsolve 226-240)
+__global__ __launch_bounds__(NT) void k_direction(int npw, const int* l2i, const double* beta, z64* h,
+                                                  const z64* hold, z64* bc, long ld) {
+  const int lb = blockIdx.y;
+  const int ib = l2i[lb];
+  const int q = blockIdx.x * NT + threadIdx.x;
+  if (q >= npw) return;
+  const double bt = beta[lb];
+  const long o = static_cast<long>(ib) * ld + q;
+  z64 v = h[o];
+  v = mk(-v.x, -v.y);
+  if (bt != 0.0) {
+    const z64 w = hold[o];
+    v = mk(v.x + bt * w.x, v.y + bt * w.y);
+  }
+  h[o] = v;
+  bc[static_cast<long>(lb) * ld + q] = v;
+}
+
+// a = <h|g>, c = <h|t>; x -= (a/c) h; g -= (a/c) t   (linsolve 253-321)
+__global__ __launch_bounds__(NT) void k_cg_update(int npw, int gstart, const int* l2i, const z64* h, z64* g,
+                                                  const z64* tc, z64* x, long ld, double* ac_out) {
+  __shared__ double red[NT / 64];
+  const int lb = blockIdx.x;
+  const int ib = l2i[lb];
+  const z64* hi = h + static_cast<long>(ib) * ld;
+  z64* gi = g + static_cast<long>(ib) * ld;
+  const z64* ti = tc + static_cast<long>(lb) * ld;
+  z64* xi = x + static_cast<long>(ib) * ld;
+  double sa = 0.0, sc = 0.0;
+  for (int q = threadIdx.x; q < npw; q += NT) {
+    const z64 hv = hi[q];
+    const z64 gv = gi[q];
+    const z64 tv = ti[q];
+    sa += hv.x * gv.x + hv.y * gv.y;
+    sc += hv.x * tv.x + hv.y * tv.y;
+  }
+  sa = block_sum(sa, red);
+  sc = block_sum(sc, red);
+  double a = 2.0 * sa, c = 2.0 * sc;
+  if (gstart == 2) {
+    a -= hi[0].x * gi[0].x;
+    c -= hi[0].x * ti[0].x;
+  }
+  __syncthreads();   // every thread has read gi[0] before it is updated
+  const double step = a / c;
+  if (threadIdx.x == 0 && ac_out) {
+    ac_out[2 * lb] = a;
+    ac_out[2 * lb + 1] = c;
+  }
+  for (int q = threadIdx.x; q < npw; q += NT) {
+    const z64 hv = hi[q];
+    const z64 tv = ti[q];
+    z64 xv = xi[q];
+    z64 gv = gi[q];
+    xi[q] = mk(xv.x - step * hv.x, xv.y - step * hv.y);
+    gi[q] = mk(gv.x - step * tv.x, gv.y - step * tv.y);
+  }
+}
+
+// out[r][s] = in[r][perm[s]] * (w ? w[s] : 1)     caller order -> stick order
+__global__ void k_gather(int m, int npw, const int* perm, const z64* in, long ldin, z64* out, long ldout,
+                         const double* w) {
+  const int r = blockIdx.y;
+  const int s = blockIdx.x * NT + threadIdx.x;
+  if (s >= npw || r >= m) return;
+  z64 v = in[static_cast<long>(r) * ldin + perm[s]];
+  if (w) v = cscale(v, w[s]);
+  out[static_cast<long>(r) * ldout + s] = v;
+}
+
+// out[r][perm[s]] = in[r][s]; out[r][npw..ldout) = 0        stick order -> caller order
+__global__ void k_scatter(int m, int npw, int npwx, const int* perm, const z64* in, long ldin, z64* out,
+                          long ldout) {
+  const int r = blockIdx.y;
+  const int s = blockIdx.x * NT + threadIdx.x;
+  if (r >= m) return;
+  if (s < npw) out[static_cast<long>(r) * ldout + perm[s]] = in[static_cast<long>(r) * ldin + s];
+  else if (s < npwx) out[static_cast<long>(r) * ldout + s] = mk(0.0, 0.0);
+}
+
+// pf[r] = f[r] / max(1, g2kin / eprec[r % period])
+__global__ void k_precondition(int m, int npw, const z64* f, z64* pf, long ld, const double* g2kin,
+                               const double* eprec, int period) {
+  const int r = blockIdx.y;
+  const int s = blockIdx.x * NT + threadIdx.x;
+  if (s >= npw || r >= m) return;
+  const double ep = eprec[r % period];
+  const double fac = 1.0 / fmax(1.0, g2kin[s] / ep);
+  pf[static_cast<long>(r) * ld + s] = cscale(f[static_cast<long>(r) * ld + s], fac);
+}
+
+// t[r] = (g2kin - e[r]) * x[r]      (k_psi - e psi)
+__global__ void k_kinetic(int m, int npw, const z64* x, z64* t, long ld, const double* g2kin, const double* e) {
+  const int r = blockIdx.y;
+  const int s = blockIdx.x * NT + threadIdx.x;
+  if (s >= npw || r >= m) return;
+  t[static_cast<long>(r) * ld + s] = cscale(x[static_cast<long>(r) * ld + s], g2kin[s] - e[r]);
+}
+
+// y = a*x + b*y over n complex
+__global__ void k_axpby(long n, double a, const z64* x, double b, z64* y) {
+  const long i = static_cast<long>(blockIdx.x) * NT + threadIdx.x;
+  if (i >= n) return;
+  const z64 xv = x[i];
+  const z64 yv = y[i];
+  y[i] = mk(a * xv.x + b * yv.x, a * xv.y + b * yv.y);
+}
+
+__global__ void k_fill_pairs(FftItem* items, int n, int m, int row0) {
+  const int q = blockIdx.x * NT + threadIdx.x;
+  if (q >= n) return;
+  FftItem it;
+  it.a = row0 + 2 * q;
+  it.b = (2 * q + 1 < m) ? row0 + 2 * q + 1 : -1;
+  it.pot = -1;
+  it.pad = 0;
+  items[q] = it;
+}
+
+__global__ void k_fill_ident(FftItem* items, int n) {
+  const int q = blockIdx.x * NT + threadIdx.x;
+  if (q >= n) return;
+  FftItem it;
+  it.a = q;
+  it.b = -1;
+  it.pot = -1;
+  it.pad = 0;
+  items[q] = it;
+}
+
+// eprec[v] = 1.35 sum_{g<npw} |evc_v(g)|^2 g2kin(g)   (Tools/set_eprec.f90:40-51)
+__global__ __launch_bounds__(NT) void k_eprec(int npw, const z64* evc, long ld, const double* g2kin, double* eprec) {
+  __shared__ double red[NT / 64];
+  const int v = blockIdx.x;
+  const z64* e = evc + static_cast<long>(v) * ld;
+  double s = 0.0;
+  for (int q = threadIdx.x; q < npw; q += NT) s += (e[q].x * e[q].x + e[q].y * e[q].y) * g2kin[q];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) eprec[v] = 1.35 * s;
+}
+
+}  // namespace
+
+void prec_rho(int nc, int npw, int gstart, const int* l2i, const z64* g, z64* h, long ld, const double* g2kin,
+              const double* eprec, double* rho, hipStream_t s) {
+  if (nc <= 0) return;
+  k_prec_rho<<<nc, NT, 0, s>>>(npw, gstart, l2i, g, h, ld, g2kin, eprec, rho);
+  WH_HIP(hipGetLastError());
+}
+
+void direction(int nc, int npw, const int* l2i, const double* beta, z64* h, const z64* hold, z64* bc, long ld,
+               hipStream_t s) {
+  if (nc <= 0) return;
+  k_direction<<<dim3(ceil_div(npw, NT), nc), NT, 0, s>>>(npw, l2i, beta, h, hold, bc, ld);
+  WH_HIP(hipGetLastError());
+}
+
+void cg_update(int nc, int npw, int gstart, const int* l2i, const z64* h, z64* g, const z64* tc, z64* x, long ld,
+               double* ac_out, hipStream_t s) {
+  if (nc <= 0) return;
+  k_cg_update<<<nc, NT, 0, s>>>(npw, gstart, l2i, h, g, tc, x, ld, ac_out);
+  WH_HIP(hipGetLastError());
+}
+
+void gather_sticks(int m, int npw, const int* perm, const z64* in, long ldin, z64* out, long ldout, const double* w,
+                   hipStream_t s) {
+  if (m <= 0) return;
+  k_gather<<<dim3(ceil_div(npw, NT), m), NT, 0, s>>>(m, npw, perm, in, ldin, out, ldout, w);
+  WH_HIP(hipGetLastError());
+}
+
+void scatter_caller(int m, int npw, int npwx, const int* perm, const z64* in, long ldin, z64* out, long ldout,
+                    hipStream_t s) {
+  if (m <= 0) return;
+  k_scatter<<<dim3(ceil_div(npwx, NT), m), NT, 0, s>>>(m, npw, npwx, perm, in, ldin, out, ldout);
+  WH_HIP(hipGetLastError());
+}
+
+void precondition(int m, int npw, const z64* f, z64* pf, long ld, const double* g2kin, const double* eprec,
+                  int period, hipStream_t s) {
+  if (m <= 0) return;
+  k_precondition<<<dim3(ceil_div(npw, NT), m), NT, 0, s>>>(m, npw, f, pf, ld, g2kin, eprec, period);
+  WH_HIP(hipGetLastError());
+}
+
+void kinetic_minus_e(int m, int npw, const z64* x, z64* t, long ld, const double* g2kin, const double* e,
+                     hipStream_t s) {
+  if (m <= 0) return;
+  k_kinetic<<<dim3(ceil_div(npw, NT), m), NT, 0, s>>>(m, npw, x, t, ld, g2kin, e);
+  WH_HIP(hipGetLastError());
+}
+
+void axpby(long n, double a, const z64* x, double b, z64* y, hipStream_t s) {
+  if (n <= 0) return;
+  k_axpby<<<ceil_div(n, NT), NT, 0, s>>>(n, a, x, b, y);
+  WH_HIP(hipGetLastError());
+}
+
+void fill_pairs(FftItem* items, int m, int row0, hipStream_t s) {
+  const int n = (m + 1) / 2;
+  if (n <= 0) return;
+  k_fill_pairs<<<ceil_div(n, NT), NT, 0, s>>>(items, n, m, row0);
+  WH_HIP(hipGetLastError());
+}
+
+void fill_ident(FftItem* items, int n, hipStream_t s) {
+  if (n <= 0) return;
+  k_fill_ident<<<ceil_div(n, NT), NT, 0, s>>>(items, n);
+  WH_HIP(hipGetLastError());
+}
+
+void eprec_of(int nb, int npw, const z64* evc, long ld, const double* g2kin, double* eprec, hipStream_t s) {
+  if (nb <= 0) return;
+  k_eprec<<<nb, NT, 0, s>>>(npw, evc, ld, g2kin, eprec);
+  WH_HIP(hipGetLastError());
+}
+
+}  // namespace wh

@@ -1,0 +1,183 @@
+"""Host-side mirror of the reference's Wstat call surfaces, running on the
+MI355X library.
+
+Names, argument meaning and error behaviour follow the reference:
+
+* :func:`apply_sternheimerop_to_m_wfcs` -- ``DFPT_kernel/apply_sternheimerop_to_m_wfcs.f90:14``
+* :func:`linsolve_sternheimer_m_wfcts`  -- ``DFPT_kernel/linsolve_sternheimer_m_wfcts.f90:14``
+  (``b`` may be destroyed, ``x`` is refined in place, returns ``ierr``)
+* :func:`dfpt`                          -- ``DFPT_kernel/dfpt_module.f90:22``
+* :func:`apply_operator`                -- ``Wstat/apply_operator.f90:14``
+* :func:`davidson_diago`                -- ``Wstat/davidson_diago.f90:31``
+
+Arrays are numpy ``(m, npwx)`` complex128 (the memory layout of the
+reference's Fortran ``(npwx, m)``) or torch tensors of the same shape on the
+device.  Non-convergence is reported like the reference: a warning count
+(``ierr``), not an exception.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import warnings
+
+import numpy as np
+
+from .lib import DavidsonParams, DavidsonStep, check, load, ptr
+
+_state = {"device": None, "gs": None}
+
+
+def init(device: int = 0):
+    lib = load()
+    check(lib.wh_init(int(device)))
+    _state["device"] = device
+    return lib
+
+
+def set_ground_state(gs, sqvc=None, n_dfpt_maxiter: int = 250, l_kinetic_only: bool = False):
+    """Register a :class:`west_amd.groundstate.GroundState` (the module globals
+    that the reference's dfpt reads: dffts, g2kin, vrs, vkb, evc, et ...)."""
+    lib = load()
+    b = gs.basis
+    nl = np.ascontiguousarray(b.nl, dtype=np.int32)
+    nlm = np.ascontiguousarray(b.nlm, dtype=np.int32)
+    g2 = np.zeros(gs.npwx)
+    g2[:b.npw] = b.g2kin
+    check(lib.wh_set_grid(b.nr[0], b.nr[1], b.nr[2], b.npw, gs.npwx, b.gstart, ptr(nl), ptr(nlm),
+                          float(b.omega), float(b.tpiba2), ptr(g2)))
+    check(lib.wh_set_params(int(n_dfpt_maxiter), int(bool(l_kinetic_only))))
+    check(lib.wh_set_potential(ptr(np.ascontiguousarray(gs.vrs, dtype=np.float64))))
+    vkb = np.ascontiguousarray(gs.vkb, dtype=np.complex128)
+    check(lib.wh_set_nonlocal(int(vkb.shape[0]), ptr(vkb) if vkb.size else None,
+                              ptr(np.ascontiguousarray(gs.dvan, dtype=np.float64)) if vkb.size else None))
+    nbv = gs.nbndval
+    evc = np.ascontiguousarray(gs.evc[:nbv], dtype=np.complex128)
+    et = np.ascontiguousarray(np.asarray(gs.et)[:nbv], dtype=np.float64)
+    occ = np.ascontiguousarray(np.asarray(gs.occ)[:nbv], dtype=np.float64)
+    check(lib.wh_set_states(nbv, ptr(evc), ptr(et), ptr(occ), float(gs.weight), float(gs.alpha_pv)))
+    if sqvc is not None:
+        sq = np.zeros(gs.npwx)
+        sq[:len(sqvc)] = sqvc
+        check(lib.wh_set_coulomb(ptr(sq)))
+    _state["gs"] = gs
+    _state["grid_key"] = None
+
+
+def _out_like(a, shape):
+    if isinstance(a, np.ndarray):
+        return np.zeros(shape, dtype=np.complex128)
+    import torch
+    return torch.zeros(shape, dtype=torch.complex128, device=a.device)
+
+
+def apply_sternheimerop_to_m_wfcs(nbndval: int, psi, e, alpha, m: int | None = None, hpsi=None):
+    """hpsi = H psi - e psi + alpha P_v psi."""
+    lib = load()
+    m = psi.shape[0] if m is None else m
+    if hpsi is None:
+        hpsi = _out_like(psi, psi.shape)
+    e = np.ascontiguousarray(np.asarray(e, dtype=np.float64)) if not hasattr(e, "data_ptr") else e
+    al = np.array([complex(alpha).real, complex(alpha).imag])
+    check(lib.wh_apply_sternheimerop_to_m_wfcs(int(nbndval), ptr(psi), ptr(hpsi), ptr(e), ptr(al), int(m)))
+    return hpsi
+
+
+def linsolve_sternheimer_m_wfcts(nbndval: int, m: int, b, x, e, eprec, tr2: float) -> int:
+    """Solves (H - e_i + alpha_pv P_v) x_i = b_i in place of x; returns ierr."""
+    lib = load()
+    e = np.ascontiguousarray(np.asarray(e, dtype=np.float64))
+    eprec = np.ascontiguousarray(np.asarray(eprec, dtype=np.float64))
+    ierr = C.c_int(0)
+    rc = check(lib.wh_linsolve_sternheimer_m_wfcts(int(nbndval), int(m), ptr(b), ptr(x), ptr(e), ptr(eprec),
+                                                    float(tr2), C.addressof(ierr)), allow_positive=True)
+    return int(ierr.value) if rc >= 0 else rc
+
+
+def dfpt(m: int, dvg, tr2: float, iq: int = 1, dng=None, return_ierr: bool = False):
+    """dng = chi0 dvg (m perturbations)."""
+    lib = load()
+    if dng is None:
+        dng = _out_like(dvg, dvg.shape)
+    ierr = np.zeros(max(m, 1), dtype=np.int32)
+    rc = check(lib.wh_dfpt(int(m), ptr(dvg), ptr(dng), float(tr2), int(iq), ptr(ierr)), allow_positive=True)
+    for p in np.flatnonzero(ierr[:m]):
+        warnings.warn(f"** WARNING : PERT {p + 1} not converged, ierr = {ierr[p]}")
+    return (dng, ierr[:m]) if return_ierr else dng
+
+
+def apply_operator(m: int, dvg, tr2: float, iq: int = 1, dng=None):
+    """dng = sqvc chi0 (sqvc dvg)."""
+    lib = load()
+    if dng is None:
+        dng = _out_like(dvg, dvg.shape)
+    rc = check(lib.wh_apply_operator(int(m), ptr(dvg), ptr(dng), float(tr2), int(iq)), allow_positive=True)
+    if rc > 0:
+        warnings.warn(f"** WARNING : {rc} Sternheimer systems not converged")
+    return dng
+
+
+def davidson_diago(n_pdep_eigen: int, n_pdep_times: int = 4, n_pdep_maxiter: int = 100, trev_pdep: float = 1e-3,
+                   trev_pdep_rel: float = 0.1, tr2_dfpt: float = 1e-16, want_vectors: bool = True):
+    """Wstat Davidson; returns (ev, conv, dvg or None, davitr records)."""
+    lib = load()
+    gs = _state["gs"]
+    prm = DavidsonParams(n_pdep_eigen, n_pdep_times, n_pdep_maxiter, trev_pdep, trev_pdep_rel, tr2_dfpt)
+    ev = np.zeros(n_pdep_eigen)
+    conv = np.zeros(n_pdep_eigen, dtype=np.int32)
+    dvg = np.zeros((n_pdep_eigen, gs.npwx), dtype=np.complex128) if want_vectors else None
+    cap = n_pdep_maxiter + 2
+    log = (DavidsonStep * cap)()
+    nlog = C.c_int(0)
+    rc = check(lib.wh_davidson_diago(C.addressof(prm), ptr(ev), ptr(conv), ptr(dvg) if dvg is not None else None,
+                                     C.addressof(log), cap, C.addressof(nlog)), allow_positive=True)
+    recs = [{"dav_iter": s.dav_iter, "notcnv": s.notcnv, "dfpt_dim": s.dfpt_dim, "diago_dim": s.diago_dim,
+             "sternop_ncalls": s.sternop_ncalls, "dfpt_tr2": s.dfpt_tr2, "time_elap:sec": s.time_elap,
+             "time_iter:sec": s.time_iter} for s in log[:nlog.value]]
+    if rc > 0:
+        warnings.warn(f"WARNING : {rc} eigenvalues not converged in chidiago")
+    return ev, conv.astype(bool), dvg, recs
+
+
+def gpu_h_factory(gs):
+    """``apply_h_factory`` for :func:`west_amd.groundstate.make_ground_state`:
+    registers the grid (once), the potential and the projectors of the partial
+    ground state ``gs`` and returns ``psi -> H psi`` computed by the library
+    (the Sternheimer operator with no registered states, e = 0)."""
+    lib = load()
+    b = gs.basis
+    if _state.get("grid_key") != id(b):
+        nl = np.ascontiguousarray(b.nl, dtype=np.int32)
+        nlm = np.ascontiguousarray(b.nlm, dtype=np.int32)
+        g2 = np.zeros(gs.npwx)
+        g2[:b.npw] = b.g2kin
+        check(lib.wh_set_grid(b.nr[0], b.nr[1], b.nr[2], b.npw, gs.npwx, b.gstart, ptr(nl), ptr(nlm),
+                              float(b.omega), float(b.tpiba2), ptr(g2)))
+        vkb = np.ascontiguousarray(gs.vkb, dtype=np.complex128)
+        check(lib.wh_set_nonlocal(int(vkb.shape[0]), ptr(vkb) if vkb.size else None,
+                                  ptr(np.ascontiguousarray(gs.dvan, dtype=np.float64)) if vkb.size else None))
+        check(lib.wh_set_params(250, 0))
+        _state["grid_key"] = id(b)
+    check(lib.wh_set_potential(ptr(np.ascontiguousarray(gs.vrs, dtype=np.float64))))
+
+    def h_psi(psi):
+        psi = np.ascontiguousarray(psi, dtype=np.complex128)
+        return apply_sternheimerop_to_m_wfcs(0, psi, np.zeros(psi.shape[0]), 0.0)
+
+    return h_psi
+
+
+def counters():
+    lib = load()
+    a, b = C.c_long(0), C.c_long(0)
+    check(lib.wh_get_counters(C.addressof(a), C.addressof(b)))
+    return int(a.value), int(b.value)
+
+
+def profile(enable: bool):
+    check(load().wh_profile(int(bool(enable))))
+
+
+def profile_query(name: str):
+    ms, cnt = C.c_double(0), C.c_long(0)
+    check(load().wh_profile_query(name.encode(), C.addressof(ms), C.addressof(cnt)))
+    return float(ms.value), int(cnt.value)
