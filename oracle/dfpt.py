@@ -27,19 +27,22 @@ def sqvc_init(basis, npwqx=None):
     return sq
 
 
-def dfpt(gs, dvg, tr2, n_dfpt_maxiter=250, stats=None, ierr_out=None):
+def dfpt(gs, dvg, tr2, n_dfpt_maxiter=250, stats=None, ierr_out=None, bands=None):
     """dng = chi0 dvg.  dvg: (m, npwqx).  tr2 < 0: no Sternheimer solve
-    (the initial guess M^-1 (-P_c dv psi) is used, ``dfpt_module.f90:130,348-361``)."""
+    (the initial guess M^-1 (-P_c dv psi) is used, ``dfpt_module.f90:130,348-361``).
+    ``bands`` (timing samples only) restricts the band loop to a subset of the
+    occupied bands; the projectors still span all of them."""
     b = gs.basis
     npw = b.npw
     npwx = gs.npwx
-    nbndval = gs.nbndval
     m = dvg.shape[0]
     dng = np.zeros_like(dvg)
-    evc = gs.evc[:nbndval]
+    sel = np.arange(gs.nbndval) if bands is None else np.asarray(bands)
+    evc = gs.evc[sel]
+    nbndval = len(sel)
     eprec = set_eprec(gs, evc)                                   # (211)
-    et = np.asarray(gs.et)[:nbndval]
-    occ = np.asarray(gs.occ)[:nbndval]
+    et = np.asarray(gs.et)[sel]
+    occ = np.asarray(gs.occ)[sel]
     npair = nbndval // 2
     for ipert in range(m):
         aux_r = single_invfft_gamma(b, npw, dvg[ipert:ipert + 1])[0]      # (229-235)
@@ -71,11 +74,11 @@ def dfpt(gs, dvg, tr2, n_dfpt_maxiter=250, stats=None, ierr_out=None):
     return dng
 
 
-def apply_operator(gs, sqvc, dvg, tr2, n_dfpt_maxiter=250, stats=None):
+def apply_operator(gs, sqvc, dvg, tr2, n_dfpt_maxiter=250, stats=None, bands=None):
     """``Wstat/apply_operator.f90:49-67``: dng = sqvc * chi0(sqvc * dvg)."""
     npwq = gs.basis.npw
     aux = np.zeros_like(dvg)
     aux[:, :npwq] = dvg[:, :npwq] * sqvc[None, :npwq]
-    dng = dfpt(gs, aux, tr2, n_dfpt_maxiter, stats)
+    dng = dfpt(gs, aux, tr2, n_dfpt_maxiter, stats, bands=bands)
     dng[:, :npwq] *= sqvc[None, :npwq]
     return dng

@@ -40,6 +40,37 @@ def test_sternheimer_operator(gpu, key, m, request):
     assert _rel(out, ref) < 1e-12
 
 
+@pytest.mark.parametrize("nrep,ecut", [(1, 25.0), (2, 25.0)])
+def test_operator_workload_grids(gpu, nrep, ecut):
+    """The operator on the workload grids (36^3 and 72^3; the plane kernel of
+    the latter needs more than 64 KB of LDS), with the Harris potential and
+    random orthonormal occupied states."""
+    from oracle.hamiltonian import apply_sternheimerop
+    from west_amd.groundstate import GroundState, _orthonormalize, from_real, to_real
+    from west_amd.systems import atomic_density_g, kb_projectors, ks_potential, make_si_system
+    s = make_si_system(nrep=nrep, ecutwfc=ecut, seed=9)
+    b = s.basis
+    vkb, dvan = kb_projectors(s)
+    nv = 8
+    ev = from_real(_orthonormalize(to_real(random_wfc_basis(b, nv, 3), b.npw, 2)), b.npw, b.npw, 2)
+    gs = GroundState(basis=b, npwx=b.npw, vrs=ks_potential(s, atomic_density_g(s)), vkb=np.ascontiguousarray(vkb.T),
+                     dvan=dvan, evc=ev, et=np.linspace(-0.5, 0.2, nv), nbndval=nv, occ=np.ones(nv), alpha_pv=1.4)
+    _register(gpu, gs)
+    print(f"case nrep={nrep} nr={b.nr} npw={b.npw}", flush=True)
+    psi = random_wfc(gs, 3, seed=7)
+    e = np.array([-0.4, -0.1, 0.05])
+    ref = apply_sternheimerop(gs, psi, e, gs.alpha_pv)
+    out = gpu.apply_sternheimerop_to_m_wfcs(nv, psi, e, gs.alpha_pv)
+    assert _rel(out, ref) < 1e-12
+
+
+def random_wfc_basis(b, m, seed):
+    rng = np.random.default_rng(seed)
+    a = (rng.normal(size=(m, b.npw)) + 1j * rng.normal(size=(m, b.npw))) / (1.0 + b.g2kin)[None, :]
+    a[:, 0] = a[:, 0].real
+    return a
+
+
 def test_kinetic_only_operator(gpu, gs_si8):
     from oracle.hamiltonian import apply_sternheimerop
     gs = gs_si8

@@ -8,8 +8,8 @@ systems of :mod:`west_amd.systems`:
   superposition of Gaussian atomic valence densities, or the self-consistent
   density when ``scf_steps > 0`` (Pulay mixing of the density);
 * the lowest bands of ``H = -lap + vrs + sum |beta> D <beta|`` are found by a
-  block LOBPCG in the real representation of Gamma-point wave functions (or by
-  dense diagonalisation for tiny bases).  ``apply_h`` is a caller-supplied
+  preconditioned block Davidson in the real representation of Gamma-point
+  wave functions (or by dense diagonalisation for tiny bases).  ``apply_h`` is a caller-supplied
   callable ``psi (m, npwx) -> H psi``; the GPU library provides one.
 
 A Gamma wave function with half-sphere coefficients ``c`` (``c(G=0)`` real) is
@@ -72,66 +72,67 @@ def from_real(u: np.ndarray, npw: int, npwx: int, gstart: int) -> np.ndarray:
     return out
 
 
-def _orthonormalize(X: np.ndarray) -> np.ndarray:
-    """Rows of X orthonormalised (stable: eigen-decomposition of the Gram matrix)."""
-    S = X @ X.T
-    w, V = np.linalg.eigh(S)
-    keep = w > w.max() * 1e-14
-    V = V[:, keep] / np.sqrt(w[keep])
-    return V.T @ X
+def _orthonormalize(X: np.ndarray, cutoff: float = 1e-14) -> np.ndarray:
+    """Rows of X orthonormalised (eigen-decomposition of the Gram matrix of the
+    row-normalised block; directions with relative weight below ``cutoff``
+    are dropped), followed by a second pass for orthogonality to rounding."""
+    if X.shape[0] == 0:
+        return X
+    n = np.linalg.norm(X, axis=1)
+    X = X[n > 0] / n[n > 0, None]
+    for _ in range(2):
+        S = X @ X.T
+        w, V = np.linalg.eigh(0.5 * (S + S.T))
+        keep = w > w.max() * cutoff
+        X = (V[:, keep] / np.sqrt(w[keep])).T @ X
+    return X
 
 
-def lobpcg(apply_real, precond, X0: np.ndarray, nev: int, tol: float = 1e-9, maxiter: int = 400,
-           verbose: bool = False):
-    """Block LOBPCG for the ``nev`` lowest eigenpairs of a symmetric operator.
-    X0: (nblock, dim) start block (nblock >= nev).  Returns (eigvals, X)."""
-    X = _orthonormalize(X0)
-    HX = apply_real(X)
-    T = X @ HX.T
-    w, V = np.linalg.eigh(0.5 * (T + T.T))
-    X = V.T @ X
-    HX = V.T @ HX
-    lam = w
-    P = HP = None
-    nb = X.shape[0]
+def _expand(W: np.ndarray, V: np.ndarray) -> np.ndarray:
+    """New orthonormal directions of W outside span(V) (V orthonormal rows)."""
+    n = np.linalg.norm(W, axis=1)
+    W = W[n > 0] / n[n > 0, None]
+    for _ in range(2):
+        W = W - (W @ V.T) @ V
+    n = np.linalg.norm(W, axis=1)
+    W = W[n > 1e-8] / n[n > 1e-8, None]
+    W = _orthonormalize(W, cutoff=1e-10)
+    W = W - (W @ V.T) @ V
+    return W / np.linalg.norm(W, axis=1)[:, None] if W.shape[0] else W
+
+
+def block_davidson(apply_real, precond, X0: np.ndarray, nev: int, tol: float = 1e-9, maxiter: int = 400,
+                   max_blocks: int = 4, verbose: bool = False):
+    """Block Davidson with preconditioned residuals and thick restart for the
+    ``nev`` lowest eigenpairs of a symmetric operator.  The basis is kept
+    orthonormal, so the restart combinations are well conditioned.
+    X0: (nblock, dim) start block (nblock >= nev).  Returns (eigvals, X, iters)."""
+    V = _orthonormalize(X0)
+    HV = apply_real(V)
+    nb = X0.shape[0]
+    maxdim = max_blocks * nb
+    it = 0
     for it in range(maxiter):
+        T = V @ HV.T
+        w, C = np.linalg.eigh(0.5 * (T + T.T))
+        X = C[:, :nb].T @ V
+        HX = C[:, :nb].T @ HV
+        lam = w[:nb]
         R = HX - lam[:, None] * X
         rn = np.linalg.norm(R, axis=1)
         if verbose and it % 10 == 0:
-            print(f"  lobpcg it {it}: max res {rn[:nev].max():.3e}")
+            print(f"  davidson it {it}: dim {V.shape[0]} max res {rn[:nev].max():.3e}")
         if rn[:nev].max() < tol:
             break
         active = rn > tol * 0.1
         W = precond(R[active], lam[active])
-        W = W - (W @ X.T) @ X
-        if P is not None:
-            W = W - (W @ P.T) @ P if P.shape[0] else W
-        W = _orthonormalize(W)
-        HW = apply_real(W)
-        blocks = [X, W] + ([P] if P is not None and P.shape[0] else [])
-        hblocks = [HX, HW] + ([HP] if P is not None and P.shape[0] else [])
-        S = np.concatenate(blocks, axis=0)
-        HS = np.concatenate(hblocks, axis=0)
-        G = S @ S.T
-        A = S @ HS.T
-        A = 0.5 * (A + A.T)
-        gw, gv = np.linalg.eigh(G)
-        keep = gw > gw.max() * 1e-13
-        Bm = gv[:, keep] / np.sqrt(gw[keep])
-        aw, av = np.linalg.eigh(Bm.T @ A @ Bm)
-        C = Bm @ av[:, :nb]
-        Xn = C.T @ S
-        HXn = C.T @ HS
-        # new search direction: the part of the update outside the old X
-        Cp = C.copy()
-        Cp[:nb, :] = 0.0
-        P = Cp.T @ S
-        HP = Cp.T @ HS
-        pn = np.linalg.norm(P, axis=1)
-        good = pn > 1e-14
-        P = P[good] / pn[good, None]
-        HP = HP[good] / pn[good, None]
-        X, HX, lam = Xn, HXn, aw[:nb]
+        if V.shape[0] + W.shape[0] > maxdim:
+            V, HV = X, HX
+        W = _expand(W, V)
+        if W.shape[0] == 0:
+            break
+        V = np.concatenate([V, W])
+        HV = np.concatenate([HV, apply_real(W)])
     return lam[:nev], X[:nev], it
 
 
@@ -178,7 +179,7 @@ def solve_bands(basis: GammaBasis, apply_h, nbnd: int, npwx: int | None = None, 
             if X0.shape[0] < nb:
                 rng = np.random.default_rng(seed)
                 X0 = np.concatenate([X0, rng.normal(size=(nb - X0.shape[0], dim)) / (1.0 + g2r[None, :])])
-        et, X, _ = lobpcg(apply_real, precond, X0, nbnd, tol=tol, maxiter=maxiter, verbose=verbose)
+        et, X, _ = block_davidson(apply_real, precond, X0, nbnd, tol=tol, maxiter=maxiter, verbose=verbose)
     X = _orthonormalize(X)
     # final Rayleigh-Ritz so that (et, X) are exact Ritz pairs of the returned basis
     HX = apply_real(X)
