@@ -71,6 +71,27 @@ def random_wfc_basis(b, m, seed):
     return a
 
 
+def test_runtime_size_fft_path(gpu, gs_si8):
+    """The runtime-size kernels (used for grids without a compiled
+    specialisation) agree with the oracle too."""
+    import os
+    from oracle.dfpt import dfpt
+    from oracle.hamiltonian import apply_sternheimerop
+    gs = gs_si8
+    os.environ["WH_FFT_GENERIC"] = "1"
+    try:
+        _register(gpu, gs)
+        psi = random_wfc(gs, 3, seed=17)
+        e = np.array([-0.3, 0.0, 0.2])
+        assert _rel(gpu.apply_sternheimerop_to_m_wfcs(gs.nbndval, psi, e, gs.alpha_pv),
+                    apply_sternheimerop(gs, psi, e, gs.alpha_pv)) < 1e-12
+        dvg = random_wfc(gs, 2, seed=18)
+        assert _rel(gpu.dfpt(2, dvg, -1.0), dfpt(gs, dvg, -1.0)) < 1e-12
+    finally:
+        del os.environ["WH_FFT_GENERIC"]
+        _register(gpu, gs)
+
+
 def test_kinetic_only_operator(gpu, gs_si8):
     from oracle.hamiltonian import apply_sternheimerop
     gs = gs_si8

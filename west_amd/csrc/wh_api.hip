@@ -134,8 +134,8 @@ void glbrak_q(int row0, int nq, const z64* f, int m, double* cmat) {
   const long k2 = 2L * c.npw;
   const double* q = reinterpret_cast<const double*>(c.d_q) + static_cast<long>(row0) * k2;
   const double* fr = reinterpret_cast<const double*>(f);
-  WH_TIMED("gemm_bra", dgemm(true, nq, m, static_cast<int>(k2), 2.0, q, k2, fr, k2, 0.0, cmat, nq, c.stream));
-  WH_TIMED("gemm_fix", g0_fix_scale(nq, m, c.gstart == 2 ? 1.0 : 0.0, q, k2, fr, k2, nullptr, cmat, nq, c.stream));
+  WH_TIMED("gemm_bra", dgemm_bra(nq, m, static_cast<int>(k2), 2.0, q, k2, fr, k2, c.gstart == 2 ? 1.0 : 0.0,
+                                 nullptr, cmat, nq, c.stream));
 }
 
 // ---------------------------------------------------------------------------
@@ -177,8 +177,8 @@ void stern_apply(const z64* x, z64* t, const double* d_e, int m, double alpha) {
     double* cm = c.w_cmat.as<double>(static_cast<size_t>(nq) * m);
     const double* q = reinterpret_cast<const double*>(c.d_q);
     const double* xr = reinterpret_cast<const double*>(x);
-    WH_TIMED("gemm_bra", dgemm(true, nq, m, static_cast<int>(k2), 2.0, q, k2, xr, k2, 0.0, cm, nq, s));
-    WH_TIMED("gemm_fix", g0_fix_scale(nq, m, c.gstart == 2 ? 1.0 : 0.0, q, k2, xr, k2, c.d_qscale, cm, nq, s));
+    WH_TIMED("gemm_bra", dgemm_bra(nq, m, static_cast<int>(k2), 2.0, q, k2, xr, k2, c.gstart == 2 ? 1.0 : 0.0,
+                                   c.d_qscale, cm, nq, s));
     WH_TIMED("gemm_ket", dgemm(false, static_cast<int>(k2), m, nq, 1.0, q, k2, cm, nq, 1.0,
                                reinterpret_cast<double*>(t), k2, s));
   }

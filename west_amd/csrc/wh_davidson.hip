@@ -115,8 +115,8 @@ void glbrak_rows(const z64* A, int na, const z64* B, int nb, double* Cm, int ldc
   const long k2 = 2L * c.npw;
   const double* ar = reinterpret_cast<const double*>(A);
   const double* br = reinterpret_cast<const double*>(B);
-  WH_TIMED("gemm_bra", dgemm(true, na, nb, static_cast<int>(k2), 2.0, ar, k2, br, k2, 0.0, Cm, ldc, c.stream));
-  WH_TIMED("gemm_fix", g0_fix_scale(na, nb, c.gstart == 2 ? 1.0 : 0.0, ar, k2, br, k2, nullptr, Cm, ldc, c.stream));
+  WH_TIMED("gemm_bra", dgemm_bra(na, nb, static_cast<int>(k2), 2.0, ar, k2, br, k2, c.gstart == 2 ? 1.0 : 0.0,
+                                 nullptr, Cm, ldc, c.stream));
 }
 
 // Y (nb rows) = alpha * X (na rows)^T-combination: Y_j = alpha sum_i W(i,j) X_i + beta Y_j

@@ -46,8 +46,11 @@ struct StickPlanDev {
   const double* tw3;
 };
 
+struct TplKernels;
+
 struct StickPlan {
   StickPlanDev dev{};
+  const TplKernels* tpl = nullptr;   // size-specialised kernels for this grid, if compiled
   std::vector<int> col_off, col_kmin, col_pos, pt_col, pt_k;
   std::vector<int> col_i, col_j;   // stored columns' Miller (i, j)
   std::vector<int> perm;           // stick position -> caller index (0-based)

@@ -13,9 +13,11 @@
 // workgroup barriers, so it runs in place.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <map>
 
 #include "wh_fft.h"
+#include "wh_fft_tpl.h"
 
 namespace wh {
 
@@ -600,12 +602,20 @@ void StickPlan::build(int nr1, int nr2, int nr3, int npw, const int* nl, const i
   // the workgroup-level FFTs must fit their register staging
   pick_maxe(static_cast<long>(nrows) * nr2);
   pick_maxe(static_cast<long>(2 * kCPW) * nr3);
+  // kernels specialised on the grid size, unless WH_FFT_GENERIC is set
+  const char* gen = std::getenv("WH_FFT_GENERIC");
+  tpl = (gen && gen[0] == '1') ? nullptr : tpl_kernels_f64(dev);
 }
 
 void launch_zinv(const StickPlan& p, const FftItem* d_items, int nitems, const z64* psiA, long ldA,
                  const z64* psiB, long ldB, z64* sticks, int mode, hipStream_t s) {
   if (nitems <= 0) return;
   const StickPlanDev& P = p.dev;
+  if (p.tpl) {
+    p.tpl->zinv(P, d_items, nitems, psiA, ldA, psiB, ldB, sticks, mode, s);
+    WH_HIP(hipGetLastError());
+    return;
+  }
   const int maxe = pick_maxe(2L * kCPW * P.n3);
   const size_t smem = sizeof(z64) * (P.n3 + 2 * kCPW * P.n3);
   dim3 grid(ceil_div(P.ncs, kCPW), nitems);
@@ -634,6 +644,11 @@ void launch_plane_vloc(const StickPlan& p, const FftItem* d_items, int nitems, c
                        const double* vpot, long ldpot, double scale, hipStream_t s) {
   if (nitems <= 0) return;
   const StickPlanDev& P = p.dev;
+  if (p.tpl) {
+    p.tpl->vloc(P, d_items, nitems, sin, sout, vpot, ldpot, scale, s);
+    WH_HIP(hipGetLastError());
+    return;
+  }
   const int njc = plane_njc(P);
   const int maxe = pick_maxe(std::max<long>(static_cast<long>(P.nrows) * P.n2, static_cast<long>(njc) * P.n1));
   const size_t smem = plane_smem_bytes(P, njc, false);
@@ -646,6 +661,11 @@ void launch_plane_to_real(const StickPlan& p, const FftItem* d_items, int nitems
                           double* out, long ldout, hipStream_t s) {
   if (nitems <= 0) return;
   const StickPlanDev& P = p.dev;
+  if (p.tpl) {
+    p.tpl->to_real(P, d_items, nitems, sin, out, ldout, s);
+    WH_HIP(hipGetLastError());
+    return;
+  }
   const int njc = plane_njc(P);
   const int maxe = pick_maxe(std::max<long>(static_cast<long>(P.nrows) * P.n2, static_cast<long>(njc) * P.n1));
   const size_t smem = plane_smem_bytes(P, njc, false);
@@ -658,9 +678,15 @@ void launch_plane_density(const StickPlan& p, int npert, int nband, const z64* s
                           z64* sout, hipStream_t s) {
   if (npert <= 0) return;
   const StickPlanDev& P = p.dev;
+  if (p.tpl && p.tpl->density_fits) {
+    p.tpl->density(P, npert, nband, sin, occ, sout, s);
+    WH_HIP(hipGetLastError());
+    return;
+  }
   const int njc = plane_njc(P);
   const int maxe = pick_maxe(std::max<long>(static_cast<long>(P.nrows) * P.n2, static_cast<long>(njc) * P.n1));
   const size_t smem = plane_smem_bytes(P, njc, true);
+  WH_REQUIRE(smem <= 160u * 1024u, "response-density plane does not fit in LDS for this grid in FP64");
   dim3 grid(P.n3, npert);
   WH_PLANE_DISPATCH(k_plane_density, smem, P, nband, sin, occ, sout, njc);
   WH_HIP(hipGetLastError());
@@ -670,6 +696,11 @@ void launch_zfwd(const StickPlan& p, const FftItem* d_items, int nitems, const z
                  long ldin, z64* out, long ldout, const ZfwdArgs& args, hipStream_t s) {
   if (nitems <= 0) return;
   const StickPlanDev& P = p.dev;
+  if (p.tpl) {
+    p.tpl->zfwd(P, d_items, nitems, sticks, in, ldin, out, ldout, args, s);
+    WH_HIP(hipGetLastError());
+    return;
+  }
   const int maxe = pick_maxe(2L * kCPW * P.n3);
   const size_t smem = sizeof(z64) * (P.n3 + 2 * kCPW * P.n3);
   dim3 grid(ceil_div(P.ncs, kCPW), nitems);

@@ -1,0 +1,22 @@
+// Compile-time-size FFT kernels for cubic grids (declarations).
+#pragma once
+
+#include "wh_fft.h"
+
+namespace wh {
+
+struct TplKernels {
+  void (*zinv)(const StickPlanDev&, const FftItem*, int, const z64*, long, const z64*, long, void*, int, hipStream_t);
+  void (*zfwd)(const StickPlanDev&, const FftItem*, int, const void*, const z64*, long, z64*, long, const ZfwdArgs&,
+               hipStream_t);
+  void (*vloc)(const StickPlanDev&, const FftItem*, int, const void*, void*, const double*, long, double, hipStream_t);
+  void (*to_real)(const StickPlanDev&, const FftItem*, int, const void*, double*, long, hipStream_t);
+  void (*density)(const StickPlanDev&, int, int, const void*, const double*, void*, hipStream_t);
+  bool density_fits;
+};
+
+// kernels specialised on N = nr1 = nr2 = nr3, or null when the grid is not
+// one of the compiled sizes (the runtime-size kernels are used then)
+const TplKernels* tpl_kernels_f64(const StickPlanDev& P);
+
+}  // namespace wh

@@ -1,0 +1,688 @@
+// Gamma-point FFT kernels with compile-time line length (cubic grids).
+//
+// Same algorithm and data layout as the runtime-size kernels of wh_fft.hip
+// (stick layout, zinv -> plane -> zfwd), specialised on N = nr1 = nr2 = nr3:
+//  * each 1D transform is a Stockham sequence of large radices chosen at
+//    compile time (N = 72 = 9 x 8: two LDS passes instead of four), with
+//    composite radices 6, 8, 9, 10, 12 done in registers as two-level
+//    Cooley-Tukey butterflies with constant twiddles;
+//  * all index arithmetic is by compile-time constants;
+//  * in the plane kernel the zero padding of the a1 transform is folded into
+//    the first pass (loads from the |i| <= imax rows), the real-space
+//    potential multiply into the last inverse pass, and the |i| <= imax
+//    selection into the last forward pass, so the padded lines are never
+//    materialised and no separate scatter/gather sweeps remain.
+#include <map>
+
+#include "wh_fft.h"
+#include "wh_fft_tpl.h"
+
+namespace wh {
+
+namespace {
+
+constexpr int NT = 256;
+constexpr int CPW = 8;
+
+// ---------------------------------------------------------------------------
+// compile-time trigonometry (Taylor series after reduction to [-pi/4, pi/4])
+// ---------------------------------------------------------------------------
+constexpr long double kPi = 3.141592653589793238462643383279502884L;
+
+constexpr long double taylor_sin(long double x) {
+  long double t = x, s = x;
+  for (int k = 1; k < 14; ++k) {
+    t *= -x * x / ((2 * k) * (2 * k + 1));
+    s += t;
+  }
+  return s;
+}
+constexpr long double taylor_cos(long double x) {
+  long double t = 1.0L, s = 1.0L;
+  for (int k = 1; k < 14; ++k) {
+    t *= -x * x / ((2 * k - 1) * (2 * k));
+    s += t;
+  }
+  return s;
+}
+// cos and sin of 2 pi m / n for integers
+constexpr long double cos2pi(int m, int n) {
+  m %= n;
+  if (m < 0) m += n;
+  // reduce by octants: angle = 2 pi m / n
+  const long double a = 2.0L * kPi * m / n;
+  if (8 * m <= n) return taylor_cos(a);
+  if (8 * m <= 3 * n) return taylor_sin(kPi / 2 - a);
+  if (8 * m <= 5 * n) return -taylor_cos(kPi - a);
+  if (8 * m <= 7 * n) return taylor_sin(a - 3 * kPi / 2);
+  return taylor_cos(2 * kPi - a);
+}
+// sin(2 pi m / n) = cos(2 pi (4m - n) / (4n))
+constexpr long double sin2pi(int m, int n) { return cos2pi(4 * m - n, 4 * n); }
+
+// (long double is the 64-bit double on the device target: a few ulp of 1)
+constexpr bool near(long double a, long double b) { return (a - b) < 5e-16L && (b - a) < 5e-16L; }
+static_assert(near(cos2pi(1, 4), 0.0L) && near(cos2pi(1, 3), -0.5L) && near(cos2pi(1, 6), 0.5L), "cos");
+static_assert(near(cos2pi(1, 8), 0.70710678118654752440L) && near(cos2pi(3, 8), -0.70710678118654752440L), "cos");
+static_assert(near(cos2pi(5, 8), -0.70710678118654752440L) && near(cos2pi(7, 8), 0.70710678118654752440L), "cos");
+static_assert(near(sin2pi(1, 3), 0.86602540378443864676L) && near(sin2pi(2, 3), -0.86602540378443864676L), "sin");
+static_assert(near(sin2pi(5, 6), -0.86602540378443864676L) && near(sin2pi(1, 12), 0.5L), "sin");
+static_assert(near(sin2pi(7, 12), -0.5L) && near(cos2pi(2, 5), -0.80901699437494742410L), "sin/cos");
+
+// ---------------------------------------------------------------------------
+// small DFTs in registers; dir = -1: exp(-i), +1: exp(+i)
+// ---------------------------------------------------------------------------
+template <typename T, int R>
+__device__ __forceinline__ void dft(cplx<T>* v, int dir);
+
+template <typename T>
+__device__ __forceinline__ void dft2(cplx<T>* v) {
+  const cplx<T> a = v[0] + v[1];
+  v[1] = v[0] - v[1];
+  v[0] = a;
+}
+template <typename T>
+__device__ __forceinline__ void dft3(cplx<T>* v, int dir) {
+  const cplx<T> t1 = v[1] + v[2];
+  const cplx<T> t2 = v[1] - v[2];
+  const cplx<T> m = v[0] + cscale(t1, T(-0.5));
+  const cplx<T> u = mul_i(cscale(t2, T(0.86602540378443864676)), dir);
+  v[0] = v[0] + t1;
+  v[1] = m + u;
+  v[2] = m - u;
+}
+template <typename T>
+__device__ __forceinline__ void dft4(cplx<T>* v, int dir) {
+  const cplx<T> t0 = v[0] + v[2];
+  const cplx<T> t1 = v[0] - v[2];
+  const cplx<T> t2 = v[1] + v[3];
+  const cplx<T> t3 = v[1] - v[3];
+  const cplx<T> u = mul_i(t3, dir);
+  v[0] = t0 + t2;
+  v[2] = t0 - t2;
+  v[1] = t1 + u;
+  v[3] = t1 - u;
+}
+template <typename T>
+__device__ __forceinline__ void dft5(cplx<T>* v, int dir) {
+  constexpr T c1 = T(0.30901699437494742410), c2 = T(-0.80901699437494742410);
+  constexpr T s1 = T(0.95105651629515357212), s2 = T(0.58778525229247312917);
+  const cplx<T> t1 = v[1] + v[4];
+  const cplx<T> t2 = v[2] + v[3];
+  const cplx<T> t3 = v[1] - v[4];
+  const cplx<T> t4 = v[2] - v[3];
+  const cplx<T> m1 = v[0] + cscale(t1, c1) + cscale(t2, c2);
+  const cplx<T> m2 = v[0] + cscale(t1, c2) + cscale(t2, c1);
+  const cplx<T> u1 = mul_i(cscale(t3, s1) + cscale(t4, s2), dir);
+  const cplx<T> u2 = mul_i(cscale(t3, s2) - cscale(t4, s1), dir);
+  v[0] = v[0] + t1 + t2;
+  v[1] = m1 + u1;
+  v[4] = m1 - u1;
+  v[2] = m2 + u2;
+  v[3] = m2 - u2;
+}
+
+// two-level Cooley-Tukey: R = A*B, j = j1 + A j2, k = k2 + B k1
+template <typename T, int A, int B>
+__device__ __forceinline__ void dft_ct(cplx<T>* v, int dir) {
+  constexpr int R = A * B;
+  cplx<T> y[A][B];
+#pragma unroll
+  for (int j1 = 0; j1 < A; ++j1)
+#pragma unroll
+    for (int j2 = 0; j2 < B; ++j2) y[j1][j2] = v[j1 + A * j2];
+#pragma unroll
+  for (int j1 = 0; j1 < A; ++j1) dft<T, B>(y[j1], dir);
+#pragma unroll
+  for (int j1 = 1; j1 < A; ++j1)
+#pragma unroll
+    for (int k2 = 1; k2 < B; ++k2) {
+      const T c = T(cos2pi(j1 * k2, R));
+      const T s = T(sin2pi(j1 * k2, R));
+      // exp(dir * 2 pi i j1 k2 / R)
+      y[j1][k2] = cmul(y[j1][k2], mk<T>(c, dir > 0 ? s : -s));
+    }
+#pragma unroll
+  for (int k2 = 0; k2 < B; ++k2) {
+    cplx<T> col[A];
+#pragma unroll
+    for (int j1 = 0; j1 < A; ++j1) col[j1] = y[j1][k2];
+    dft<T, A>(col, dir);
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) v[k2 + B * k1] = col[k1];
+  }
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void dft(cplx<T>* v, int dir) {
+  if constexpr (R == 2) dft2(v);
+  else if constexpr (R == 3) dft3(v, dir);
+  else if constexpr (R == 4) dft4(v, dir);
+  else if constexpr (R == 5) dft5(v, dir);
+  else if constexpr (R == 6) dft_ct<T, 2, 3>(v, dir);
+  else if constexpr (R == 8) dft_ct<T, 2, 4>(v, dir);
+  else if constexpr (R == 9) dft_ct<T, 3, 3>(v, dir);
+  else if constexpr (R == 10) dft_ct<T, 2, 5>(v, dir);
+  else {
+    static_assert(R == 12, "unsupported radix");
+    dft_ct<T, 4, 3>(v, dir);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// compile-time radix plan
+// ---------------------------------------------------------------------------
+struct RadixPlan {
+  int n;
+  int r[8];
+};
+
+constexpr RadixPlan make_plan(int N) {
+  RadixPlan p{0, {0, 0, 0, 0, 0, 0, 0, 0}};
+  constexpr int pref[9] = {9, 8, 12, 10, 6, 5, 4, 3, 2};
+  int m = N;
+  while (m > 1 && p.n < 8) {
+    int pick = 0;
+    for (int i = 0; i < 9; ++i)
+      if (m % pref[i] == 0) {
+        pick = pref[i];
+        break;
+      }
+    if (pick == 0) return RadixPlan{-1, {0, 0, 0, 0, 0, 0, 0, 0}};
+    p.r[p.n++] = pick;
+    m /= pick;
+  }
+  return p;
+}
+
+template <int N>
+struct Plan {
+  static constexpr RadixPlan P = make_plan(N);
+  static_assert(P.n > 0, "N must factor into 2, 3, 5");
+  static constexpr int nst = P.n;
+  static constexpr int radix(int s) { return P.r[s]; }
+  static constexpr int stride(int s) {   // product of the radices of passes before s
+    int p = 1;
+    for (int i = 0; i < s; ++i) p *= P.r[i];
+    return p;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// one Stockham pass over nlines lines of length N: loads through ld(line, idx),
+// stores through st(line, idx, value); a barrier separates the two phases and
+// follows the stores.  MAXL bounds nlines (register staging).
+// ---------------------------------------------------------------------------
+template <typename T, int N, int S, int MAXL, class Ld, class St>
+__device__ __forceinline__ void pass(int nlines, const cplx<T>* tw, int dir, Ld ld, St st) {
+  constexpr int R = Plan<N>::radix(S);
+  constexpr int P = Plan<N>::stride(S);
+  constexpr int NBL = N / R;
+  constexpr int NB = (MAXL * NBL + NT - 1) / NT;
+  constexpr int STEP = N / (P * R);
+  const int nbf = nlines * NBL;
+  cplx<T> v[NB][R];
+#pragma unroll
+  for (int t = 0; t < NB; ++t) {
+    const int bf = threadIdx.x + t * NT;
+    if (bf < nbf) {
+      const int line = bf / NBL;
+      const int i = bf - line * NBL;
+#pragma unroll
+      for (int j = 0; j < R; ++j) v[t][j] = ld(line, i + j * NBL);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < NB; ++t) {
+    const int bf = threadIdx.x + t * NT;
+    if (bf < nbf) {
+      const int line = bf / NBL;
+      const int i = bf - line * NBL;
+      const int k = (P == 1) ? 0 : i % P;
+      if (P > 1 && k != 0) {
+#pragma unroll
+        for (int j = 1; j < R; ++j) {
+          cplx<T> w = tw[j * k * STEP];
+          if (dir > 0) w.y = -w.y;
+          v[t][j] = cmul(v[t][j], w);
+        }
+      }
+      dft<T, R>(v[t], dir);
+      const int base = (i - k) * R + k;
+#pragma unroll
+      for (int j = 0; j < R; ++j) st(line, base + j * P, v[t][j]);
+    }
+  }
+  __syncthreads();
+}
+
+// full transform: pass 0 loads through first(), the last pass stores through
+// last(), the passes in between work in place on buf (line stride N)
+template <typename T, int N, int S, int MAXL, class First, class Last>
+__device__ __forceinline__ void passes(int nlines, cplx<T>* buf, const cplx<T>* tw, int dir, First first, Last last) {
+  constexpr int NS = Plan<N>::nst;
+  auto in_buf = [buf](int line, int idx) { return buf[line * N + idx]; };
+  auto to_buf = [buf](int line, int idx, cplx<T> v) { buf[line * N + idx] = v; };
+  if constexpr (NS == 1) {
+    pass<T, N, 0, MAXL>(nlines, tw, dir, first, last);
+  } else if constexpr (S == 0) {
+    pass<T, N, 0, MAXL>(nlines, tw, dir, first, to_buf);
+    passes<T, N, 1, MAXL>(nlines, buf, tw, dir, first, last);
+  } else if constexpr (S == NS - 1) {
+    pass<T, N, S, MAXL>(nlines, tw, dir, in_buf, last);
+  } else {
+    pass<T, N, S, MAXL>(nlines, tw, dir, in_buf, to_buf);
+    passes<T, N, S + 1, MAXL>(nlines, buf, tw, dir, first, last);
+  }
+}
+
+template <typename T, int N, int MAXL>
+__device__ __forceinline__ void fft_inplace(int nlines, cplx<T>* buf, const cplx<T>* tw, int dir) {
+  auto in_buf = [buf](int line, int idx) { return buf[line * N + idx]; };
+  auto to_buf = [buf](int line, int idx, cplx<T> v) { buf[line * N + idx] = v; };
+  passes<T, N, 0, MAXL>(nlines, buf, tw, dir, in_buf, to_buf);
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void load_tw(cplx<T>* tw, const double* g) {
+  for (int t = threadIdx.x; t < N; t += NT) tw[t] = mk<T>(T(g[2 * t]), T(g[2 * t + 1]));
+}
+
+// maximum number of |i| <= imax rows: the wave-function sphere spans at most
+// about half of the grid along a1
+template <int N>
+constexpr int max_rows() {
+  return N / 2 + 2;
+}
+// x-line chunk of the plane kernel (LDS budget 160 KB)
+template <typename T, int N>
+constexpr int x_chunk(bool density) {
+  const int budget = 160 * 1024 - 2048;
+  const int fixed = static_cast<int>(sizeof(cplx<T>)) * (N + max_rows<N>() * N) +
+                    (density ? static_cast<int>(sizeof(T)) * N * N : 0);
+  int jc = (budget - fixed) / (static_cast<int>(sizeof(cplx<T>)) * N);
+  if (jc > N) jc = N;
+  if (jc < 1) return 1;
+  const int nchunk = (N + jc - 1) / jc;   // equal chunks
+  return (N + nchunk - 1) / nchunk;
+}
+
+// ---------------------------------------------------------------------------
+// sphere -> sticks
+// ---------------------------------------------------------------------------
+template <typename T, int N>
+__global__ __launch_bounds__(NT) void k_zinv_t(StickPlanDev P, const FftItem* items, const z64* psiA, long ldA,
+                                               const z64* psiB, long ldB, cplx<T>* sticks, int mode) {
+  __shared__ cplx<T> tw[N];
+  __shared__ cplx<T> lines[2 * CPW * N];
+  const int it = blockIdx.y;
+  const int c0 = blockIdx.x * CPW;
+  const int c1 = min(c0 + CPW, P.ncs);
+  const int ncl = c1 - c0;
+  const FftItem item = items[it];
+  const z64* A = psiA + static_cast<long>(item.a) * ldA;
+  const z64* B = item.b >= 0 ? psiB + static_cast<long>(item.b) * ldB : nullptr;
+  load_tw<T, N>(tw, P.tw3);
+  for (int t = threadIdx.x; t < 2 * ncl * N; t += NT) lines[t] = mk<T>(T(0), T(0));
+  __syncthreads();
+  const int s0 = P.col_off[c0];
+  const int s1 = P.col_off[c1];
+  for (int s = s0 + threadIdx.x; s < s1; s += NT) {
+    const int lc = P.pt_col[s] - c0;
+    int z = P.pt_k[s] % N;
+    if (z < 0) z += N;
+    const z64 a1 = A[s];
+    const z64 a2 = B ? B[s] : mk(0.0, 0.0);
+    lines[(2 * lc) * N + z] = mk<T>(T(a1.x - a2.y), T(a1.y + a2.x));
+  }
+  __syncthreads();
+  for (int s = s0 + threadIdx.x; s < s1; s += NT) {
+    const int lc = P.pt_col[s] - c0;
+    int zm = (-P.pt_k[s]) % N;
+    if (zm < 0) zm += N;
+    const z64 a1 = A[s];
+    const z64 a2 = B ? B[s] : mk(0.0, 0.0);
+    z64 v = mk(a1.x + a2.y, a2.x - a1.y);
+    if (mode == ZINV_PAIR_REALG0 && s == 0) v = mk(a1.x, a2.x);
+    const int line = (c0 + lc == 0) ? 0 : 2 * lc + 1;
+    lines[line * N + zm] = mk<T>(T(v.x), T(v.y));
+  }
+  __syncthreads();
+  fft_inplace<T, N, 2 * CPW>(2 * ncl, lines, tw, +1);
+  cplx<T>* out = sticks + static_cast<long>(it) * N * P.nct;
+  for (int t = threadIdx.x; t < N * 2 * ncl; t += NT) {
+    const int z = t / (2 * ncl);
+    const int r = t - z * 2 * ncl;
+    const bool own = r < ncl;
+    const int lc = own ? r : r - ncl;
+    const int c = c0 + lc;
+    if (!own && c == 0) continue;
+    const int col = own ? c : c + P.ncs - 1;
+    out[static_cast<long>(z) * P.nct + col] = lines[(2 * lc + (own ? 0 : 1)) * N + z];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// sticks -> sphere
+// ---------------------------------------------------------------------------
+template <typename T, int N>
+__global__ __launch_bounds__(NT) void k_zfwd_t(StickPlanDev P, const FftItem* items, const cplx<T>* sticks,
+                                               const z64* in, long ldin, z64* out, long ldout, ZfwdArgs A) {
+  __shared__ cplx<T> tw[N];
+  __shared__ cplx<T> lines[2 * CPW * N];
+  const int it = blockIdx.y;
+  const int c0 = blockIdx.x * CPW;
+  const int c1 = min(c0 + CPW, P.ncs);
+  const int ncl = c1 - c0;
+  const FftItem item = items[it];
+  load_tw<T, N>(tw, P.tw3);
+  const cplx<T>* src = sticks + static_cast<long>(it) * N * P.nct;
+  const bool single = A.mode == ZFWD_SINGLE;
+  const int nl = single ? ncl : 2 * ncl;   // SINGLE needs no mirror lines
+  for (int t = threadIdx.x; t < N * nl; t += NT) {
+    const int z = t / nl;
+    const int r = t - z * nl;
+    const bool own = r < ncl;
+    const int lc = own ? r : r - ncl;
+    const int c = c0 + lc;
+    cplx<T> v = mk<T>(T(0), T(0));
+    if (own || c != 0) {
+      const int col = own ? c : c + P.ncs - 1;
+      v = src[static_cast<long>(z) * P.nct + col];
+    }
+    lines[(single ? lc : 2 * lc + (own ? 0 : 1)) * N + z] = v;
+  }
+  __syncthreads();
+  fft_inplace<T, N, 2 * CPW>(nl, lines, tw, -1);
+  const int s0 = P.col_off[c0];
+  const int s1 = P.col_off[c1];
+  z64* oa = out + static_cast<long>(item.a) * ldout;
+  z64* ob = item.b >= 0 ? out + static_cast<long>(item.b) * ldout : nullptr;
+  for (int s = s0 + threadIdx.x; s < s1; s += NT) {
+    const int lc = P.pt_col[s] - c0;
+    const int k = P.pt_k[s];
+    int z = k % N;
+    if (z < 0) z += N;
+    if (single) {
+      const cplx<T> F = lines[lc * N + z];
+      const double w = A.scale * (A.weight ? A.weight[s] : 1.0);
+      z64 v = mk(static_cast<double>(F.x) * w, static_cast<double>(F.y) * w);
+      if (A.zero_g0 && s == 0) v = mk(0.0, 0.0);
+      oa[s] = v;
+      continue;
+    }
+    int zm = (-k) % N;
+    if (zm < 0) zm += N;
+    const cplx<T> Ft = lines[(2 * lc) * N + z];
+    const int ml = (c0 + lc == 0) ? 0 : 2 * lc + 1;
+    const cplx<T> Fmt = lines[ml * N + zm];
+    const z64 F = mk(static_cast<double>(Ft.x), static_cast<double>(Ft.y));
+    const z64 Fm = mk(static_cast<double>(Fmt.x), static_cast<double>(Fmt.y));
+    const z64 fp = cscale(F + Fm, 0.5);
+    const z64 fm = cscale(F - Fm, 0.5);
+    const z64 b1 = mk(fp.x, fm.y);
+    const z64 b2 = mk(fp.y, -fm.x);
+    if (A.mode == ZFWD_PAIR_STORE) {
+      oa[s] = cscale(b1, A.scale);
+      if (ob) ob[s] = cscale(b2, A.scale);
+    } else {
+      const double g2 = A.g2kin[s];
+      const z64 ia = in[static_cast<long>(item.a) * ldin + s];
+      oa[s] = b1 + cscale(ia, g2 - A.e[item.a]);
+      if (ob) {
+        const z64 ib = in[static_cast<long>(item.b) * ldin + s];
+        ob[s] = b2 + cscale(ib, g2 - A.e[item.b]);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// per-plane 2D part
+// ---------------------------------------------------------------------------
+template <typename T, int N>
+struct PlaneLds {
+  cplx<T>* tw;
+  cplx<T>* pl;   // max_rows x N
+  cplx<T>* x;    // JC x N
+};
+
+template <typename T, int N>
+__device__ __forceinline__ PlaneLds<T, N> plane_lds() {
+  extern __shared__ char smem_raw[];
+  PlaneLds<T, N> s;
+  s.tw = reinterpret_cast<cplx<T>*>(smem_raw);
+  s.pl = s.tw + N;
+  s.x = s.pl + max_rows<N>() * N;
+  return s;
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void plane_fill(const StickPlanDev& P, PlaneLds<T, N>& S, const cplx<T>* src) {
+  for (int t = threadIdx.x; t < P.nrows * N; t += NT) S.pl[t] = mk<T>(T(0), T(0));
+  __syncthreads();
+  for (int c = threadIdx.x; c < P.nct; c += NT) S.pl[P.col_pos[c]] = src[c];
+  __syncthreads();
+}
+
+// row of the sphere holding a1-frequency x, or -1
+__device__ __forceinline__ int row_of(int x, int N, int imax) {
+  if (x <= imax) return x + imax;
+  if (x >= N - imax) return x - N + imax;
+  return -1;
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(NT) void k_plane_vloc_t(StickPlanDev P, const FftItem* items, const cplx<T>* sin,
+                                                     cplx<T>* sout, const double* vpot, long ldpot, double scale) {
+  constexpr int JC = x_chunk<T, N>(false);
+  constexpr int MR = max_rows<N>();
+  PlaneLds<T, N> S = plane_lds<T, N>();
+  const int z = blockIdx.x;
+  const int it = blockIdx.y;
+  const FftItem item = items[it];
+  const long sl = static_cast<long>(N) * P.nct;
+  const int imax = P.imax;
+  load_tw<T, N>(S.tw, P.tw1);
+  plane_fill<T, N>(P, S, sin + static_cast<long>(item.a) * sl + static_cast<long>(z) * P.nct);
+  fft_inplace<T, N, MR>(P.nrows, S.pl, S.tw, +1);
+  const double* V = vpot + (item.pot >= 0 ? static_cast<long>(item.pot) * ldpot : 0L) + static_cast<long>(z) * N * N;
+  const T sc = T(scale);
+  cplx<T>* pl = S.pl;
+  cplx<T>* xb = S.x;
+  for (int j0 = 0; j0 < N; j0 += JC) {
+    const int nj = min(JC, N - j0);
+    // inverse along a1: zero-padded rows in, V(r) applied on the way out
+    auto from_rows = [=](int jj, int x) {
+      const int r = row_of(x, N, imax);
+      return r >= 0 ? pl[r * N + j0 + jj] : mk<T>(T(0), T(0));
+    };
+    auto times_v = [=](int jj, int x, cplx<T> v) {
+      xb[jj * N + x] = cscale(v, T(V[(j0 + jj) * N + x]) * sc);
+    };
+    passes<T, N, 0, JC>(nj, xb, S.tw, +1, from_rows, times_v);
+    // forward along a1: only |i| <= imax frequencies go back to the rows
+    auto from_x = [=](int jj, int x) { return xb[jj * N + x]; };
+    auto to_rows = [=](int jj, int x, cplx<T> v) {
+      const int r = row_of(x, N, imax);
+      if (r >= 0) pl[r * N + j0 + jj] = v;
+    };
+    passes<T, N, 0, JC>(nj, xb, S.tw, -1, from_x, to_rows);
+  }
+  fft_inplace<T, N, MR>(P.nrows, S.pl, S.tw, -1);
+  cplx<T>* dst = sout + static_cast<long>(it) * sl + static_cast<long>(z) * P.nct;
+  for (int c = threadIdx.x; c < P.nct; c += NT) dst[c] = S.pl[P.col_pos[c]];
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(NT) void k_plane_to_real_t(StickPlanDev P, const FftItem* items, const cplx<T>* sin,
+                                                        double* out, long ldout) {
+  constexpr int JC = x_chunk<T, N>(false);
+  constexpr int MR = max_rows<N>();
+  PlaneLds<T, N> S = plane_lds<T, N>();
+  const int z = blockIdx.x;
+  const int it = blockIdx.y;
+  const FftItem item = items[it];
+  const long sl = static_cast<long>(N) * P.nct;
+  const int imax = P.imax;
+  load_tw<T, N>(S.tw, P.tw1);
+  plane_fill<T, N>(P, S, sin + static_cast<long>(it) * sl + static_cast<long>(z) * P.nct);
+  fft_inplace<T, N, MR>(P.nrows, S.pl, S.tw, +1);
+  double* oa = out + static_cast<long>(item.a) * ldout + static_cast<long>(z) * N * N;
+  double* ob = item.b >= 0 ? out + static_cast<long>(item.b) * ldout + static_cast<long>(z) * N * N : nullptr;
+  cplx<T>* pl = S.pl;
+  for (int j0 = 0; j0 < N; j0 += JC) {
+    const int nj = min(JC, N - j0);
+    auto from_rows = [=](int jj, int x) {
+      const int r = row_of(x, N, imax);
+      return r >= 0 ? pl[r * N + j0 + jj] : mk<T>(T(0), T(0));
+    };
+    auto to_out = [=](int jj, int x, cplx<T> v) {
+      oa[(j0 + jj) * N + x] = static_cast<double>(v.x);
+      if (ob) ob[(j0 + jj) * N + x] = static_cast<double>(v.y);
+    };
+    passes<T, N, 0, JC>(nj, S.x, S.tw, +1, from_rows, to_out);
+  }
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(NT) void k_plane_density_t(StickPlanDev P, int nband, const cplx<T>* sin,
+                                                        const double* occ, cplx<T>* sout) {
+  constexpr int JC = x_chunk<T, N>(true);
+  constexpr int MR = max_rows<N>();
+  PlaneLds<T, N> S = plane_lds<T, N>();
+  T* acc = reinterpret_cast<T*>(S.x + JC * N);
+  const int z = blockIdx.x;
+  const int p = blockIdx.y;
+  const long sl = static_cast<long>(N) * P.nct;
+  const int imax = P.imax;
+  load_tw<T, N>(S.tw, P.tw1);
+  for (int t = threadIdx.x; t < N * N; t += NT) acc[t] = T(0);
+  cplx<T>* pl = S.pl;
+  for (int v = 0; v < nband; ++v) {
+    plane_fill<T, N>(P, S, sin + (static_cast<long>(p) * nband + v) * sl + static_cast<long>(z) * P.nct);
+    fft_inplace<T, N, MR>(P.nrows, S.pl, S.tw, +1);
+    const T o = T(occ[v]);
+    for (int j0 = 0; j0 < N; j0 += JC) {
+      const int nj = min(JC, N - j0);
+      auto from_rows = [=](int jj, int x) {
+        const int r = row_of(x, N, imax);
+        return r >= 0 ? pl[r * N + j0 + jj] : mk<T>(T(0), T(0));
+      };
+      // each (jj, x) is produced by exactly one thread: no race on acc
+      auto accumulate = [=](int jj, int x, cplx<T> w) { acc[(j0 + jj) * N + x] += o * w.x * w.y; };
+      passes<T, N, 0, JC>(nj, S.x, S.tw, +1, from_rows, accumulate);
+    }
+  }
+  // forward of the real density: a1 then a2, keeping |i| <= imax and the columns
+  for (int j0 = 0; j0 < N; j0 += JC) {
+    const int nj = min(JC, N - j0);
+    auto from_acc = [=](int jj, int x) { return mk<T>(acc[(j0 + jj) * N + x], T(0)); };
+    auto to_rows = [=](int jj, int x, cplx<T> w) {
+      const int r = row_of(x, N, imax);
+      if (r >= 0) pl[r * N + j0 + jj] = w;
+    };
+    passes<T, N, 0, JC>(nj, S.x, S.tw, -1, from_acc, to_rows);
+  }
+  fft_inplace<T, N, MR>(P.nrows, S.pl, S.tw, -1);
+  cplx<T>* dst = sout + static_cast<long>(p) * sl + static_cast<long>(z) * P.nct;
+  for (int c = threadIdx.x; c < P.nct; c += NT) dst[c] = S.pl[P.col_pos[c]];
+}
+
+// ---------------------------------------------------------------------------
+// dispatch
+// ---------------------------------------------------------------------------
+template <typename T, int N>
+size_t plane_smem(bool density) {
+  const int jc = x_chunk<T, N>(density);
+  size_t b = sizeof(cplx<T>) * (static_cast<size_t>(N) + static_cast<size_t>(max_rows<N>()) * N +
+                                static_cast<size_t>(jc) * N);
+  if (density) b += sizeof(T) * static_cast<size_t>(N) * N;
+  return b;
+}
+
+template <class K>
+void allow_smem(K kernel, size_t bytes) {
+  static std::map<const void*, size_t> done;
+  auto key = reinterpret_cast<const void*>(kernel);
+  auto itr = done.find(key);
+  if (itr != done.end() && itr->second >= bytes) return;
+  WH_HIP(hipFuncSetAttribute(key, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
+  done[key] = bytes;
+}
+
+template <typename T, int N>
+struct Launch {
+  static void zinv(const StickPlanDev& P, const FftItem* it, int n, const z64* a, long lda, const z64* b, long ldb,
+                   void* st, int mode, hipStream_t s) {
+    k_zinv_t<T, N><<<dim3(ceil_div(P.ncs, CPW), n), NT, 0, s>>>(P, it, a, lda, b, ldb, static_cast<cplx<T>*>(st), mode);
+  }
+  static void zfwd(const StickPlanDev& P, const FftItem* it, int n, const void* st, const z64* in, long ldin, z64* out,
+                   long ldout, const ZfwdArgs& A, hipStream_t s) {
+    k_zfwd_t<T, N><<<dim3(ceil_div(P.ncs, CPW), n), NT, 0, s>>>(P, it, static_cast<const cplx<T>*>(st), in, ldin, out,
+                                                                 ldout, A);
+  }
+  static void vloc(const StickPlanDev& P, const FftItem* it, int n, const void* si, void* so, const double* v, long ldv,
+                   double scale, hipStream_t s) {
+    const size_t sm = plane_smem<T, N>(false);
+    allow_smem(k_plane_vloc_t<T, N>, sm);
+    k_plane_vloc_t<T, N><<<dim3(N, n), NT, sm, s>>>(P, it, static_cast<const cplx<T>*>(si), static_cast<cplx<T>*>(so),
+                                                     v, ldv, scale);
+  }
+  static void to_real(const StickPlanDev& P, const FftItem* it, int n, const void* si, double* out, long ldout,
+                      hipStream_t s) {
+    const size_t sm = plane_smem<T, N>(false);
+    allow_smem(k_plane_to_real_t<T, N>, sm);
+    k_plane_to_real_t<T, N><<<dim3(N, n), NT, sm, s>>>(P, it, static_cast<const cplx<T>*>(si), out, ldout);
+  }
+  static void density(const StickPlanDev& P, int np, int nb, const void* si, const double* occ, void* so,
+                      hipStream_t s) {
+    const size_t sm = plane_smem<T, N>(true);
+    allow_smem(k_plane_density_t<T, N>, sm);
+    k_plane_density_t<T, N><<<dim3(N, np), NT, sm, s>>>(P, nb, static_cast<const cplx<T>*>(si), occ,
+                                                        static_cast<cplx<T>*>(so));
+  }
+};
+
+template <typename T>
+const TplKernels* table_for(int n) {
+#define WH_TPL_CASE(NN)                                                                                       \
+  case NN: {                                                                                                  \
+    static const TplKernels k{&Launch<T, NN>::zinv, &Launch<T, NN>::zfwd, &Launch<T, NN>::vloc,             \
+                              &Launch<T, NN>::to_real, &Launch<T, NN>::density, plane_smem<T, NN>(true) <= 160u * 1024u}; \
+    return &k;                                                                                                \
+  }
+  switch (n) {
+    WH_TPL_CASE(18)
+    WH_TPL_CASE(24)
+    WH_TPL_CASE(30)
+    WH_TPL_CASE(32)
+    WH_TPL_CASE(36)
+    WH_TPL_CASE(40)
+    WH_TPL_CASE(45)
+    WH_TPL_CASE(48)
+    WH_TPL_CASE(54)
+    WH_TPL_CASE(60)
+    WH_TPL_CASE(64)
+    WH_TPL_CASE(72)
+    WH_TPL_CASE(80)
+    WH_TPL_CASE(90)
+    WH_TPL_CASE(96)
+    WH_TPL_CASE(100)
+    WH_TPL_CASE(108)
+    default:
+      return nullptr;
+  }
+#undef WH_TPL_CASE
+}
+
+}  // namespace
+
+const TplKernels* tpl_kernels_f64(const StickPlanDev& P) {
+  if (P.n1 != P.n2 || P.n2 != P.n3) return nullptr;
+  if (P.nrows > P.n1 / 2 + 2) return nullptr;
+  return table_for<double>(P.n1);
+}
+
+}  // namespace wh

@@ -8,30 +8,104 @@
 // apply_alpha_pc_to_m_wfcs.f90:59, apply_alpha_pv_to_m_wfcs.f90:71) and the
 // Davidson subspace products (Wstat/wstat_tools.f90 build_hr/update/refresh).
 //
-// Tile 64 x 64 x 16 per 256-thread workgroup: four wavefronts in a 2 x 2
-// arrangement, each owning 32 x 32 = 2 x 2 MFMA 16x16 blocks.  Operand
-// layout of the 16x16x4 f64 instruction (checked on the device): lane l
-// holds A[l%16][l/16] and B[l/16][l%16]; result register r of lane l holds
-// D[l/16 + 4r][l%16].
+// Tile 128 x 128 x 16 per 256-thread workgroup: four wavefronts in a 2 x 2
+// arrangement, each owning 64 x 64 = 4 x 4 MFMA 16x16 blocks, with the next
+// K tile prefetched into registers while the current one is multiplied out
+// of a double-buffered LDS tile.  The projection ("bra") products have a
+// small M (occupied bands + projectors) and a long K (2 npw), so they split K
+// over blockIdx.z; the partial products are summed in a fixed order by the
+// epilogue kernel, which also applies the G = 0 correction of glbrak_gamma
+// (its DGER) and the per-row scale, so results do not depend on scheduling.
+//
+// Operand layout of the 16x16x4 f64 instruction (checked on the device):
+// lane l holds A[l%16][l/16] and B[l/16][l%16]; result register r of lane l
+// holds D[l/16 + 4r][l%16].
 #include "wh_gemm.h"
+
+#include <algorithm>
 
 namespace wh {
 
 namespace {
 
-constexpr int BM = 64;
-constexpr int BN = 64;
+constexpr int BM = 128;
+constexpr int BN = 128;
 constexpr int BK = 16;
 constexpr int NT = 256;
+constexpr int PAD = 2;
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 template <bool TA>
-__global__ __launch_bounds__(NT) void k_dgemm(int M, int N, int K, double alpha, const double* __restrict__ A,
-                                              long lda, const double* __restrict__ B, long ldb, double beta,
-                                              double* __restrict__ C, long ldc) {
-  __shared__ double As[BK][BM + 1];
-  __shared__ double Bs[BK][BN + 1];
+__device__ __forceinline__ void load_a(double (&ra)[8], const double* __restrict__ A, long lda, int M, int m0, int k0,
+                                       int kend) {
+  const int tid = threadIdx.x;
+  if (TA) {   // element (k, m) at A[m*lda + k]: 128 m x 16 k, 8 consecutive k per thread
+    const int m = tid >> 1;
+    const int kq = (tid & 1) * 8;
+    const int gm = m0 + m;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int gk = k0 + kq + q;
+      ra[q] = (gm < M && gk < kend) ? A[static_cast<long>(gm) * lda + gk] : 0.0;
+    }
+  } else {    // element (m, k) at A[k*lda + m]: 16 k x 128 m, 8 consecutive m per thread
+    const int k = tid >> 4;
+    const int mq = (tid & 15) * 8;
+    const int gk = k0 + k;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int gm = m0 + mq + q;
+      ra[q] = (gm < M && gk < kend) ? A[static_cast<long>(gk) * lda + gm] : 0.0;
+    }
+  }
+}
+
+template <bool TA>
+__device__ __forceinline__ void store_a(double (*As)[BM + PAD], const double (&ra)[8]) {
+  const int tid = threadIdx.x;
+  if (TA) {
+    const int m = tid >> 1;
+    const int kq = (tid & 1) * 8;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) As[kq + q][m] = ra[q];
+  } else {
+    const int k = tid >> 4;
+    const int mq = (tid & 15) * 8;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) As[k][mq + q] = ra[q];
+  }
+}
+
+__device__ __forceinline__ void load_b(double (&rb)[8], const double* __restrict__ B, long ldb, int N, int n0, int k0,
+                                       int kend) {
+  const int tid = threadIdx.x;
+  const int n = tid >> 1;
+  const int kq = (tid & 1) * 8;
+  const int gn = n0 + n;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int gk = k0 + kq + q;
+    rb[q] = (gn < N && gk < kend) ? B[static_cast<long>(gn) * ldb + gk] : 0.0;
+  }
+}
+
+__device__ __forceinline__ void store_b(double (*Bs)[BN + PAD], const double (&rb)[8]) {
+  const int tid = threadIdx.x;
+  const int n = tid >> 1;
+  const int kq = (tid & 1) * 8;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) Bs[kq + q][n] = rb[q];
+}
+
+// blockIdx.z = split index; k range [kz*kchunk, min(K, (kz+1)*kchunk)).
+// split > 1: raw partial sums go to C + kz*pstride (alpha, beta ignored).
+template <bool TA>
+__global__ __launch_bounds__(NT) void k_dgemm(int M, int N, int K, int kchunk, double alpha,
+                                              const double* __restrict__ A, long lda, const double* __restrict__ B,
+                                              long ldb, double beta, double* __restrict__ C, long ldc, long pstride) {
+  __shared__ double As[2][BK][BM + PAD];
+  __shared__ double Bs[2][BK][BN + PAD];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -39,78 +113,89 @@ __global__ __launch_bounds__(NT) void k_dgemm(int M, int N, int K, double alpha,
   const int wn = wave >> 1;
   const int m0 = blockIdx.x * BM;
   const int n0 = blockIdx.y * BN;
-  v4d acc[2][2];
+  const int kz = blockIdx.z;
+  const int kbeg = kz * kchunk;
+  const int kend = min(K, kbeg + kchunk);
+  v4d acc[4][4];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
-
-  for (int k0 = 0; k0 < K; k0 += BK) {
-    // A tile -> As[k][m]
-    if (TA) {
-      const int m = tid >> 2;
-      const int kq = (tid & 3) * 4;
-      const int gm = m0 + m;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int gk = k0 + kq + q;
-        As[kq + q][m] = (gm < M && gk < K) ? A[static_cast<long>(gm) * lda + gk] : 0.0;
-      }
-    } else {
-      const int k = tid >> 4;
-      const int mq = (tid & 15) * 4;
-      const int gk = k0 + k;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int gm = m0 + mq + q;
-        As[k][mq + q] = (gm < M && gk < K) ? A[static_cast<long>(gk) * lda + gm] : 0.0;
-      }
+    for (int j = 0; j < 4; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+  const int ntile = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
+  double ra[8], rb[8];
+  if (ntile > 0) {
+    load_a<TA>(ra, A, lda, M, m0, kbeg, kend);
+    load_b(rb, B, ldb, N, n0, kbeg, kend);
+    store_a<TA>(As[0], ra);
+    store_b(Bs[0], rb);
+  }
+  __syncthreads();
+  for (int t = 0; t < ntile; ++t) {
+    const int cur = t & 1;
+    if (t + 1 < ntile) {
+      load_a<TA>(ra, A, lda, M, m0, kbeg + (t + 1) * BK, kend);
+      load_b(rb, B, ldb, N, n0, kbeg + (t + 1) * BK, kend);
     }
-    {
-      const int n = tid >> 2;
-      const int kq = (tid & 3) * 4;
-      const int gn = n0 + n;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int gk = k0 + kq + q;
-        Bs[kq + q][n] = (gn < N && gk < K) ? B[static_cast<long>(gn) * ldb + gk] : 0.0;
-      }
-    }
-    __syncthreads();
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       const int kr = kk + (lane >> 4);
-      double a[2], b[2];
+      double a[4], b[4];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) a[i] = As[kr][wm * 32 + i * 16 + (lane & 15)];
+      for (int i = 0; i < 4; ++i) a[i] = As[cur][kr][wm * 64 + i * 16 + (lane & 15)];
 #pragma unroll
-      for (int j = 0; j < 2; ++j) b[j] = Bs[kr][wn * 32 + j * 16 + (lane & 15)];
+      for (int j = 0; j < 4; ++j) b[j] = Bs[cur][kr][wn * 64 + j * 16 + (lane & 15)];
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    if (t + 1 < ntile) {
+      store_a<TA>(As[cur ^ 1], ra);
+      store_b(Bs[cur ^ 1], rb);
     }
     __syncthreads();
   }
+  const bool partial = gridDim.z > 1;
+  double* Cz = C + (partial ? static_cast<long>(kz) * pstride : 0L);
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int col = n0 + wn * 32 + j * 16 + (lane & 15);
+    for (int j = 0; j < 4; ++j) {
+      const int col = n0 + wn * 64 + j * 16 + (lane & 15);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm * 32 + i * 16 + (lane >> 4) + 4 * r;
+        const int row = m0 + wm * 64 + i * 16 + (lane >> 4) + 4 * r;
         if (row < M && col < N) {
-          double* c = C + static_cast<long>(col) * ldc + row;
-          const double v = alpha * acc[i][j][r];
-          *c = (beta == 0.0) ? v : v + beta * (*c);
+          double* c = Cz + static_cast<long>(col) * ldc + row;
+          if (partial) {
+            *c = acc[i][j][r];
+          } else {
+            const double v = alpha * acc[i][j][r];
+            *c = (beta == 0.0) ? v : v + beta * (*c);
+          }
         }
       }
     }
 }
 
-// c(i, j) = s_i * (c(i, j) - g0 * a(1, i) * b(1, j)), column-major c (ldc),
-// a(1, i) = A[i*lda], b(1, j) = B[j*ldb]   (the DGER of glbrak_gamma + row scaling)
+// C(i, j) = s_i * (alpha * sum_z P_z(i, j) - g0 * a(1, i) * b(1, j))
+//   P_z: split partials (ldc, stride pstride); a(1, i) = A[i*lda], b(1, j) = B[j*ldb]
+__global__ void k_reduce_g0(int M, int N, int nsplit, const double* P, long pstride, double alpha, double g0,
+                            const double* A, long lda, const double* B, long ldb, const double* s, double* C,
+                            long ldc) {
+  const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= static_cast<long>(M) * N) return;
+  const int i = static_cast<int>(idx % M);
+  const int j = static_cast<int>(idx / M);
+  const long o = static_cast<long>(j) * ldc + i;
+  double acc = 0.0;
+  for (int z = 0; z < nsplit; ++z) acc += P[static_cast<long>(z) * pstride + o];
+  double v = alpha * acc - g0 * A[static_cast<long>(i) * lda] * B[static_cast<long>(j) * ldb];
+  if (s) v *= s[i];
+  C[o] = v;
+}
+
+// C(i, j) = scale_i * (C(i, j) - g0 * A[i*lda] * B[j*ldb])   (scale may be null)
 __global__ void k_g0_scale(int M, int N, double g0, const double* A, long lda, const double* B, long ldb,
                            const double* s, double* C, long ldc) {
   const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -122,23 +207,62 @@ __global__ void k_g0_scale(int M, int N, double g0, const double* A, long lda, c
   C[static_cast<long>(j) * ldc + i] = v;
 }
 
+struct Scratch {
+  double* p = nullptr;
+  size_t n = 0;
+  double* get(size_t count) {
+    if (count > n) {
+      if (p) {
+        WH_HIP(hipDeviceSynchronize());
+        WH_HIP(hipFree(p));
+      }
+      p = nullptr;
+      WH_HIP(hipMalloc(&p, sizeof(double) * count));
+      n = count;
+    }
+    return p;
+  }
+};
+
+Scratch& partials() {
+  static Scratch s;
+  return s;
+}
+
 }  // namespace
 
 void dgemm(bool transa, int M, int N, int K, double alpha, const double* A, long lda, const double* B, long ldb,
            double beta, double* C, long ldc, hipStream_t s) {
   if (M <= 0 || N <= 0) return;
-  dim3 grid(ceil_div(M, BM), ceil_div(N, BN));
-  if (K <= 0) {
-    // C = beta*C only: a zero-size product still scales C (BLAS semantics)
-    if (transa)
-      k_dgemm<true><<<grid, NT, 0, s>>>(M, N, 0, alpha, A, lda, B, ldb, beta, C, ldc);
-    else
-      k_dgemm<false><<<grid, NT, 0, s>>>(M, N, 0, alpha, A, lda, B, ldb, beta, C, ldc);
-  } else if (transa) {
-    k_dgemm<true><<<grid, NT, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+  dim3 grid(ceil_div(M, BM), ceil_div(N, BN), 1);
+  const int kc = std::max(K, 1);
+  if (transa)
+    k_dgemm<true><<<grid, NT, 0, s>>>(M, N, std::max(K, 0), kc, alpha, A, lda, B, ldb, beta, C, ldc, 0);
+  else
+    k_dgemm<false><<<grid, NT, 0, s>>>(M, N, std::max(K, 0), kc, alpha, A, lda, B, ldb, beta, C, ldc, 0);
+  WH_HIP(hipGetLastError());
+}
+
+void dgemm_bra(int M, int N, int K, double alpha, const double* A, long lda, const double* B, long ldb, double g0,
+               const double* scale, double* C, long ldc, hipStream_t s) {
+  if (M <= 0 || N <= 0) return;
+  const int tiles = ceil_div(M, BM) * ceil_div(N, BN);
+  // enough workgroups for 256 compute units, each K chunk at least 64 tiles long
+  int split = std::max(1, std::min(16, ceil_div(512, tiles)));
+  split = std::max(1, std::min(split, K / (64 * BK)));
+  const int kchunk = ceil_div(ceil_div(std::max(K, 1), split), BK) * BK;
+  split = ceil_div(std::max(K, 1), kchunk);
+  const long pstride = static_cast<long>(ldc) * N;
+  double* P = partials().get(static_cast<size_t>(split) * pstride);
+  dim3 grid(ceil_div(M, BM), ceil_div(N, BN), split);
+  if (split > 1) {
+    k_dgemm<true><<<grid, NT, 0, s>>>(M, N, K, kchunk, 1.0, A, lda, B, ldb, 0.0, P, ldc, pstride);
   } else {
-    k_dgemm<false><<<grid, NT, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+    k_dgemm<true><<<grid, NT, 0, s>>>(M, N, K, kchunk, 1.0, A, lda, B, ldb, 0.0, P, ldc, pstride);
   }
+  WH_HIP(hipGetLastError());
+  const long n = static_cast<long>(M) * N;
+  k_reduce_g0<<<ceil_div(n, 256), 256, 0, s>>>(M, N, split, P, pstride, alpha, g0, A, lda, B, ldb, scale, C, ldc);
   WH_HIP(hipGetLastError());
 }
 
