@@ -64,6 +64,9 @@ def main():
     ap.add_argument("--scf", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-bands", type=int, default=48)
+    ap.add_argument("--gs-cache", default=None,
+                    help="npz file: load the ground state from it if present, else write it there "
+                         "(lets a profiled run skip the ground-state kernels)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -82,7 +85,13 @@ def main():
     b = system.basis
     log(f"[bench] {system.name}: npw={b.npw} nr={b.nr} nbndval={system.nbnd_occ} nkb={system.nkb}")
     wstat.init(0)
-    gs = make_ground_state(system, wstat.gpu_h_factory, nbnd=system.nbnd_occ + 4, scf_steps=args.scf, tol=1e-7)
+    gs = None
+    if args.gs_cache and Path(args.gs_cache).exists():
+        gs = _load_gs(system, args.gs_cache)
+    if gs is None:
+        gs = make_ground_state(system, wstat.gpu_h_factory, nbnd=system.nbnd_occ + 4, scf_steps=args.scf, tol=1e-7)
+        if args.gs_cache:
+            _save_gs(gs, args.gs_cache)
     log(f"[bench] ground state in {time.perf_counter() - t_setup:.1f}s; "
         f"gap {gs.et[gs.nbndval] - gs.et[gs.nbndval - 1] if len(gs.et) > gs.nbndval else float('nan'):.4f} Ry")
     sq = sqvc_init(b, gs.npwx)
@@ -171,6 +180,24 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(rec), flush=True)
+
+
+def _save_gs(gs, path):
+    np.savez(path, evc=gs.evc, et=gs.et, vrs=gs.vrs, alpha_pv=gs.alpha_pv, nbndval=gs.nbndval)
+
+
+def _load_gs(system, path):
+    from west_amd.groundstate import GroundState
+    from west_amd.systems import kb_projectors
+    d = np.load(path)
+    b = system.basis
+    if d["evc"].shape[1] != b.npw or int(d["nbndval"]) != system.nbnd_occ:
+        return None
+    vkb, dvan = kb_projectors(system)
+    nv = int(d["nbndval"])
+    return GroundState(basis=b, npwx=b.npw, vrs=d["vrs"], vkb=np.ascontiguousarray(vkb.T), dvan=dvan,
+                       evc=d["evc"], et=d["et"], nbndval=nv, occ=np.ones(nv), alpha_pv=float(d["alpha_pv"]),
+                       meta={"system": system.name, "cached": True})
 
 
 def _ncols(b):
