@@ -64,6 +64,11 @@ def main():
     ap.add_argument("--scf", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-bands", type=int, default=48)
+    ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"],
+                    help="fp64: the FP64 solve; mixed: FP32 CG with FP64 refinement; both: the FP64 line "
+                         "with the mixed-precision run of the same batch beside it")
+    ap.add_argument("--no-davidson", action="store_true",
+                    help="skip the Wstat Davidson wall-time run on the 6 H2 workload")
     ap.add_argument("--gs-cache", default=None,
                     help="npz file: load the ground state from it if present, else write it there "
                          "(lets a profiled run skip the ground-state kernels)")
@@ -103,43 +108,39 @@ def main():
     nv = gs.nbndval
     pairs = npert * nv
 
-    for i in range(args.warmup):
-        s0 = wstat.counters()
-        t = time.perf_counter()
-        wstat.apply_operator(npert, dvg, args.tr2, dng=dng)
-        torch.cuda.synchronize()
-        s1 = wstat.counters()
-        log(f"[bench] warmup {i}: {time.perf_counter() - t:.3f}s, stern={s1[0] - s0[0]}, cg_iters={s1[1] - s0[1]}")
-
-    wstat.profile(True)
-    s0 = wstat.counters()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        wstat.apply_operator(npert, dvg, args.tr2, dng=dng)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    s1 = wstat.counters()
-    elapsed = t1 - t0
-    ms_step = 1000.0 * elapsed / args.steps
-    value = pairs * args.steps / elapsed
-    stern_per_step = (s1[0] - s0[0]) / args.steps
-    iters_per_step = (s1[1] - s0[1]) / args.steps
-    kernels = {}
-    for name in ("zinv", "plane_vloc", "zfwd", "plane_density", "plane_to_real", "gemm_bra", "gemm_fix",
-                 "gemm_ket", "cg_prec", "cg_dir", "cg_update", "precondition"):
-        ms, cnt = wstat.profile_query(name)
-        if cnt:
-            kernels[name] = {"ms_per_step": ms / args.steps, "launches": cnt, "avg_ms": ms / cnt}
-    wstat.profile(False)
-    log("[bench] kernel ms/step: " + ", ".join(f"{k}={v['ms_per_step']:.1f}" for k, v in kernels.items()))
+    head = _run_mode(wstat, torch, args, "mixed" if args.precision == "mixed" else "fp64", npert, dvg, dng)
+    value, ms_step = head["value"], head["ms_per_step"]
+    stern_per_step, iters_per_step, kernels = head["stern_per_step"], head["iters_per_step"], head["kernels"]
 
     # ---- roofline of the dominant kernel and the end-to-end figure
     n1, n2, n3 = b.nr
-    model = _byte_model(b, gs, npert, stern_per_step)
+    model = _byte_model(b, gs, npert, head)
     dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
     roof = _roofline(dom, kernels[dom], model, args.steps)
-    e2e = _end_to_end(b, nv, value, stern_per_step / pairs)
+    e2e = _end_to_end(b, nv, value, head)
+
+    # ---- mixed precision beside the FP64 line: same batch, same tr2
+    mixed = None
+    if args.precision == "both":
+        dng64 = dng.clone()
+        dng32 = torch.zeros_like(dng)
+        mx = _run_mode(wstat, torch, args, "mixed", npert, dvg, dng32)
+        mmodel = _byte_model(b, gs, npert, mx)
+        mdom = max(mx["kernels"], key=lambda k: mx["kernels"][k]["ms_per_step"])
+        l2 = float(torch.linalg.norm(dng32 - dng64) / torch.linalg.norm(dng64))
+        mixed = {"value": round(mx["value"], 3), "unit": "pairs/s", "ms_per_step": round(mx["ms_per_step"], 3),
+                 "speedup_vs_fp64": round(mx["value"] / value, 3),
+                 "dtype": "f32 FFT/CG + split-bf16 MFMA projections (f32 accumulate) + f64 refinement",
+                 "refine_delta": 1e-4, "dng_l2_rel_err_vs_fp64": l2,
+                 "sternheimer_applications_per_step": mx["stern_per_step"],
+                 "fp32_applications_per_step": mx["fp32_per_step"],
+                 "fp64_refinements_per_step": mx["refine_per_step"],
+                 "fp64_fallback_bands_per_step": mx["fallback_per_step"],
+                 "cg_iterations_per_step": mx["iters_per_step"],
+                 "kernels_ms_per_step": {k: round(v["ms_per_step"], 3) for k, v in mx["kernels"].items()},
+                 "roofline": _roofline(mdom, mx["kernels"][mdom], mmodel, args.steps),
+                 "roofline_end_to_end": _end_to_end(b, nv, mx["value"], mx)}
+        log(f"[bench] mixed: {mx['value']:.1f} pairs/s ({mixed['speedup_vs_fp64']}x), dng L2 rel err {l2:.2e}")
 
     # ---- CPU baseline: the FP64 oracle on a bounded sample of the same batch
     cpu = None
@@ -157,6 +158,9 @@ def main():
                "sample": f"perturbation 1 of the batch x occupied bands 1..{nb} (of {nv}), tr2={args.tr2:g}, "
                          f"FP64 oracle (SciPy FFT on {threads} threads, OpenBLAS GEMM), {tc:.1f}s"}
 
+    # ---- Wstat Davidson wall time (6 H2 molecules, 32 PDEP, tr2_dfpt 1e-12), FP64 and mixed
+    dav = None if args.no_davidson else _davidson_h2(wstat, torch, args)
+
     rec = {
         "metric": "chi0.dv applies/s (pert x band), Wstat dfpt at tr2_dfpt",
         "value": round(value, 3),
@@ -168,7 +172,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision != "mixed" else "f32+bf16x3, f64 refinement",
         "data": "synthetic (seeded Si supercell ground state computed on the device; seeded smooth perturbations)",
         "config": {"workload": f"{args.workload}: {system.name}, npw={b.npw}, fft={n1}x{n2}x{n3}, "
                                f"nbndval={nv}, nkb={system.nkb}, {npert} perturbations/batch",
@@ -178,8 +182,85 @@ def main():
         "roofline": roof,
         "roofline_end_to_end": e2e,
         "cpu_baseline": cpu,
+        "mixed_precision": mixed,
+        "davidson": dav,
     }
     print(json.dumps(rec), flush=True)
+
+
+def _davidson_h2(wstat, torch, args):
+    """Wstat Davidson on 6 H2 molecules (bond 1.40 bohr, random orientations,
+    >= 5 bohr apart, 18 bohr cell, 25 Ry, 6 occupied bands): 32 PDEP,
+    n_pdep_times 4, tr2_dfpt 1e-12, default trev; wall time of the FP64 and
+    of the mixed-precision solve, and the mixed PDEP eigenvalues against the
+    FP64 ones."""
+    from west_amd.coulomb import sqvc_init
+    from west_amd.groundstate import make_ground_state
+    from west_amd.systems import make_h2_system
+    s = make_h2_system(nmol=6, L=18.0, ecutwfc=25.0, seed=2024)
+    t = time.perf_counter()
+    gs = make_ground_state(s, wstat.gpu_h_factory, nbnd=s.nbnd_occ + 4, scf_steps=args.scf, tol=1e-7)
+    log(f"[bench] {s.name}: npw={s.basis.npw} nr={s.basis.nr}; ground state in {time.perf_counter() - t:.1f}s")
+    wstat.set_ground_state(gs, sqvc=sqvc_init(s.basis, gs.npwx))
+    res = {}
+    for mode in ("fp64", "mixed"):
+        wstat.set_precision(mode)
+        torch.cuda.synchronize()
+        c0 = wstat.counters()
+        t = time.perf_counter()
+        ev, conv, _, recs = wstat.davidson_diago(32, n_pdep_times=4, tr2_dfpt=1e-12, want_vectors=False)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t
+        c1 = wstat.counters()
+        res[mode] = {"wall_s": round(wall, 3), "iterations": len(recs), "converged": int(conv.sum()),
+                     "sternheimer_applications": c1[0] - c0[0], "ev": ev}
+        log(f"[bench] davidson {mode}: {wall:.2f}s, {len(recs)} iterations, {int(conv.sum())}/32 converged")
+    wstat.set_precision("fp64")
+    e64, e32 = res["fp64"].pop("ev"), res["mixed"].pop("ev")
+    return {"workload": f"{s.name}: npw={s.basis.npw}, fft={'x'.join(map(str, s.basis.nr))}, nbndval={gs.nbndval}",
+            "n_pdep_eigen": 32, "n_pdep_times": 4, "tr2_dfpt": 1e-12, "fp64": res["fp64"], "mixed": res["mixed"],
+            "ev_max_rel_err_mixed_vs_fp64": float(np.max(np.abs(e32 - e64) / np.abs(e64))),
+            "ev_fp64_lowest": [round(float(x), 8) for x in e64[:4]]}
+
+
+def _run_mode(wstat, torch, args, mode, npert, dvg, dng):
+    """Warmup + timed chi0 batches in one precision mode; per-step counters and kernel times."""
+    wstat.set_precision(mode)
+    nv = wstat._state["gs"].nbndval
+    pairs = npert * nv
+    for i in range(args.warmup):
+        s0 = wstat.counters()
+        t = time.perf_counter()
+        wstat.apply_operator(npert, dvg, args.tr2, dng=dng)
+        torch.cuda.synchronize()
+        s1 = wstat.counters()
+        log(f"[bench] {mode} warmup {i}: {time.perf_counter() - t:.3f}s, stern={s1[0] - s0[0]}, "
+            f"cg_iters={s1[1] - s0[1]}")
+    wstat.profile(True)
+    s0, p0 = wstat.counters(), wstat.precision_counters()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        wstat.apply_operator(npert, dvg, args.tr2, dng=dng)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    s1, p1 = wstat.counters(), wstat.precision_counters()
+    elapsed = t1 - t0
+    kernels = {}
+    for name in ("zinv", "plane_vloc", "zfwd", "plane_density", "plane_to_real", "gemm_bra", "gemm_ket", "cg_prec",
+                 "cg_dir", "cg_update", "precondition", "zinv32", "plane_vloc32", "zfwd32", "gemm_bra32",
+                 "gemm_ket32", "cg_prec32", "cg_dir32", "cg_update32", "refine_fold", "refine_residual"):
+        ms, cnt = wstat.profile_query(name)
+        if cnt:
+            kernels[name] = {"ms_per_step": ms / args.steps, "launches": cnt, "avg_ms": ms / cnt}
+    wstat.profile(False)
+    wstat.set_precision("fp64")
+    log(f"[bench] {mode} kernel ms/step: " + ", ".join(f"{k}={v['ms_per_step']:.1f}" for k, v in kernels.items()))
+    return {"value": pairs * args.steps / elapsed, "ms_per_step": 1000.0 * elapsed / args.steps,
+            "stern_per_step": (s1[0] - s0[0]) / args.steps, "iters_per_step": (s1[1] - s0[1]) / args.steps,
+            "fp32_per_step": (p1[0] - p0[0]) / args.steps, "refine_per_step": (p1[1] - p0[1]) / args.steps,
+            "fallback_per_step": (p1[2] - p0[2]) / args.steps,
+            "pairs": pairs, "kernels": kernels}
 
 
 def _save_gs(gs, path):
@@ -205,30 +286,42 @@ def _ncols(b):
     return len({(int(m[0]), int(m[1])) for m in b.mill})
 
 
-def _byte_model(b, gs, npert, stern_per_step):
+def _byte_model(b, gs, npert, run):
     """Compulsory HBM bytes (or MFMA flops) per step of each kernel class.
 
     Units: a transform item is one band pair (two real bands packed in one
-    complex transform); its stick block is n3 x (2*ncols - 1) complex128.
-    plane_vloc : read + write one stick block, read V(r) (nnr f64) per item
-    zinv       : read two bands (2 npw complex128), write one stick block
+    complex transform); its stick block is n3 x (2*ncols - 1) complex numbers
+    (16 bytes in FP64, 8 in FP32).
+    plane_vloc : read + write one stick block, read V(r) (nnr reals) per item
+    zinv       : read two bands (2 npw complex), write one stick block
     zfwd       : read one stick block, read two bands, write two bands
     gemm_bra / gemm_ket : 2 * (2 npw) * nq flops per Sternheimer application
-    cg_update  : read h, g, t, x and write x, g per application (6 npw complex128)
+    cg_update  : read h, g, t, x and write x, g per application (6 npw complex)
+    FP64 kernels of the mixed run are the refinement residuals and the
+    right-hand sides; the FP32 kernels carry the CG iterations.
     """
     npw = b.npw
     nv = gs.nbndval
     nq = gs.nbndval + gs.nkb
-    sticks = b.nr[2] * (2 * _ncols(b) - 1) * 16
-    items_cg = stern_per_step / 2.0
+    ncol = 2 * _ncols(b) - 1
+    st64 = b.nr[2] * ncol * 16
+    st32 = st64 // 2
+    n32 = run["fp32_per_step"]
+    n64 = run["stern_per_step"] - n32
+    it64 = n64 / 2.0
+    it32 = n32 / 2.0
     items_rhs = npert * ((nv + 1) // 2)
     return {
-        "plane_vloc": ("hbm", (items_cg + items_rhs) * (2 * sticks + 8 * b.nnr)),
-        "zinv": ("hbm", items_cg * (2 * 16 * npw + sticks)),
-        "zfwd": ("hbm", items_cg * (sticks + 4 * 16 * npw)),
-        "gemm_bra": ("mfma", 2.0 * 2 * npw * nq * stern_per_step),
-        "gemm_ket": ("mfma", 2.0 * 2 * npw * nq * stern_per_step),
-        "cg_update": ("hbm", 6 * 16 * npw * stern_per_step),
+        "plane_vloc": ("hbm", (it64 + items_rhs) * (2 * st64 + 8 * b.nnr)),
+        "zinv": ("hbm", it64 * (2 * 16 * npw + st64)),
+        "zfwd": ("hbm", it64 * (st64 + 4 * 16 * npw)),
+        "gemm_bra": ("mfma", 2.0 * 2 * npw * nq * n64),
+        "gemm_ket": ("mfma", 2.0 * 2 * npw * nq * n64),
+        "cg_update": ("hbm", 6 * 16 * npw * n64),
+        "plane_vloc32": ("hbm", it32 * (2 * st32 + 4 * b.nnr)),
+        "zinv32": ("hbm", it32 * (2 * 8 * npw + st32)),
+        "zfwd32": ("hbm", it32 * (st32 + 4 * 8 * npw)),
+        "cg_update32": ("hbm", 6 * 8 * npw * n32),
     }
 
 
@@ -252,17 +345,26 @@ def _roofline(name, k, model, steps):
             "avg_launch_ms": k["avg_ms"]}
 
 
-def _end_to_end(b, nv, pairs_per_s, n_it):
-    """Bytes per (pert, band) pair of the chi0 model (FP64, b = 16, s = 8):
+def _end_to_end(b, nv, pairs_per_s, run):
+    """Bytes per (pert, band) pair of the chi0 model, b bytes per complex and
+    s per real (FP64: b = 16, s = 8; FP32: 8, 4):
     B_iter = 2b N_r + (s/2) N_r + 16 b N_pw per operator application and CG step,
-    B_pair = n_it B_iter + 4b N_r + 1.5 s N_r + 4b N_pw, with n_it the measured
-    number of operator applications per pair."""
+    B_pair = n_it B_iter + 4b N_r + 1.5 s N_r + 4b N_pw (FP64 right-hand side and
+    density), with n_it the measured operator applications per pair, split into
+    FP32 and FP64 ones for the mixed-precision run."""
     nr, npw = b.nnr, b.npw
-    biter = 32 * nr + 4 * nr + 256 * npw
-    bpair = n_it * biter + 64 * nr + 12 * nr + 64 * npw
+    pairs = run["pairs"]
+    n32 = run["fp32_per_step"] / pairs
+    n64 = (run["stern_per_step"] - run["fp32_per_step"]) / pairs
+
+    def biter(bc, s):
+        return 2 * bc * nr + s / 2 * nr + 16 * bc * npw
+
+    bpair = n64 * biter(16, 8) + n32 * biter(8, 4) + 64 * nr + 12 * nr + 64 * npw
     ach = pairs_per_s * bpair / 1e9
     return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "bytes_per_pair": int(bpair), "n_it": round(n_it, 2)}
+            "frac": round(ach / HBM_PEAK_GBS, 4), "bytes_per_pair": int(bpair), "n_it": round(n32 + n64, 2),
+            "n_it_fp32": round(n32, 2)}
 
 
 if __name__ == "__main__":

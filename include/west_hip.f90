@@ -86,6 +86,12 @@ MODULE west_hip
        INTEGER(C_INT), VALUE :: n_dfpt_maxiter, l_kinetic_only
      END FUNCTION
      !
+     INTEGER(C_INT) FUNCTION wh_set_precision(mode, refine_delta) BIND(C, NAME="wh_set_precision")
+       IMPORT :: C_INT, C_DOUBLE
+       INTEGER(C_INT), VALUE :: mode
+       REAL(C_DOUBLE), VALUE :: refine_delta
+     END FUNCTION
+     !
      INTEGER(C_INT) FUNCTION wh_apply_sternheimerop_to_m_wfcs(nbndval, psi, hpsi, e, alpha, m) &
           & BIND(C, NAME="wh_apply_sternheimerop_to_m_wfcs")
        IMPORT :: C_INT, C_DOUBLE, C_DOUBLE_COMPLEX
@@ -138,6 +144,12 @@ MODULE west_hip
      INTEGER(C_INT) FUNCTION wh_get_counters(stern_ncalls, cg_iters) BIND(C, NAME="wh_get_counters")
        IMPORT :: C_INT, C_LONG
        INTEGER(C_LONG), INTENT(OUT) :: stern_ncalls, cg_iters
+     END FUNCTION
+     !
+     INTEGER(C_INT) FUNCTION wh_get_precision_counters(fp32_applies, refinements, fp64_fallbacks) &
+          & BIND(C, NAME="wh_get_precision_counters")
+       IMPORT :: C_INT, C_LONG
+       INTEGER(C_LONG), INTENT(OUT) :: fp32_applies, refinements, fp64_fallbacks
      END FUNCTION
      !
   END INTERFACE

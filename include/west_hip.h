@@ -76,6 +76,23 @@ int wh_set_coulomb(const double* sqvc);
  * (apply_sternheimerop_to_m_wfcs.f90:49-50). */
 int wh_set_params(int n_dfpt_maxiter, int l_kinetic_only);
 
+/* Arithmetic of the Sternheimer operator and solve inside
+ * apply_sternheimerop_to_m_wfcs, linsolve_sternheimer_m_wfcts, dfpt and
+ * apply_operator. mode 0: FP64 throughout (the drop-in, same arithmetic as
+ * DFPT_kernel/linsolve_sternheimer_m_wfcts.f90:107-332).
+ * modes 1 and 2: the operator runs in FP32 (FFTs, vectors; projector GEMMs on
+ * the matrix cores in split BF16 with FP32 accumulation: 3 bf16 products per
+ * product in mode 1, about 16 bits; 6 in mode 2, about FP32). The CG iterates
+ * in FP32 on an increment of the FP64 solution; the FP64 residual b - A x is
+ * recomputed whenever the FP32 rho has dropped by refine_delta (<= 0: 1e-4)
+ * since the last recomputation, and a band converges only when that FP64 rho
+ * is below tr2 (the reference's test). A band whose FP64 residual twice stays
+ * far above the FP32 estimate is finished by the FP64 CG. The reference has
+ * no such mode; call surfaces and convergence test are unchanged. Modes 1-2
+ * need a cubic FFT grid of a size with compiled kernels (18, 20, 24, 30, 32,
+ * 36, 40, 45, 48, 54, 60, 64, 72, 80, 90, 96, 100, 108). */
+int wh_set_precision(int mode, double refine_delta);
+
 /* ---- the hot path --------------------------------------------------- */
 
 /* hpsi = H psi - e psi + alpha P_v psi for m bands (psi, hpsi: (npwx, m)
@@ -135,6 +152,11 @@ int wh_davidson_diago(const wh_davidson_params* params, double* ev, int* conv, d
 /* Operator applications so far (the 'stern' clock count of the reference,
  * test-suite/test001/ref/wstat.json:820-826) and CG iterations. */
 int wh_get_counters(long* stern_ncalls, long* cg_iters);
+
+/* Of the operator applications: those made in FP32; the band residuals
+ * recomputed in FP64 by the mixed-precision solve; the bands it handed to the
+ * FP64 CG. */
+int wh_get_precision_counters(long* fp32_applies, long* refinements, long* fp64_fallbacks);
 
 /* Per-kernel device timing with HIP events on the library stream.
  * wh_profile(1) enables and clears, wh_profile(0) disables.

@@ -7,7 +7,9 @@
 
 #include "../../include/west_hip.h"
 #include "wh_context.h"
+#include "wh_fft_tpl.h"
 #include "wh_gemm.h"
+#include "wh_gemm_bf16.h"
 #include "wh_vec.h"
 
 namespace wh {
@@ -99,6 +101,7 @@ static const T* dev_in(const T* p, size_t count, DevBuf& tmp) {
 // ---------------------------------------------------------------------------
 static void rebuild_q() {
   Context& c = ctx();
+  c.mixed_valid = false;
   const int rows = c.nbndval + c.nkb;
   if (c.d_q) WH_HIP(hipFree(c.d_q));
   if (c.d_qscale) WH_HIP(hipFree(c.d_qscale));
@@ -252,6 +255,246 @@ int linsolve_dev(int m, z64* b, z64* x, const std::vector<double>& e_col, const 
 }
 
 // ---------------------------------------------------------------------------
+// Mixed precision: the CG iterates in FP32 (FFTs, vectors, split-BF16
+// projector GEMMs with FP32 accumulation) on an increment dx of the FP64
+// iterate x; the residual b - A x is recomputed with the FP64 operator
+// whenever the FP32 residual has dropped by refine_delta (in rho) since the
+// last recomputation, or below tr2. Convergence is decided on that FP64
+// residual only (rho_64 < tr2), so a converged band meets the FP64 criterion
+// of linsolve_sternheimer_m_wfcts.f90:190-205.
+// ---------------------------------------------------------------------------
+static long round_up(long a, long b) { return (a + b - 1) / b * b; }
+
+static void prepare_mixed() {
+  Context& c = ctx();
+  if (c.mixed_valid) return;
+  WH_REQUIRE(c.plan.tpl32 != nullptr,
+             "mixed precision needs a cubic FFT grid of a compiled size (18, 20, 24, 30, 32, 36, 40, 45, 48, 54, 60, "
+             "64, 72, 80, 90, 96, 100, 108)");
+  hipStream_t s = c.stream;
+  c.ld32 = round_up(c.npw, 16);
+  const long lda = 2 * c.ld32;
+  float* v32 = c.m_vrs32.as<float>(c.nnr);
+  if (c.have_pot) to_f32(1, c.nnr, c.d_vrs, c.nnr, v32, c.nnr, s);
+  const int nq = c.q_rows;
+  c.nq_pad = round_up(std::max(nq, 1), 32);
+  if (nq > 0) {
+    float* q32 = c.m_q32.as<float>(static_cast<size_t>(nq) * lda);
+    WH_HIP(hipMemsetAsync(q32, 0, sizeof(float) * nq * lda, s));
+    to_f32(nq, 2 * c.npw, reinterpret_cast<const double*>(c.d_q), 2L * c.npw, q32, lda, s);
+    const size_t qb = sizeof(uint16_t) * nq * lda;
+    void* const qp[3] = {c.m_qp[0].get(qb), c.m_qp[1].get(qb), c.m_qp[2].get(qb)};
+    split_bf16(nq, 2 * c.npw, q32, lda, qp, lda, s);
+    const size_t tb = sizeof(uint16_t) * 2 * c.npw * c.nq_pad;
+    void* const qtp[3] = {c.m_qtp[0].get(tb), c.m_qtp[1].get(tb), c.m_qtp[2].get(tb)};
+    split_bf16_t(nq, 2 * c.npw, q32, lda, qtp, c.nq_pad, s);
+  }
+  WH_HIP(hipStreamSynchronize(s));
+  c.mixed_valid = true;
+}
+
+// t = (H - e + alpha P_v) x in FP32; x, t rows of ld32 complex
+void stern_apply32(const z32* x, z32* t, const double* d_e, int m, double alpha) {
+  Context& c = ctx();
+  if (m <= 0) return;
+  hipStream_t s = c.stream;
+  const int npw = c.npw;
+  const long l32 = c.ld32;
+  if (c.l_kinetic_only) {
+    WH_TIMED("kinetic", kinetic_minus_e(m, npw, x, t, l32, c.d_g2kin, d_e, s));
+  } else {
+    const TplKernels32* K = c.plan.tpl32;
+    const int npair = (m + 1) / 2;
+    const int cap = c.items_capacity();
+    FftItem* items = c.w_items.as<FftItem>(npair);
+    fill_pairs(items, m, 0, s);
+    FftItem* ident = c.w_items_id.as<FftItem>(std::min(cap, npair));
+    fill_ident(ident, std::min(cap, npair), s);
+    const int chunk = std::min(cap, npair);
+    z32* sa = c.w_sticks_a.as<z32>(static_cast<size_t>(chunk) * c.plan.sticks_elems());
+    z32* sb = c.w_sticks_b.as<z32>(static_cast<size_t>(chunk) * c.plan.sticks_elems());
+    const float* v32 = static_cast<const float*>(c.m_vrs32.p);
+    ZfwdArgs za{ZFWD_PAIR_STERN, 1.0, c.d_g2kin, d_e, nullptr, 0};
+    for (int q0 = 0; q0 < npair; q0 += chunk) {
+      const int nq = std::min(chunk, npair - q0);
+      WH_TIMED("zinv32", K->zinv(c.plan.dev, items + q0, nq, x, l32, x, l32, sa, ZINV_PAIR, s));
+      WH_TIMED("plane_vloc32", plane_occ_launch(
+                                   c.plan.vloc32_occ, K->max_occ, nq,
+                                   [&](int occ) { K->vloc(c.plan.dev, ident, nq, sa, sb, v32, 0, 1.0 / c.nnr, occ, s); },
+                                   s));
+      WH_TIMED("zfwd32", K->zfwd(c.plan.dev, items + q0, nq, sb, x, l32, t, l32, za, s));
+      WH_HIP(hipGetLastError());
+    }
+  }
+  const int nq = c.l_kinetic_only ? c.nbndval : c.nbndval + c.nkb;
+  if (nq > 0) {
+    set_qscale_alpha(alpha);
+    const int K2 = 2 * npw;
+    const long lda = 2 * l32;
+    const int terms = c.precision == 2 ? 6 : 3;
+    const size_t wf = gemm_bra_bf16_work(nq, m, K2);
+    float* work = c.m_work.as<float>(wf);
+    float* cm = c.m_cm.as<float>(static_cast<size_t>(c.nq_pad) * m);
+    const void* const qp[3] = {c.m_qp[0].p, c.m_qp[1].p, c.m_qp[2].p};
+    const void* const qtp[3] = {c.m_qtp[0].p, c.m_qtp[1].p, c.m_qtp[2].p};
+    WH_TIMED("gemm_bra32", gemm_bra_bf16(terms, nq, m, K2, qp, lda, static_cast<const float*>(c.m_q32.p),
+                                         reinterpret_cast<const float*>(x), lda, c.gstart == 2 ? 1.f : 0.f,
+                                         c.d_qscale, cm, c.nq_pad, work, wf, s));
+    WH_TIMED("gemm_ket32", gemm_tn_bf16(terms, K2, m, nq, 1.f, qtp, c.nq_pad, cm, c.nq_pad, 1.f,
+                                        reinterpret_cast<float*>(t), 1L, lda, s));
+  }
+  c.nstern += m;
+  c.nstern32 += m;
+}
+
+int linsolve_mixed(int m, z64* b, z64* x, const std::vector<double>& e_col, const double* d_e_col,
+                   const double* d_eprec_col, double tr2, std::vector<int>* ierr_col) {
+  Context& c = ctx();
+  prepare_mixed();
+  hipStream_t s = c.stream;
+  const int npw = c.npw;
+  const long ld = npw;
+  const long l32 = c.ld32;
+  const size_t v32 = static_cast<size_t>(m) * l32;
+  z32* g = c.m_g.as<z32>(v32);
+  z32* h = c.m_h.as<z32>(v32);
+  z32* hold = c.m_hold.as<z32>(v32);
+  z32* dx = c.m_dx.as<z32>(v32);
+  z32* bc = c.m_bc.as<z32>(v32);
+  z32* tc = c.m_tc.as<z32>(v32);
+  z64* xc = c.w_bc.as<z64>(static_cast<size_t>(m) * npw);
+  z64* tc64 = c.w_tc.as<z64>(static_cast<size_t>(m) * npw);
+  int* d_l2i = c.w_l2i.as<int>(m);
+  int* d_u2i = c.m_u2i.as<int>(m);
+  double* d_beta = c.w_beta.as<double>(m);
+  double* d_eu = c.w_eu.as<double>(m);
+  double* d_rho = c.w_rho.as<double>(m);
+  double* d_rho64 = c.m_rho64.as<double>(m);
+  double* d_eu64 = c.m_eu64.as<double>(m);
+  WH_HIP(hipMemsetAsync(dx, 0, sizeof(z32) * v32, s));
+
+  std::vector<char> is_conv(m, 0), to_fp64(m, 0);
+  std::vector<int> l2i(m), list, stall(m, 0);
+  std::vector<double> rho(m, 0.0), rhoold(m, 0.0), rho_ref(m, 0.0), tmp(m), beta(m), eu(m), eu64(m);
+  list.reserve(m);
+
+  // x += dx; r = A x - b in FP64; g = r in FP32; rho_ref = <M^-1 r|r>; converged if rho_ref < tr2.
+  // A band whose FP64 residual twice stays far above what the FP32 iteration
+  // reported (the FP32 operator is not accurate enough for its conditioning)
+  // is finished by the FP64 CG instead.
+  auto refine = [&](const std::vector<int>& u, bool first) {
+    const int nu = static_cast<int>(u.size());
+    for (int k = 0; k < nu; ++k) eu64[k] = e_col[u[k]];
+    WH_HIP(hipMemcpyAsync(d_u2i, u.data(), sizeof(int) * nu, hipMemcpyHostToDevice, s));
+    WH_HIP(hipMemcpyAsync(d_eu64, eu64.data(), sizeof(double) * nu, hipMemcpyHostToDevice, s));
+    WH_TIMED("refine_fold", fold_increment(nu, npw, d_u2i, x, ld, dx, l32, xc, s));
+    stern_apply(xc, tc64, d_eu64, nu, c.alpha_pv);
+    WH_TIMED("refine_residual", true_residual(nu, npw, c.gstart, d_u2i, tc64, b, ld, c.d_g2kin, d_eprec_col, g, l32,
+                                              d_rho64, s));
+    WH_HIP(hipMemcpyAsync(tmp.data(), d_rho64, sizeof(double) * nu, hipMemcpyDeviceToHost, s));
+    WH_HIP(hipStreamSynchronize(s));
+    for (int k = 0; k < nu; ++k) {
+      const int ib = u[k];
+      const double prev = rho_ref[ib];
+      rho_ref[ib] = tmp[k];
+      if (tmp[k] < tr2) {
+        is_conv[ib] = 1;
+      } else if (!first && tmp[k] > 0.1 * prev && tmp[k] > 100.0 * rho[ib]) {
+        if (++stall[ib] >= 2) to_fp64[ib] = 1;
+      }
+    }
+    c.nrefine += nu;
+  };
+  // h = M^-1 g and rho for the bands in idx (device list d_idx)
+  auto precond = [&](const std::vector<int>& idx, int* d_idx) {
+    const int n = static_cast<int>(idx.size());
+    WH_HIP(hipMemcpyAsync(d_idx, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    WH_TIMED("cg_prec32", prec_rho(n, npw, c.gstart, d_idx, g, h, l32, c.d_g2kin, d_eprec_col, d_rho, s));
+    WH_HIP(hipMemcpyAsync(tmp.data(), d_rho, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    WH_HIP(hipStreamSynchronize(s));
+    for (int k = 0; k < n; ++k) rho[idx[k]] = tmp[k];
+  };
+
+  list.resize(m);
+  std::iota(list.begin(), list.end(), 0);
+  refine(list, true);
+  std::vector<int> act;
+  for (int ib = 0; ib < m; ++ib)
+    if (!is_conv[ib]) act.push_back(ib);
+  for (int iter = 1; iter <= c.n_dfpt_maxiter && !act.empty(); ++iter) {
+    precond(act, d_l2i);
+    ++c.cg_iters;
+    list.clear();
+    for (int ib : act)
+      if (rho[ib] < tr2 || rho[ib] < c.refine_delta * rho_ref[ib]) list.push_back(ib);
+    if (!list.empty()) {
+      refine(list, false);
+      std::vector<int> again;
+      for (int ib : list)
+        if (!is_conv[ib] && !to_fp64[ib]) again.push_back(ib);
+      if (!again.empty()) precond(again, d_u2i);
+    }
+    int nc = 0;
+    for (int ib : act)
+      if (!is_conv[ib] && !to_fp64[ib]) l2i[nc++] = ib;
+    act.assign(l2i.begin(), l2i.begin() + nc);
+    if (nc == 0) break;
+    for (int lb = 0; lb < nc; ++lb) {
+      const int ib = l2i[lb];
+      beta[lb] = (iter != 1) ? rho[ib] / rhoold[ib] : 0.0;
+      eu[lb] = e_col[ib];
+    }
+    WH_HIP(hipMemcpyAsync(d_l2i, l2i.data(), sizeof(int) * nc, hipMemcpyHostToDevice, s));
+    WH_HIP(hipMemcpyAsync(d_beta, beta.data(), sizeof(double) * nc, hipMemcpyHostToDevice, s));
+    WH_HIP(hipMemcpyAsync(d_eu, eu.data(), sizeof(double) * nc, hipMemcpyHostToDevice, s));
+    WH_TIMED("cg_dir32", direction(nc, npw, d_l2i, d_beta, h, hold, bc, l32, s));
+    stern_apply32(bc, tc, d_eu, nc, c.alpha_pv);
+    WH_TIMED("cg_update32", cg_update(nc, npw, c.gstart, d_l2i, h, g, tc, dx, l32, nullptr, s));
+    std::swap(h, hold);
+    for (int lb = 0; lb < nc; ++lb) rhoold[l2i[lb]] = rho[l2i[lb]];
+    WH_HIP(hipStreamSynchronize(s));
+  }
+  // bands stopped by n_dfpt_maxiter keep their last FP32 increment
+  if (!act.empty()) {
+    WH_HIP(hipMemcpyAsync(d_u2i, act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice, s));
+    fold_increment(static_cast<int>(act.size()), npw, d_u2i, x, ld, dx, l32, xc, s);
+    WH_HIP(hipStreamSynchronize(s));
+  }
+  if (ierr_col) ierr_col->assign(m, 0);
+  for (int ib : act)
+    if (ierr_col) (*ierr_col)[ib] = 1;
+  int nonconv = static_cast<int>(act.size());
+  // bands handed to the FP64 CG, restarted from their FP64 iterate
+  std::vector<int> fl;
+  for (int ib = 0; ib < m; ++ib)
+    if (to_fp64[ib] && !is_conv[ib]) fl.push_back(ib);
+  if (!fl.empty()) {
+    const int nf = static_cast<int>(fl.size());
+    z64* fb = c.m_fb.as<z64>(static_cast<size_t>(nf) * npw);
+    z64* fx = c.m_fx.as<z64>(static_cast<size_t>(nf) * npw);
+    double* fe = c.m_fe.as<double>(nf);
+    double* fep = c.m_fep.as<double>(nf);
+    std::vector<double> e_f(nf);
+    for (int k = 0; k < nf; ++k) e_f[k] = e_col[fl[k]];
+    WH_HIP(hipMemcpyAsync(d_u2i, fl.data(), sizeof(int) * nf, hipMemcpyHostToDevice, s));
+    WH_HIP(hipMemcpyAsync(fe, e_f.data(), sizeof(double) * nf, hipMemcpyHostToDevice, s));
+    copy_rows(nf, npw, d_u2i, b, ld, fb, ld, false, s);
+    copy_rows(nf, npw, d_u2i, x, ld, fx, ld, false, s);
+    gather_real(nf, d_u2i, d_eprec_col, fep, s);
+    WH_HIP(hipStreamSynchronize(s));
+    c.nfallback += nf;
+    std::vector<int> ierr_f;
+    nonconv += linsolve_dev(nf, fb, fx, e_f, fe, fep, tr2, &ierr_f);
+    WH_HIP(hipMemcpyAsync(d_u2i, fl.data(), sizeof(int) * nf, hipMemcpyHostToDevice, s));
+    copy_rows(nf, npw, d_u2i, fx, ld, x, ld, true, s);
+    WH_HIP(hipStreamSynchronize(s));
+    if (ierr_col)
+      for (int k = 0; k < nf; ++k) (*ierr_col)[fl[k]] = ierr_f[k];
+  }
+  return nonconv;
+}
+
+// ---------------------------------------------------------------------------
 // chi0 at Gamma  (DFPT_kernel/dfpt_module.f90:139-503, Gamma branches)
 //   dvg: m perturbations (rows, stick order); dng: m rows out.
 //   out_weight: per-G factor applied to dng (sqvc for apply_operator) or null.
@@ -343,7 +586,8 @@ int dfpt_dev(int m, const z64* dvg, z64* dng, double tr2, const double* out_weig
     WH_HIP(hipMemcpyAsync(de, e_col.data(), sizeof(double) * M, hipMemcpyHostToDevice, s));
     WH_HIP(hipMemcpyAsync(dep, ep_col.data(), sizeof(double) * M, hipMemcpyHostToDevice, s));
     std::vector<int> ierr_col;
-    nonconv = linsolve_dev(static_cast<int>(M), dvpsi, dpsi, e_col, de, dep, tr2, &ierr_col);
+    nonconv = c.precision != 0 ? linsolve_mixed(static_cast<int>(M), dvpsi, dpsi, e_col, de, dep, tr2, &ierr_col)
+                               : linsolve_dev(static_cast<int>(M), dvpsi, dpsi, e_col, de, dep, tr2, &ierr_col);
     if (ierr_pert)
       for (long i = 0; i < M; ++i) (*ierr_pert)[i / nv] += ierr_col[i];
   }
@@ -435,7 +679,10 @@ int wh_finalize(void) {
     if (p) (void)hipFree(p);
   for (DevBuf* b : {&c.evc_sticks, &c.w_items, &c.w_items2, &c.w_items_id, &c.w_sticks_a, &c.w_sticks_b,
                     &c.w_cmat, &c.w_x, &c.w_t, &c.w_g, &c.w_h, &c.w_hold, &c.w_bc, &c.w_tc, &c.w_b, &c.w_dvr,
-                    &c.w_in, &c.w_out, &c.w_dvin, &c.w_dng, &c.w_l2i, &c.w_beta, &c.w_eu, &c.w_rho, &c.w_ecol, &c.w_epcol})
+                    &c.w_in, &c.w_out, &c.w_dvin, &c.w_dng, &c.w_l2i, &c.w_beta, &c.w_eu, &c.w_rho, &c.w_ecol, &c.w_epcol,
+                    &c.m_vrs32, &c.m_q32, &c.m_qp[0], &c.m_qp[1], &c.m_qp[2], &c.m_qtp[0], &c.m_qtp[1], &c.m_qtp[2],
+                    &c.m_g, &c.m_h, &c.m_hold, &c.m_dx, &c.m_bc, &c.m_tc, &c.m_cm, &c.m_work, &c.m_u2i, &c.m_rho64,
+                    &c.m_eu64, &c.m_fb, &c.m_fx, &c.m_fe, &c.m_fep})
     b->release();
   (void)hipStreamDestroy(c.stream);
   delete g_ctx;
@@ -472,6 +719,7 @@ int wh_set_grid(int nr1, int nr2, int nr3, int npw, int npwx, int gstart, const 
   WH_HIP(hipMemcpy(c.d_perm, c.plan.perm.data(), sizeof(int) * npw, hipMemcpyHostToDevice));
   WH_HIP(hipMemcpy(c.d_g2kin, c.g2kin_s.data(), sizeof(double) * npw, hipMemcpyHostToDevice));
   c.have_grid = true;
+  c.mixed_valid = false;
   c.have_pot = false;
   c.have_states = false;
   c.have_sqvc = false;
@@ -490,6 +738,7 @@ int wh_set_potential(const double* vrs) {
   WH_HIP(hipMalloc(&c.d_vrs, sizeof(double) * c.nnr));
   WH_HIP(hipMemcpy(c.d_vrs, vrs, sizeof(double) * c.nnr, hipMemcpyDefault));
   c.have_pot = true;
+  c.mixed_valid = false;
   return 0;
   WH_API_END
 }
@@ -595,7 +844,18 @@ int wh_apply_sternheimerop_to_m_wfcs(int nbndval, const double* psi, double* hps
   z64* x = c.w_x.as<z64>(static_cast<size_t>(m) * c.npw);
   z64* t = c.w_t.as<z64>(static_cast<size_t>(m) * c.npw);
   gather_sticks(m, c.npw, c.d_perm, pin, c.npwx, x, c.npw, nullptr, s);
-  stern_apply(x, t, pe, m, a[0]);
+  if (c.precision != 0) {
+    // the FP32 operator of the mixed-precision solve, on FP64 data
+    prepare_mixed();
+    const long l32 = c.ld32;
+    z32* x32 = c.m_bc.as<z32>(static_cast<size_t>(m) * l32);
+    z32* t32 = c.m_tc.as<z32>(static_cast<size_t>(m) * l32);
+    to_f32(m, 2 * c.npw, reinterpret_cast<const double*>(x), 2L * c.npw, reinterpret_cast<float*>(x32), 2 * l32, s);
+    stern_apply32(x32, t32, pe, m, a[0]);
+    to_f64(m, 2 * c.npw, reinterpret_cast<const float*>(t32), 2 * l32, reinterpret_cast<double*>(t), 2L * c.npw, s);
+  } else {
+    stern_apply(x, t, pe, m, a[0]);
+  }
   const bool dout = on_device(hpsi);
   z64* o = dout ? reinterpret_cast<z64*>(hpsi) : c.w_out.as<z64>(cin);
   scatter_caller(m, c.npw, c.npwx, c.d_perm, t, c.npw, o, c.npwx, s);
@@ -629,7 +889,8 @@ int wh_linsolve_sternheimer_m_wfcts(int nbndval, int m, double* b, double* x, co
   gather_sticks(m, c.npw, c.d_perm, bin, c.npwx, db, c.npw, nullptr, s);
   const z64* xin = dev_in(reinterpret_cast<const z64*>(x), cin, c.w_out);
   gather_sticks(m, c.npw, c.d_perm, xin, c.npwx, dx, c.npw, nullptr, s);
-  const int nc = linsolve_dev(m, db, dx, e_col, de, dep, tr2, nullptr);
+  const int nc = c.precision != 0 ? linsolve_mixed(m, db, dx, e_col, de, dep, tr2, nullptr)
+                                  : linsolve_dev(m, db, dx, e_col, de, dep, tr2, nullptr);
   const bool dout = on_device(x);
   z64* o = dout ? reinterpret_cast<z64*>(x) : c.w_out.as<z64>(cin);
   scatter_caller(m, c.npw, c.npwx, c.d_perm, dx, c.npw, o, c.npwx, s);
@@ -673,6 +934,29 @@ int wh_dfpt(int m, const double* dvg, double* dng, double tr2, int iq, int* ierr
 int wh_apply_operator(int m, const double* dvg, double* dng, double tr2, int iq) {
   WH_API_BEGIN
   return dfpt_entry(m, dvg, dng, tr2, iq, nullptr, true);
+  WH_API_END
+}
+
+int wh_set_precision(int mode, double refine_delta) {
+  WH_API_BEGIN
+  Context& c = ctx();
+  WH_REQUIRE(mode >= 0 && mode <= 2, "precision mode must be 0 (FP64), 1 or 2 (mixed)");
+  WH_REQUIRE(!(refine_delta >= 1.0), "refine_delta must be < 1 (<= 0 selects the default 1e-4)");
+  c.precision = mode;
+  c.refine_delta = refine_delta > 0.0 ? refine_delta : 1e-4;
+  if (mode != 0 && c.have_grid)
+    WH_REQUIRE(c.plan.tpl32 != nullptr, "mixed precision needs a cubic FFT grid of a compiled size");
+  return 0;
+  WH_API_END
+}
+
+int wh_get_precision_counters(long* fp32_applies, long* refinements, long* fp64_fallbacks) {
+  WH_API_BEGIN
+  Context& c = ctx();
+  if (fp32_applies) *fp32_applies = c.nstern32;
+  if (refinements) *refinements = c.nrefine;
+  if (fp64_fallbacks) *fp64_fallbacks = c.nfallback;
+  return 0;
   WH_API_END
 }
 

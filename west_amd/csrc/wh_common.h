@@ -77,6 +77,7 @@ __host__ __device__ __forceinline__ cplx<T> mul_i(cplx<T> a, int dir) {
 }
 
 using z64 = cplx<double>;
+using z32 = cplx<float>;
 
 inline int ceil_div(long a, long b) { return static_cast<int>((a + b - 1) / b); }
 

@@ -94,8 +94,20 @@ struct Context {
   DevBuf w_items, w_items2, w_items_id, w_sticks_a, w_sticks_b, w_cmat;
   DevBuf w_x, w_t, w_g, w_h, w_hold, w_bc, w_tc, w_b, w_dvr, w_in, w_out, w_dvin, w_dng;
   DevBuf w_l2i, w_beta, w_eu, w_rho, w_ecol, w_epcol;
+  // mixed precision: FP32 operator with split-BF16 projectors, FP64 refinement
+  int precision = 0;              // 0: FP64, 1: mixed (3-product split BF16), 2: mixed (6-product)
+  double refine_delta = 1e-4;     // refine when rho < refine_delta * rho of the last refinement
+  bool mixed_valid = false;       // FP32 copies below match the registered state
+  long ld32 = 0;                  // FP32 vector row length in complex numbers (npw rounded up to 16)
+  long nq_pad = 0;                // Q rows rounded up to 32
+  DevBuf m_vrs32, m_q32, m_qp[3], m_qtp[3];   // Q in FP32; its bf16 pieces; pieces of Q^T
+  DevBuf m_g, m_h, m_hold, m_dx, m_bc, m_tc, m_cm, m_work, m_u2i, m_rho64, m_eu64;
+  DevBuf m_fb, m_fx, m_fe, m_fep;   // bands finished by the FP64 CG (right sides, solutions, e, eprec)
   // counters
-  long nstern = 0;
+  long nstern = 0;     // operator applications (all precisions)
+  long nstern32 = 0;   // of which FP32
+  long nrefine = 0;    // FP64 refinement residuals (band count)
+  long nfallback = 0;  // bands the mixed solve handed to the FP64 CG
   long cg_iters = 0;
   Profiler prof;
 
@@ -126,6 +138,9 @@ void stern_apply(const z64* x, z64* t, const double* d_e, int m, double alpha);
 int linsolve_dev(int m, z64* b, z64* x, const std::vector<double>& e_col, const double* d_e_col,
                  const double* d_eprec_col, double tr2, std::vector<int>* ierr_col);
 int dfpt_dev(int m, const z64* dvg, z64* dng, double tr2, const double* out_weight, std::vector<int>* ierr_pert);
-void glbrak_q(int row0, int nq, const z64* f, int m, double* cmat);   // cmat (nq x m) = <Q_row|f_col>
+void glbrak_q(int row0, int nq, const z64* f, int m, double* cmat);
+void stern_apply32(const z32* x, z32* t, const double* d_e, int m, double alpha);
+int linsolve_mixed(int m, z64* b, z64* x, const std::vector<double>& e_col, const double* d_e_col,
+                   const double* d_eprec_col, double tr2, std::vector<int>* ierr_col);   // cmat (nq x m) = <Q_row|f_col>
 
 }  // namespace wh

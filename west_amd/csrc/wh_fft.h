@@ -16,6 +16,8 @@
 // so the full real-space grid never goes to HBM for the local-potential term.
 #pragma once
 
+#include <algorithm>
+
 #include "wh_common.h"
 
 namespace wh {
@@ -47,10 +49,16 @@ struct StickPlanDev {
 };
 
 struct TplKernels;
+struct TplKernels32;
 
 struct StickPlan {
   StickPlanDev dev{};
   const TplKernels* tpl = nullptr;   // size-specialised kernels for this grid, if compiled
+  const TplKernels32* tpl32 = nullptr;   // FP32 operator kernels for this grid, if compiled
+  // workgroups per CU the size-specialised plane kernel's LDS is sized for
+  // (FP64 / FP32 operator); 0 until timed on the first large launch
+  mutable int vloc_occ = 0;
+  mutable int vloc32_occ = 0;
   std::vector<int> col_off, col_kmin, col_pos, pt_col, pt_k;
   std::vector<int> col_i, col_j;   // stored columns' Miller (i, j)
   std::vector<int> perm;           // stick position -> caller index (0-based)
@@ -59,6 +67,40 @@ struct StickPlan {
   void release();
   long sticks_elems() const { return static_cast<long>(dev.n3) * dev.nct; }
 };
+
+// Occupancy variant of the plane kernel for a launch of nitems items: a fixed
+// choice (WH_PLANE_OCC or an earlier timing) or, for the first launch of at
+// least 256 items, the fastest of 1..max_occ timed on that launch itself
+// (the variants differ only in LDS chunking and give identical results; the
+// launch's output is the last variant's). run(occ) enqueues one launch.
+template <class Run>
+void plane_occ_launch(int& chosen, int max_occ, int nitems, Run run, hipStream_t s) {
+  if (chosen > 0 || max_occ <= 1 || nitems < 256) {
+    run(chosen > 0 ? std::min(chosen, max_occ) : 1);
+    return;
+  }
+  hipEvent_t a, b;
+  WH_HIP(hipEventCreate(&a));
+  WH_HIP(hipEventCreate(&b));
+  run(1);   // warm-up (first-launch costs)
+  float best = 0.f;
+  int pick = 1;
+  for (int occ = 1; occ <= max_occ; ++occ) {
+    WH_HIP(hipEventRecord(a, s));
+    run(occ);
+    WH_HIP(hipEventRecord(b, s));
+    WH_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    WH_HIP(hipEventElapsedTime(&ms, a, b));
+    if (occ == 1 || ms < best) {
+      best = ms;
+      pick = occ;
+    }
+  }
+  WH_HIP(hipEventDestroy(a));
+  WH_HIP(hipEventDestroy(b));
+  chosen = pick;
+}
 
 // One transform item: bands a and b (row indices into a vector block; b < 0 -> zero),
 // pot = index of the real-space potential (per-item multiply) or -1 for the shared one.

@@ -605,6 +605,12 @@ void StickPlan::build(int nr1, int nr2, int nr3, int npw, const int* nl, const i
   // kernels specialised on the grid size, unless WH_FFT_GENERIC is set
   const char* gen = std::getenv("WH_FFT_GENERIC");
   tpl = (gen && gen[0] == '1') ? nullptr : tpl_kernels_f64(dev);
+  tpl32 = tpl_kernels_f32(dev);
+  // plane-kernel LDS sizing: WH_PLANE_OCC (1..3) fixes it, else timed on the first large launch
+  const char* occ_env = std::getenv("WH_PLANE_OCC");
+  const int occ = (occ_env && occ_env[0] >= '1' && occ_env[0] <= '3') ? occ_env[0] - '0' : 0;
+  vloc_occ = occ;
+  vloc32_occ = occ;
 }
 
 void launch_zinv(const StickPlan& p, const FftItem* d_items, int nitems, const z64* psiA, long ldA,
@@ -645,7 +651,10 @@ void launch_plane_vloc(const StickPlan& p, const FftItem* d_items, int nitems, c
   if (nitems <= 0) return;
   const StickPlanDev& P = p.dev;
   if (p.tpl) {
-    p.tpl->vloc(P, d_items, nitems, sin, sout, vpot, ldpot, scale, s);
+    WH_REQUIRE(static_cast<const void*>(sin) != static_cast<const void*>(sout), "plane_vloc: input and output alias");
+    plane_occ_launch(
+        p.vloc_occ, p.tpl->max_occ, nitems,
+        [&](int occ) { p.tpl->vloc(P, d_items, nitems, sin, sout, vpot, ldpot, scale, occ, s); }, s);
     WH_HIP(hipGetLastError());
     return;
   }

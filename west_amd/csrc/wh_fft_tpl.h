@@ -9,14 +9,28 @@ struct TplKernels {
   void (*zinv)(const StickPlanDev&, const FftItem*, int, const z64*, long, const z64*, long, void*, int, hipStream_t);
   void (*zfwd)(const StickPlanDev&, const FftItem*, int, const void*, const z64*, long, z64*, long, const ZfwdArgs&,
                hipStream_t);
-  void (*vloc)(const StickPlanDev&, const FftItem*, int, const void*, void*, const double*, long, double, hipStream_t);
+  // occ: workgroups per CU the plane kernel's LDS is sized for (1..max_occ)
+  void (*vloc)(const StickPlanDev&, const FftItem*, int, const void*, void*, const double*, long, double, int,
+               hipStream_t);
   void (*to_real)(const StickPlanDev&, const FftItem*, int, const void*, double*, long, hipStream_t);
   void (*density)(const StickPlanDev&, int, int, const void*, const double*, void*, hipStream_t);
   bool density_fits;
+  int max_occ;
+};
+
+// FP32 operator (sticks, vectors and potential in FP32)
+struct TplKernels32 {
+  void (*zinv)(const StickPlanDev&, const FftItem*, int, const z32*, long, const z32*, long, void*, int, hipStream_t);
+  void (*zfwd)(const StickPlanDev&, const FftItem*, int, const void*, const z32*, long, z32*, long, const ZfwdArgs&,
+               hipStream_t);
+  void (*vloc)(const StickPlanDev&, const FftItem*, int, const void*, void*, const float*, long, double, int,
+               hipStream_t);
+  int max_occ;
 };
 
 // kernels specialised on N = nr1 = nr2 = nr3, or null when the grid is not
 // one of the compiled sizes (the runtime-size kernels are used then)
 const TplKernels* tpl_kernels_f64(const StickPlanDev& P);
+const TplKernels32* tpl_kernels_f32(const StickPlanDev& P);
 
 }  // namespace wh

@@ -213,18 +213,18 @@ struct Plan {
 // stores through st(line, idx, value); a barrier separates the two phases and
 // follows the stores.  MAXL bounds nlines (register staging).
 // ---------------------------------------------------------------------------
-template <typename T, int N, int S, int MAXL, class Ld, class St>
+template <typename T, int N, int S, int MAXL, int BT, class Ld, class St>
 __device__ __forceinline__ void pass(int nlines, const cplx<T>* tw, int dir, Ld ld, St st) {
   constexpr int R = Plan<N>::radix(S);
   constexpr int P = Plan<N>::stride(S);
   constexpr int NBL = N / R;
-  constexpr int NB = (MAXL * NBL + NT - 1) / NT;
+  constexpr int NB = (MAXL * NBL + BT - 1) / BT;
   constexpr int STEP = N / (P * R);
   const int nbf = nlines * NBL;
   cplx<T> v[NB][R];
 #pragma unroll
   for (int t = 0; t < NB; ++t) {
-    const int bf = threadIdx.x + t * NT;
+    const int bf = threadIdx.x + t * BT;
     if (bf < nbf) {
       const int line = bf / NBL;
       const int i = bf - line * NBL;
@@ -235,7 +235,7 @@ __device__ __forceinline__ void pass(int nlines, const cplx<T>* tw, int dir, Ld 
   __syncthreads();
 #pragma unroll
   for (int t = 0; t < NB; ++t) {
-    const int bf = threadIdx.x + t * NT;
+    const int bf = threadIdx.x + t * BT;
     if (bf < nbf) {
       const int line = bf / NBL;
       const int i = bf - line * NBL;
@@ -259,35 +259,38 @@ __device__ __forceinline__ void pass(int nlines, const cplx<T>* tw, int dir, Ld 
 
 // full transform: pass 0 loads through first(), the last pass stores through
 // last(), the passes in between work in place on buf (line stride N)
-template <typename T, int N, int S, int MAXL, class First, class Last>
+template <typename T, int N, int S, int MAXL, int BT, class First, class Last>
 __device__ __forceinline__ void passes(int nlines, cplx<T>* buf, const cplx<T>* tw, int dir, First first, Last last) {
   constexpr int NS = Plan<N>::nst;
   auto in_buf = [buf](int line, int idx) { return buf[line * N + idx]; };
   auto to_buf = [buf](int line, int idx, cplx<T> v) { buf[line * N + idx] = v; };
   if constexpr (NS == 1) {
-    pass<T, N, 0, MAXL>(nlines, tw, dir, first, last);
+    pass<T, N, 0, MAXL, BT>(nlines, tw, dir, first, last);
   } else if constexpr (S == 0) {
-    pass<T, N, 0, MAXL>(nlines, tw, dir, first, to_buf);
-    passes<T, N, 1, MAXL>(nlines, buf, tw, dir, first, last);
+    pass<T, N, 0, MAXL, BT>(nlines, tw, dir, first, to_buf);
+    passes<T, N, 1, MAXL, BT>(nlines, buf, tw, dir, first, last);
   } else if constexpr (S == NS - 1) {
-    pass<T, N, S, MAXL>(nlines, tw, dir, in_buf, last);
+    pass<T, N, S, MAXL, BT>(nlines, tw, dir, in_buf, last);
   } else {
-    pass<T, N, S, MAXL>(nlines, tw, dir, in_buf, to_buf);
-    passes<T, N, S + 1, MAXL>(nlines, buf, tw, dir, first, last);
+    pass<T, N, S, MAXL, BT>(nlines, tw, dir, in_buf, to_buf);
+    passes<T, N, S + 1, MAXL, BT>(nlines, buf, tw, dir, first, last);
   }
 }
 
-template <typename T, int N, int MAXL>
+template <typename T, int N, int MAXL, int BT>
 __device__ __forceinline__ void fft_inplace(int nlines, cplx<T>* buf, const cplx<T>* tw, int dir) {
   auto in_buf = [buf](int line, int idx) { return buf[line * N + idx]; };
   auto to_buf = [buf](int line, int idx, cplx<T> v) { buf[line * N + idx] = v; };
-  passes<T, N, 0, MAXL>(nlines, buf, tw, dir, in_buf, to_buf);
+  passes<T, N, 0, MAXL, BT>(nlines, buf, tw, dir, in_buf, to_buf);
 }
 
 template <typename T, int N>
 __device__ __forceinline__ void load_tw(cplx<T>* tw, const double* g) {
-  for (int t = threadIdx.x; t < N; t += NT) tw[t] = mk<T>(T(g[2 * t]), T(g[2 * t + 1]));
+  for (int t = threadIdx.x; t < N; t += blockDim.x) tw[t] = mk<T>(T(g[2 * t]), T(g[2 * t + 1]));
 }
+
+// threads of the per-plane kernels (two waves per SIMD)
+constexpr int NTP = 512;
 
 // maximum number of |i| <= imax rows: the wave-function sphere spans at most
 // about half of the grid along a1
@@ -295,13 +298,23 @@ template <int N>
 constexpr int max_rows() {
   return N / 2 + 2;
 }
-// x-line chunk of the plane kernel (LDS budget 160 KB)
+// x-line chunk of the plane kernel when `occ` workgroups share the 160 KB of
+// LDS of a CU (more resident workgroups hide the barriers of the LDS passes)
 template <typename T, int N>
-constexpr int x_chunk(bool density) {
-  const int budget = 160 * 1024 - 2048;
+constexpr int x_budget(bool density, int occ) {
+  const int budget = 160 * 1024 / occ - 2048;
   const int fixed = static_cast<int>(sizeof(cplx<T>)) * (N + max_rows<N>() * N) +
                     (density ? static_cast<int>(sizeof(T)) * N * N : 0);
-  int jc = (budget - fixed) / (static_cast<int>(sizeof(cplx<T>)) * N);
+  return (budget - fixed) / (static_cast<int>(sizeof(cplx<T>)) * N);
+}
+// occ workgroups per CU leave room for at least 4 x-lines per chunk
+template <typename T, int N>
+constexpr bool occ_fits(int occ) {
+  return occ == 1 || x_budget<T, N>(false, occ) >= 4;
+}
+template <typename T, int N>
+constexpr int x_chunk(bool density, int occ = 1) {
+  int jc = x_budget<T, N>(density, occ);
   if (jc > N) jc = N;
   if (jc < 1) return 1;
   const int nchunk = (N + jc - 1) / jc;   // equal chunks
@@ -311,9 +324,10 @@ constexpr int x_chunk(bool density) {
 // ---------------------------------------------------------------------------
 // sphere -> sticks
 // ---------------------------------------------------------------------------
-template <typename T, int N>
-__global__ __launch_bounds__(NT) void k_zinv_t(StickPlanDev P, const FftItem* items, const z64* psiA, long ldA,
-                                               const z64* psiB, long ldB, cplx<T>* sticks, int mode) {
+// V: precision of the sphere vectors (double, or float for the FP32 operator)
+template <typename T, int N, typename V>
+__global__ __launch_bounds__(NT) void k_zinv_t(StickPlanDev P, const FftItem* items, const cplx<V>* psiA, long ldA,
+                                               const cplx<V>* psiB, long ldB, cplx<T>* sticks, int mode) {
   __shared__ cplx<T> tw[N];
   __shared__ cplx<T> lines[2 * CPW * N];
   const int it = blockIdx.y;
@@ -321,8 +335,8 @@ __global__ __launch_bounds__(NT) void k_zinv_t(StickPlanDev P, const FftItem* it
   const int c1 = min(c0 + CPW, P.ncs);
   const int ncl = c1 - c0;
   const FftItem item = items[it];
-  const z64* A = psiA + static_cast<long>(item.a) * ldA;
-  const z64* B = item.b >= 0 ? psiB + static_cast<long>(item.b) * ldB : nullptr;
+  const cplx<V>* A = psiA + static_cast<long>(item.a) * ldA;
+  const cplx<V>* B = item.b >= 0 ? psiB + static_cast<long>(item.b) * ldB : nullptr;
   load_tw<T, N>(tw, P.tw3);
   for (int t = threadIdx.x; t < 2 * ncl * N; t += NT) lines[t] = mk<T>(T(0), T(0));
   __syncthreads();
@@ -332,8 +346,8 @@ __global__ __launch_bounds__(NT) void k_zinv_t(StickPlanDev P, const FftItem* it
     const int lc = P.pt_col[s] - c0;
     int z = P.pt_k[s] % N;
     if (z < 0) z += N;
-    const z64 a1 = A[s];
-    const z64 a2 = B ? B[s] : mk(0.0, 0.0);
+    const cplx<V> a1 = A[s];
+    const cplx<V> a2 = B ? B[s] : mk<V>(V(0), V(0));
     lines[(2 * lc) * N + z] = mk<T>(T(a1.x - a2.y), T(a1.y + a2.x));
   }
   __syncthreads();
@@ -341,15 +355,15 @@ __global__ __launch_bounds__(NT) void k_zinv_t(StickPlanDev P, const FftItem* it
     const int lc = P.pt_col[s] - c0;
     int zm = (-P.pt_k[s]) % N;
     if (zm < 0) zm += N;
-    const z64 a1 = A[s];
-    const z64 a2 = B ? B[s] : mk(0.0, 0.0);
-    z64 v = mk(a1.x + a2.y, a2.x - a1.y);
-    if (mode == ZINV_PAIR_REALG0 && s == 0) v = mk(a1.x, a2.x);
+    const cplx<V> a1 = A[s];
+    const cplx<V> a2 = B ? B[s] : mk<V>(V(0), V(0));
+    cplx<V> v = mk<V>(a1.x + a2.y, a2.x - a1.y);
+    if (mode == ZINV_PAIR_REALG0 && s == 0) v = mk<V>(a1.x, a2.x);
     const int line = (c0 + lc == 0) ? 0 : 2 * lc + 1;
     lines[line * N + zm] = mk<T>(T(v.x), T(v.y));
   }
   __syncthreads();
-  fft_inplace<T, N, 2 * CPW>(2 * ncl, lines, tw, +1);
+  fft_inplace<T, N, 2 * CPW, NT>(2 * ncl, lines, tw, +1);
   cplx<T>* out = sticks + static_cast<long>(it) * N * P.nct;
   for (int t = threadIdx.x; t < N * 2 * ncl; t += NT) {
     const int z = t / (2 * ncl);
@@ -366,9 +380,9 @@ __global__ __launch_bounds__(NT) void k_zinv_t(StickPlanDev P, const FftItem* it
 // ---------------------------------------------------------------------------
 // sticks -> sphere
 // ---------------------------------------------------------------------------
-template <typename T, int N>
+template <typename T, int N, typename V>
 __global__ __launch_bounds__(NT) void k_zfwd_t(StickPlanDev P, const FftItem* items, const cplx<T>* sticks,
-                                               const z64* in, long ldin, z64* out, long ldout, ZfwdArgs A) {
+                                               const cplx<V>* in, long ldin, cplx<V>* out, long ldout, ZfwdArgs A) {
   __shared__ cplx<T> tw[N];
   __shared__ cplx<T> lines[2 * CPW * N];
   const int it = blockIdx.y;
@@ -394,11 +408,11 @@ __global__ __launch_bounds__(NT) void k_zfwd_t(StickPlanDev P, const FftItem* it
     lines[(single ? lc : 2 * lc + (own ? 0 : 1)) * N + z] = v;
   }
   __syncthreads();
-  fft_inplace<T, N, 2 * CPW>(nl, lines, tw, -1);
+  fft_inplace<T, N, 2 * CPW, NT>(nl, lines, tw, -1);
   const int s0 = P.col_off[c0];
   const int s1 = P.col_off[c1];
-  z64* oa = out + static_cast<long>(item.a) * ldout;
-  z64* ob = item.b >= 0 ? out + static_cast<long>(item.b) * ldout : nullptr;
+  cplx<V>* oa = out + static_cast<long>(item.a) * ldout;
+  cplx<V>* ob = item.b >= 0 ? out + static_cast<long>(item.b) * ldout : nullptr;
   for (int s = s0 + threadIdx.x; s < s1; s += NT) {
     const int lc = P.pt_col[s] - c0;
     const int k = P.pt_k[s];
@@ -407,8 +421,8 @@ __global__ __launch_bounds__(NT) void k_zfwd_t(StickPlanDev P, const FftItem* it
     if (single) {
       const cplx<T> F = lines[lc * N + z];
       const double w = A.scale * (A.weight ? A.weight[s] : 1.0);
-      z64 v = mk(static_cast<double>(F.x) * w, static_cast<double>(F.y) * w);
-      if (A.zero_g0 && s == 0) v = mk(0.0, 0.0);
+      cplx<V> v = mk<V>(V(static_cast<double>(F.x) * w), V(static_cast<double>(F.y) * w));
+      if (A.zero_g0 && s == 0) v = mk<V>(V(0), V(0));
       oa[s] = v;
       continue;
     }
@@ -417,22 +431,22 @@ __global__ __launch_bounds__(NT) void k_zfwd_t(StickPlanDev P, const FftItem* it
     const cplx<T> Ft = lines[(2 * lc) * N + z];
     const int ml = (c0 + lc == 0) ? 0 : 2 * lc + 1;
     const cplx<T> Fmt = lines[ml * N + zm];
-    const z64 F = mk(static_cast<double>(Ft.x), static_cast<double>(Ft.y));
-    const z64 Fm = mk(static_cast<double>(Fmt.x), static_cast<double>(Fmt.y));
-    const z64 fp = cscale(F + Fm, 0.5);
-    const z64 fm = cscale(F - Fm, 0.5);
-    const z64 b1 = mk(fp.x, fm.y);
-    const z64 b2 = mk(fp.y, -fm.x);
+    const cplx<V> F = mk<V>(V(Ft.x), V(Ft.y));
+    const cplx<V> Fm = mk<V>(V(Fmt.x), V(Fmt.y));
+    const cplx<V> fp = cscale(F + Fm, V(0.5));
+    const cplx<V> fm = cscale(F - Fm, V(0.5));
+    const cplx<V> b1 = mk<V>(fp.x, fm.y);
+    const cplx<V> b2 = mk<V>(fp.y, -fm.x);
     if (A.mode == ZFWD_PAIR_STORE) {
-      oa[s] = cscale(b1, A.scale);
-      if (ob) ob[s] = cscale(b2, A.scale);
+      oa[s] = cscale(b1, V(A.scale));
+      if (ob) ob[s] = cscale(b2, V(A.scale));
     } else {
       const double g2 = A.g2kin[s];
-      const z64 ia = in[static_cast<long>(item.a) * ldin + s];
-      oa[s] = b1 + cscale(ia, g2 - A.e[item.a]);
+      const cplx<V> ia = in[static_cast<long>(item.a) * ldin + s];
+      oa[s] = b1 + cscale(ia, V(g2 - A.e[item.a]));
       if (ob) {
-        const z64 ib = in[static_cast<long>(item.b) * ldin + s];
-        ob[s] = b2 + cscale(ib, g2 - A.e[item.b]);
+        const cplx<V> ib = in[static_cast<long>(item.b) * ldin + s];
+        ob[s] = b2 + cscale(ib, V(g2 - A.e[item.b]));
       }
     }
   }
@@ -460,9 +474,9 @@ __device__ __forceinline__ PlaneLds<T, N> plane_lds() {
 
 template <typename T, int N>
 __device__ __forceinline__ void plane_fill(const StickPlanDev& P, PlaneLds<T, N>& S, const cplx<T>* src) {
-  for (int t = threadIdx.x; t < P.nrows * N; t += NT) S.pl[t] = mk<T>(T(0), T(0));
+  for (int t = threadIdx.x; t < P.nrows * N; t += NTP) S.pl[t] = mk<T>(T(0), T(0));
   __syncthreads();
-  for (int c = threadIdx.x; c < P.nct; c += NT) S.pl[P.col_pos[c]] = src[c];
+  for (int c = threadIdx.x; c < P.nct; c += NTP) S.pl[P.col_pos[c]] = src[c];
   __syncthreads();
 }
 
@@ -473,10 +487,12 @@ __device__ __forceinline__ int row_of(int x, int N, int imax) {
   return -1;
 }
 
-template <typename T, int N>
-__global__ __launch_bounds__(NT) void k_plane_vloc_t(StickPlanDev P, const FftItem* items, const cplx<T>* sin,
-                                                     cplx<T>* sout, const double* vpot, long ldpot, double scale) {
-  constexpr int JC = x_chunk<T, N>(false);
+// PT: storage type of the potential (double, or float for the FP32 operator);
+// OCC: workgroups per CU the LDS footprint is sized for
+template <typename T, int N, typename PT, int OCC>
+__global__ __launch_bounds__(NTP) __attribute__((amdgpu_waves_per_eu(2 * OCC))) void k_plane_vloc_t(StickPlanDev P, const FftItem* items, const cplx<T>* sin,
+                                                     cplx<T>* sout, const PT* vpot, long ldpot, double scale) {
+  constexpr int JC = x_chunk<T, N>(false, OCC);
   constexpr int MR = max_rows<N>();
   PlaneLds<T, N> S = plane_lds<T, N>();
   const int z = blockIdx.x;
@@ -486,8 +502,8 @@ __global__ __launch_bounds__(NT) void k_plane_vloc_t(StickPlanDev P, const FftIt
   const int imax = P.imax;
   load_tw<T, N>(S.tw, P.tw1);
   plane_fill<T, N>(P, S, sin + static_cast<long>(item.a) * sl + static_cast<long>(z) * P.nct);
-  fft_inplace<T, N, MR>(P.nrows, S.pl, S.tw, +1);
-  const double* V = vpot + (item.pot >= 0 ? static_cast<long>(item.pot) * ldpot : 0L) + static_cast<long>(z) * N * N;
+  fft_inplace<T, N, MR, NTP>(P.nrows, S.pl, S.tw, +1);
+  const PT* V = vpot + (item.pot >= 0 ? static_cast<long>(item.pot) * ldpot : 0L) + static_cast<long>(z) * N * N;
   const T sc = T(scale);
   cplx<T>* pl = S.pl;
   cplx<T>* xb = S.x;
@@ -501,22 +517,22 @@ __global__ __launch_bounds__(NT) void k_plane_vloc_t(StickPlanDev P, const FftIt
     auto times_v = [=](int jj, int x, cplx<T> v) {
       xb[jj * N + x] = cscale(v, T(V[(j0 + jj) * N + x]) * sc);
     };
-    passes<T, N, 0, JC>(nj, xb, S.tw, +1, from_rows, times_v);
+    passes<T, N, 0, JC, NTP>(nj, xb, S.tw, +1, from_rows, times_v);
     // forward along a1: only |i| <= imax frequencies go back to the rows
     auto from_x = [=](int jj, int x) { return xb[jj * N + x]; };
     auto to_rows = [=](int jj, int x, cplx<T> v) {
       const int r = row_of(x, N, imax);
       if (r >= 0) pl[r * N + j0 + jj] = v;
     };
-    passes<T, N, 0, JC>(nj, xb, S.tw, -1, from_x, to_rows);
+    passes<T, N, 0, JC, NTP>(nj, xb, S.tw, -1, from_x, to_rows);
   }
-  fft_inplace<T, N, MR>(P.nrows, S.pl, S.tw, -1);
+  fft_inplace<T, N, MR, NTP>(P.nrows, S.pl, S.tw, -1);
   cplx<T>* dst = sout + static_cast<long>(it) * sl + static_cast<long>(z) * P.nct;
-  for (int c = threadIdx.x; c < P.nct; c += NT) dst[c] = S.pl[P.col_pos[c]];
+  for (int c = threadIdx.x; c < P.nct; c += NTP) dst[c] = S.pl[P.col_pos[c]];
 }
 
 template <typename T, int N>
-__global__ __launch_bounds__(NT) void k_plane_to_real_t(StickPlanDev P, const FftItem* items, const cplx<T>* sin,
+__global__ __launch_bounds__(NTP) void k_plane_to_real_t(StickPlanDev P, const FftItem* items, const cplx<T>* sin,
                                                         double* out, long ldout) {
   constexpr int JC = x_chunk<T, N>(false);
   constexpr int MR = max_rows<N>();
@@ -528,7 +544,7 @@ __global__ __launch_bounds__(NT) void k_plane_to_real_t(StickPlanDev P, const Ff
   const int imax = P.imax;
   load_tw<T, N>(S.tw, P.tw1);
   plane_fill<T, N>(P, S, sin + static_cast<long>(it) * sl + static_cast<long>(z) * P.nct);
-  fft_inplace<T, N, MR>(P.nrows, S.pl, S.tw, +1);
+  fft_inplace<T, N, MR, NTP>(P.nrows, S.pl, S.tw, +1);
   double* oa = out + static_cast<long>(item.a) * ldout + static_cast<long>(z) * N * N;
   double* ob = item.b >= 0 ? out + static_cast<long>(item.b) * ldout + static_cast<long>(z) * N * N : nullptr;
   cplx<T>* pl = S.pl;
@@ -542,12 +558,12 @@ __global__ __launch_bounds__(NT) void k_plane_to_real_t(StickPlanDev P, const Ff
       oa[(j0 + jj) * N + x] = static_cast<double>(v.x);
       if (ob) ob[(j0 + jj) * N + x] = static_cast<double>(v.y);
     };
-    passes<T, N, 0, JC>(nj, S.x, S.tw, +1, from_rows, to_out);
+    passes<T, N, 0, JC, NTP>(nj, S.x, S.tw, +1, from_rows, to_out);
   }
 }
 
 template <typename T, int N>
-__global__ __launch_bounds__(NT) void k_plane_density_t(StickPlanDev P, int nband, const cplx<T>* sin,
+__global__ __launch_bounds__(NTP) void k_plane_density_t(StickPlanDev P, int nband, const cplx<T>* sin,
                                                         const double* occ, cplx<T>* sout) {
   constexpr int JC = x_chunk<T, N>(true);
   constexpr int MR = max_rows<N>();
@@ -558,11 +574,11 @@ __global__ __launch_bounds__(NT) void k_plane_density_t(StickPlanDev P, int nban
   const long sl = static_cast<long>(N) * P.nct;
   const int imax = P.imax;
   load_tw<T, N>(S.tw, P.tw1);
-  for (int t = threadIdx.x; t < N * N; t += NT) acc[t] = T(0);
+  for (int t = threadIdx.x; t < N * N; t += NTP) acc[t] = T(0);
   cplx<T>* pl = S.pl;
   for (int v = 0; v < nband; ++v) {
     plane_fill<T, N>(P, S, sin + (static_cast<long>(p) * nband + v) * sl + static_cast<long>(z) * P.nct);
-    fft_inplace<T, N, MR>(P.nrows, S.pl, S.tw, +1);
+    fft_inplace<T, N, MR, NTP>(P.nrows, S.pl, S.tw, +1);
     const T o = T(occ[v]);
     for (int j0 = 0; j0 < N; j0 += JC) {
       const int nj = min(JC, N - j0);
@@ -572,7 +588,7 @@ __global__ __launch_bounds__(NT) void k_plane_density_t(StickPlanDev P, int nban
       };
       // each (jj, x) is produced by exactly one thread: no race on acc
       auto accumulate = [=](int jj, int x, cplx<T> w) { acc[(j0 + jj) * N + x] += o * w.x * w.y; };
-      passes<T, N, 0, JC>(nj, S.x, S.tw, +1, from_rows, accumulate);
+      passes<T, N, 0, JC, NTP>(nj, S.x, S.tw, +1, from_rows, accumulate);
     }
   }
   // forward of the real density: a1 then a2, keeping |i| <= imax and the columns
@@ -583,19 +599,19 @@ __global__ __launch_bounds__(NT) void k_plane_density_t(StickPlanDev P, int nban
       const int r = row_of(x, N, imax);
       if (r >= 0) pl[r * N + j0 + jj] = w;
     };
-    passes<T, N, 0, JC>(nj, S.x, S.tw, -1, from_acc, to_rows);
+    passes<T, N, 0, JC, NTP>(nj, S.x, S.tw, -1, from_acc, to_rows);
   }
-  fft_inplace<T, N, MR>(P.nrows, S.pl, S.tw, -1);
+  fft_inplace<T, N, MR, NTP>(P.nrows, S.pl, S.tw, -1);
   cplx<T>* dst = sout + static_cast<long>(p) * sl + static_cast<long>(z) * P.nct;
-  for (int c = threadIdx.x; c < P.nct; c += NT) dst[c] = S.pl[P.col_pos[c]];
+  for (int c = threadIdx.x; c < P.nct; c += NTP) dst[c] = S.pl[P.col_pos[c]];
 }
 
 // ---------------------------------------------------------------------------
 // dispatch
 // ---------------------------------------------------------------------------
 template <typename T, int N>
-size_t plane_smem(bool density) {
-  const int jc = x_chunk<T, N>(density);
+size_t plane_smem(bool density, int occ = 1) {
+  const int jc = occ == 1 ? x_chunk<T, N>(density, 1) : occ == 2 ? x_chunk<T, N>(density, 2) : x_chunk<T, N>(density, 3);
   size_t b = sizeof(cplx<T>) * (static_cast<size_t>(N) + static_cast<size_t>(max_rows<N>()) * N +
                                 static_cast<size_t>(jc) * N);
   if (density) b += sizeof(T) * static_cast<size_t>(N) * N;
@@ -612,36 +628,79 @@ void allow_smem(K kernel, size_t bytes) {
   done[key] = bytes;
 }
 
+template <typename T, int N, typename PT, int OCC>
+void vloc_occ(const StickPlanDev& P, const FftItem* it, int n, const void* si, void* so, const PT* v, long ldv,
+              double scale, hipStream_t s) {
+  const size_t sm = plane_smem<T, N>(false, OCC);
+  allow_smem(k_plane_vloc_t<T, N, PT, OCC>, sm);
+  k_plane_vloc_t<T, N, PT, OCC><<<dim3(N, n), NTP, sm, s>>>(P, it, static_cast<const cplx<T>*>(si),
+                                                            static_cast<cplx<T>*>(so), v, ldv, scale);
+}
+
+// occ: requested workgroups per CU, lowered to the largest that fits (<= 3)
+template <typename T, int N, typename PT>
+void vloc_any(const StickPlanDev& P, const FftItem* it, int n, const void* si, void* so, const PT* v, long ldv,
+              double scale, int occ, hipStream_t s) {
+  if constexpr (occ_fits<T, N>(3)) {
+    if (occ >= 3) return vloc_occ<T, N, PT, 3>(P, it, n, si, so, v, ldv, scale, s);
+  }
+  if constexpr (occ_fits<T, N>(2)) {
+    if (occ >= 2) return vloc_occ<T, N, PT, 2>(P, it, n, si, so, v, ldv, scale, s);
+  }
+  vloc_occ<T, N, PT, 1>(P, it, n, si, so, v, ldv, scale, s);
+}
+
+template <typename T, int N>
+constexpr int max_occ() {
+  return occ_fits<T, N>(3) ? 3 : occ_fits<T, N>(2) ? 2 : 1;
+}
+
 template <typename T, int N>
 struct Launch {
   static void zinv(const StickPlanDev& P, const FftItem* it, int n, const z64* a, long lda, const z64* b, long ldb,
                    void* st, int mode, hipStream_t s) {
-    k_zinv_t<T, N><<<dim3(ceil_div(P.ncs, CPW), n), NT, 0, s>>>(P, it, a, lda, b, ldb, static_cast<cplx<T>*>(st), mode);
+    k_zinv_t<T, N, double><<<dim3(ceil_div(P.ncs, CPW), n), NT, 0, s>>>(P, it, a, lda, b, ldb, static_cast<cplx<T>*>(st), mode);
   }
   static void zfwd(const StickPlanDev& P, const FftItem* it, int n, const void* st, const z64* in, long ldin, z64* out,
                    long ldout, const ZfwdArgs& A, hipStream_t s) {
-    k_zfwd_t<T, N><<<dim3(ceil_div(P.ncs, CPW), n), NT, 0, s>>>(P, it, static_cast<const cplx<T>*>(st), in, ldin, out,
+    k_zfwd_t<T, N, double><<<dim3(ceil_div(P.ncs, CPW), n), NT, 0, s>>>(P, it, static_cast<const cplx<T>*>(st), in, ldin, out,
                                                                  ldout, A);
   }
   static void vloc(const StickPlanDev& P, const FftItem* it, int n, const void* si, void* so, const double* v, long ldv,
-                   double scale, hipStream_t s) {
-    const size_t sm = plane_smem<T, N>(false);
-    allow_smem(k_plane_vloc_t<T, N>, sm);
-    k_plane_vloc_t<T, N><<<dim3(N, n), NT, sm, s>>>(P, it, static_cast<const cplx<T>*>(si), static_cast<cplx<T>*>(so),
-                                                     v, ldv, scale);
+                   double scale, int occ, hipStream_t s) {
+    vloc_any<T, N, double>(P, it, n, si, so, v, ldv, scale, occ, s);
   }
   static void to_real(const StickPlanDev& P, const FftItem* it, int n, const void* si, double* out, long ldout,
                       hipStream_t s) {
     const size_t sm = plane_smem<T, N>(false);
     allow_smem(k_plane_to_real_t<T, N>, sm);
-    k_plane_to_real_t<T, N><<<dim3(N, n), NT, sm, s>>>(P, it, static_cast<const cplx<T>*>(si), out, ldout);
+    k_plane_to_real_t<T, N><<<dim3(N, n), NTP, sm, s>>>(P, it, static_cast<const cplx<T>*>(si), out, ldout);
   }
   static void density(const StickPlanDev& P, int np, int nb, const void* si, const double* occ, void* so,
                       hipStream_t s) {
     const size_t sm = plane_smem<T, N>(true);
     allow_smem(k_plane_density_t<T, N>, sm);
-    k_plane_density_t<T, N><<<dim3(N, np), NT, sm, s>>>(P, nb, static_cast<const cplx<T>*>(si), occ,
+    k_plane_density_t<T, N><<<dim3(N, np), NTP, sm, s>>>(P, nb, static_cast<const cplx<T>*>(si), occ,
                                                         static_cast<cplx<T>*>(so));
+  }
+};
+
+// FP32 operator: FP32 sticks, vectors and potential
+template <int N>
+struct Launch32 {
+  static void zinv(const StickPlanDev& P, const FftItem* it, int n, const z32* a, long lda, const z32* b, long ldb,
+                   void* st, int mode, hipStream_t s) {
+    k_zinv_t<float, N, float><<<dim3(ceil_div(P.ncs, CPW), n), NT, 0, s>>>(P, it, a, lda, b, ldb,
+                                                                           static_cast<z32*>(st), mode);
+  }
+  static void zfwd(const StickPlanDev& P, const FftItem* it, int n, const void* st, const z32* in, long ldin, z32* out,
+                   long ldout, const ZfwdArgs& A, hipStream_t s) {
+    k_zfwd_t<float, N, float><<<dim3(ceil_div(P.ncs, CPW), n), NT, 0, s>>>(P, it, static_cast<const z32*>(st), in,
+                                                                           ldin, out, ldout, A);
+  }
+  static void vloc(const StickPlanDev& P, const FftItem* it, int n, const void* si, void* so, const float* v, long ldv,
+                   double scale, int occ, hipStream_t s) {
+    vloc_any<float, N, float>(P, it, n, si, so, v, ldv, scale, occ, s);
   }
 };
 
@@ -650,11 +709,13 @@ const TplKernels* table_for(int n) {
 #define WH_TPL_CASE(NN)                                                                                       \
   case NN: {                                                                                                  \
     static const TplKernels k{&Launch<T, NN>::zinv, &Launch<T, NN>::zfwd, &Launch<T, NN>::vloc,             \
-                              &Launch<T, NN>::to_real, &Launch<T, NN>::density, plane_smem<T, NN>(true) <= 160u * 1024u}; \
+                              &Launch<T, NN>::to_real, &Launch<T, NN>::density, plane_smem<T, NN>(true) <= 160u * 1024u, \
+                              max_occ<T, NN>()};                                                              \
     return &k;                                                                                                \
   }
   switch (n) {
     WH_TPL_CASE(18)
+    WH_TPL_CASE(20)
     WH_TPL_CASE(24)
     WH_TPL_CASE(30)
     WH_TPL_CASE(32)
@@ -677,7 +738,44 @@ const TplKernels* table_for(int n) {
 #undef WH_TPL_CASE
 }
 
+const TplKernels32* table32_for(int n) {
+#define WH_TPL32_CASE(NN)                                                                  \
+  case NN: {                                                                               \
+    static const TplKernels32 k{&Launch32<NN>::zinv, &Launch32<NN>::zfwd, &Launch32<NN>::vloc, max_occ<float, NN>()}; \
+    return &k;                                                                             \
+  }
+  switch (n) {
+    WH_TPL32_CASE(18)
+    WH_TPL32_CASE(20)
+    WH_TPL32_CASE(24)
+    WH_TPL32_CASE(30)
+    WH_TPL32_CASE(32)
+    WH_TPL32_CASE(36)
+    WH_TPL32_CASE(40)
+    WH_TPL32_CASE(45)
+    WH_TPL32_CASE(48)
+    WH_TPL32_CASE(54)
+    WH_TPL32_CASE(60)
+    WH_TPL32_CASE(64)
+    WH_TPL32_CASE(72)
+    WH_TPL32_CASE(80)
+    WH_TPL32_CASE(90)
+    WH_TPL32_CASE(96)
+    WH_TPL32_CASE(100)
+    WH_TPL32_CASE(108)
+    default:
+      return nullptr;
+  }
+#undef WH_TPL32_CASE
+}
+
 }  // namespace
+
+const TplKernels32* tpl_kernels_f32(const StickPlanDev& P) {
+  if (P.n1 != P.n2 || P.n2 != P.n3) return nullptr;
+  if (P.nrows > P.n1 / 2 + 2) return nullptr;
+  return table32_for(P.n1);
+}
 
 const TplKernels* tpl_kernels_f64(const StickPlanDev& P) {
   if (P.n1 != P.n2 || P.n2 != P.n3) return nullptr;

@@ -8,6 +8,27 @@ namespace wh {
 
 void prec_rho(int nc, int npw, int gstart, const int* l2i, const z64* g, z64* h, long ld, const double* g2kin,
               const double* eprec, double* rho, hipStream_t s);
+void prec_rho(int nc, int npw, int gstart, const int* l2i, const z32* g, z32* h, long ld, const double* g2kin,
+              const double* eprec, double* rho, hipStream_t s);
+void direction(int nc, int npw, const int* l2i, const double* beta, z32* h, const z32* hold, z32* bc, long ld,
+               hipStream_t s);
+void cg_update(int nc, int npw, int gstart, const int* l2i, const z32* h, z32* g, const z32* tc, z32* x, long ld,
+               double* ac_out, hipStream_t s);
+void kinetic_minus_e(int m, int npw, const z32* x, z32* t, long ld, const double* g2kin, const double* e,
+                     hipStream_t s);
+// mixed-precision refinement (u2i: bands of the update list)
+//   x[ib] += dx[ib]; dx[ib] = 0; xc[u] = x[ib]
+void fold_increment(int nu, int npw, const int* u2i, z64* x, long ldx, z32* dx, long ld32, z64* xc, hipStream_t s);
+//   r = tc[u] - b[ib]; rho[u] = <M^-1 r|r>; g[ib] = r (FP32)
+void true_residual(int nu, int npw, int gstart, const int* u2i, const z64* tc, const z64* b, long ldx,
+                   const double* g2kin, const double* eprec, z32* g, long ld32, double* rho, hipStream_t s);
+// out[r][c] = in[r][c] for c < cols, FP64 -> FP32, rows of ld_out (pad left untouched)
+void to_f32(int rows, int cols, const double* in, long ld_in, float* out, long ld_out, hipStream_t s);
+void to_f64(int rows, int cols, const float* in, long ld_in, double* out, long ld_out, hipStream_t s);
+// rows by list: out[u] = in[list[u]] (scatter = false) or out[list[u]] = in[u] (scatter = true)
+void copy_rows(int n, int npw, const int* list, const z64* in, long ld_in, z64* out, long ld_out, bool scatter,
+               hipStream_t s);
+void gather_real(int n, const int* list, const double* in, double* out, hipStream_t s);
 void direction(int nc, int npw, const int* l2i, const double* beta, z64* h, const z64* hold, z64* bc, long ld,
                hipStream_t s);
 void cg_update(int nc, int npw, int gstart, const int* l2i, const z64* h, z64* g, const z64* tc, z64* x, long ld,

@@ -31,87 +31,91 @@ __device__ __forceinline__ double gamma_dot_part(const z64* a, const z64* b, int
 }
 
 // h[ib] = g[ib] / max(1, g2kin/eprec[ib]);  rho[lb] = <h|g>   (linsolve 150-181)
-__global__ __launch_bounds__(NT) void k_prec_rho(int npw, int gstart, const int* l2i, const z64* g, z64* h,
+// V is the vector precision; the inner products are summed in FP64.
+template <typename V>
+__global__ __launch_bounds__(NT) void k_prec_rho(int npw, int gstart, const int* l2i, const cplx<V>* g, cplx<V>* h,
                                                  long ld, const double* g2kin, const double* eprec, double* rho) {
   __shared__ double red[NT / 64];
   const int lb = blockIdx.x;
   const int ib = l2i[lb];
-  const z64* gi = g + static_cast<long>(ib) * ld;
-  z64* hi = h + static_cast<long>(ib) * ld;
+  const cplx<V>* gi = g + static_cast<long>(ib) * ld;
+  cplx<V>* hi = h + static_cast<long>(ib) * ld;
   const double ep = eprec[ib];
   double s = 0.0;
   for (int q = threadIdx.x; q < npw; q += NT) {
-    const double f = 1.0 / fmax(1.0, g2kin[q] / ep);
-    const z64 gv = gi[q];
-    const z64 hv = mk(gv.x * f, gv.y * f);
+    const V f = V(1.0 / fmax(1.0, g2kin[q] / ep));
+    const cplx<V> gv = gi[q];
+    const cplx<V> hv = mk<V>(gv.x * f, gv.y * f);
     hi[q] = hv;
-    s += hv.x * gv.x + hv.y * gv.y;
+    s += static_cast<double>(hv.x) * gv.x + static_cast<double>(hv.y) * gv.y;
   }
   s = block_sum(s, red);
   if (threadIdx.x == 0) {
     double r = 2.0 * s;
-    if (gstart == 2) r -= hi[0].x * gi[0].x;
+    if (gstart == 2) r -= static_cast<double>(hi[0].x) * gi[0].x;
     rho[lb] = r;
   }
 }
 
 // h[ib] = -h[ib] + beta[lb] hold[ib];  bc[lb] = h[ib]   (linsolve 226-240)
-__global__ __launch_bounds__(NT) void k_direction(int npw, const int* l2i, const double* beta, z64* h,
-                                                  const z64* hold, z64* bc, long ld) {
+template <typename V>
+__global__ __launch_bounds__(NT) void k_direction(int npw, const int* l2i, const double* beta, cplx<V>* h,
+                                                  const cplx<V>* hold, cplx<V>* bc, long ld, long ldc) {
   const int lb = blockIdx.y;
   const int ib = l2i[lb];
   const int q = blockIdx.x * NT + threadIdx.x;
   if (q >= npw) return;
-  const double bt = beta[lb];
+  const V bt = V(beta[lb]);
   const long o = static_cast<long>(ib) * ld + q;
-  z64 v = h[o];
-  v = mk(-v.x, -v.y);
-  if (bt != 0.0) {
-    const z64 w = hold[o];
-    v = mk(v.x + bt * w.x, v.y + bt * w.y);
+  cplx<V> v = h[o];
+  v = mk<V>(-v.x, -v.y);
+  if (bt != V(0)) {
+    const cplx<V> w = hold[o];
+    v = mk<V>(v.x + bt * w.x, v.y + bt * w.y);
   }
   h[o] = v;
-  bc[static_cast<long>(lb) * ld + q] = v;
+  bc[static_cast<long>(lb) * ldc + q] = v;
 }
 
 // a = <h|g>, c = <h|t>; x -= (a/c) h; g -= (a/c) t   (linsolve 253-321)
-__global__ __launch_bounds__(NT) void k_cg_update(int npw, int gstart, const int* l2i, const z64* h, z64* g,
-                                                  const z64* tc, z64* x, long ld, double* ac_out) {
+template <typename V>
+__global__ __launch_bounds__(NT) void k_cg_update(int npw, int gstart, const int* l2i, const cplx<V>* h, cplx<V>* g,
+                                                  const cplx<V>* tc, cplx<V>* x, long ld, long ldc, double* ac_out) {
   __shared__ double red[NT / 64];
   const int lb = blockIdx.x;
   const int ib = l2i[lb];
-  const z64* hi = h + static_cast<long>(ib) * ld;
-  z64* gi = g + static_cast<long>(ib) * ld;
-  const z64* ti = tc + static_cast<long>(lb) * ld;
-  z64* xi = x + static_cast<long>(ib) * ld;
+  const cplx<V>* hi = h + static_cast<long>(ib) * ld;
+  cplx<V>* gi = g + static_cast<long>(ib) * ld;
+  const cplx<V>* ti = tc + static_cast<long>(lb) * ldc;
+  cplx<V>* xi = x + static_cast<long>(ib) * ld;
   double sa = 0.0, sc = 0.0;
   for (int q = threadIdx.x; q < npw; q += NT) {
-    const z64 hv = hi[q];
-    const z64 gv = gi[q];
-    const z64 tv = ti[q];
-    sa += hv.x * gv.x + hv.y * gv.y;
-    sc += hv.x * tv.x + hv.y * tv.y;
+    const cplx<V> hv = hi[q];
+    const cplx<V> gv = gi[q];
+    const cplx<V> tv = ti[q];
+    sa += static_cast<double>(hv.x) * gv.x + static_cast<double>(hv.y) * gv.y;
+    sc += static_cast<double>(hv.x) * tv.x + static_cast<double>(hv.y) * tv.y;
   }
   sa = block_sum(sa, red);
   sc = block_sum(sc, red);
   double a = 2.0 * sa, c = 2.0 * sc;
   if (gstart == 2) {
-    a -= hi[0].x * gi[0].x;
-    c -= hi[0].x * ti[0].x;
+    a -= static_cast<double>(hi[0].x) * gi[0].x;
+    c -= static_cast<double>(hi[0].x) * ti[0].x;
   }
   __syncthreads();   // every thread has read gi[0] before it is updated
-  const double step = a / c;
+  const V step = V(a / c);
   if (threadIdx.x == 0 && ac_out) {
     ac_out[2 * lb] = a;
     ac_out[2 * lb + 1] = c;
   }
   for (int q = threadIdx.x; q < npw; q += NT) {
-    const z64 hv = hi[q];
-    const z64 tv = ti[q];
-    z64 xv = xi[q];
-    z64 gv = gi[q];
-    xi[q] = mk(xv.x - step * hv.x, xv.y - step * hv.y);
-    gi[q] = mk(gv.x - step * tv.x, gv.y - step * tv.y);
+    const cplx<V> hv = hi[q];
+    const cplx<V> tv = ti[q];
+    cplx<V> xv = xi[q];
+    cplx<V> gv = gi[q];
+    xi[q] = mk<V>(xv.x - step * hv.x, xv.y - step * hv.y);
+    gi[q] = mk<V>(gv.x - step * tv.x, gv.y - step * tv.y);
   }
 }
 
@@ -148,11 +152,80 @@ __global__ void k_precondition(int m, int npw, const z64* f, z64* pf, long ld, c
 }
 
 // t[r] = (g2kin - e[r]) * x[r]      (k_psi - e psi)
-__global__ void k_kinetic(int m, int npw, const z64* x, z64* t, long ld, const double* g2kin, const double* e) {
+template <typename V>
+__global__ void k_kinetic(int m, int npw, const cplx<V>* x, cplx<V>* t, long ld, const double* g2kin,
+                          const double* e) {
   const int r = blockIdx.y;
   const int s = blockIdx.x * NT + threadIdx.x;
   if (s >= npw || r >= m) return;
-  t[static_cast<long>(r) * ld + s] = cscale(x[static_cast<long>(r) * ld + s], g2kin[s] - e[r]);
+  t[static_cast<long>(r) * ld + s] = cscale(x[static_cast<long>(r) * ld + s], V(g2kin[s] - e[r]));
+}
+
+// x[ib] += dx[ib] (FP32 increment folded into the FP64 iterate); dx[ib] = 0; xc[u] = x[ib]
+__global__ void k_fold(int npw, const int* u2i, z64* x, long ldx, z32* dx, long ld32, z64* xc) {
+  const int u = blockIdx.y;
+  const int ib = u2i[u];
+  const int q = blockIdx.x * NT + threadIdx.x;
+  if (q >= npw) return;
+  const z32 d = dx[static_cast<long>(ib) * ld32 + q];
+  const long o = static_cast<long>(ib) * ldx + q;
+  const z64 v = mk(x[o].x + static_cast<double>(d.x), x[o].y + static_cast<double>(d.y));
+  x[o] = v;
+  dx[static_cast<long>(ib) * ld32 + q] = mk(0.f, 0.f);
+  xc[static_cast<long>(u) * ldx + q] = v;
+}
+
+// FP64 residual of the refinement step: r = (A x)[u] - b[ib];
+// rho[u] = <M^-1 r|r> (Gamma product); the FP32 iteration restarts from g[ib] = r.
+__global__ __launch_bounds__(NT) void k_true_residual(int npw, int gstart, const int* u2i, const z64* tc,
+                                                      const z64* b, long ldx, const double* g2kin,
+                                                      const double* eprec, z32* g, long ld32, double* rho) {
+  __shared__ double red[NT / 64];
+  const int u = blockIdx.x;
+  const int ib = u2i[u];
+  const z64* ti = tc + static_cast<long>(u) * ldx;
+  const z64* bi = b + static_cast<long>(ib) * ldx;
+  z32* gi = g + static_cast<long>(ib) * ld32;
+  const double ep = eprec[ib];
+  double s = 0.0, r0 = 0.0;
+  for (int q = threadIdx.x; q < npw; q += NT) {
+    const z64 r = ti[q] - bi[q];
+    const double f = 1.0 / fmax(1.0, g2kin[q] / ep);
+    s += f * (r.x * r.x + r.y * r.y);
+    if (q == 0) r0 = f * r.x * r.x;
+    gi[q] = mk(static_cast<float>(r.x), static_cast<float>(r.y));
+  }
+  s = block_sum(s, red);
+  r0 = block_sum(r0, red);
+  if (threadIdx.x == 0) rho[u] = 2.0 * s - (gstart == 2 ? r0 : 0.0);
+}
+
+__global__ void k_to_f64(int cols, const float* in, long ld_in, double* out, long ld_out) {
+  const int r = blockIdx.y;
+  const long c = static_cast<long>(blockIdx.x) * NT + threadIdx.x;
+  if (c >= cols) return;
+  out[static_cast<long>(r) * ld_out + c] = static_cast<double>(in[static_cast<long>(r) * ld_in + c]);
+}
+
+__global__ void k_copy_rows(int npw, const int* list, const z64* in, long ld_in, z64* out, long ld_out, int scatter) {
+  const int u = blockIdx.y;
+  const int q = blockIdx.x * NT + threadIdx.x;
+  if (q >= npw) return;
+  const long src = static_cast<long>(scatter ? u : list[u]) * ld_in + q;
+  const long dst = static_cast<long>(scatter ? list[u] : u) * ld_out + q;
+  out[dst] = in[src];
+}
+
+__global__ void k_gather_real(int n, const int* list, const double* in, double* out) {
+  const int u = blockIdx.x * NT + threadIdx.x;
+  if (u < n) out[u] = in[list[u]];
+}
+
+__global__ void k_to_f32(int cols, const double* in, long ld_in, float* out, long ld_out) {
+  const int r = blockIdx.y;
+  const long c = static_cast<long>(blockIdx.x) * NT + threadIdx.x;
+  if (c >= cols) return;
+  out[static_cast<long>(r) * ld_out + c] = static_cast<float>(in[static_cast<long>(r) * ld_in + c]);
 }
 
 // y = a*x + b*y over n complex
@@ -202,21 +275,42 @@ __global__ __launch_bounds__(NT) void k_eprec(int npw, const z64* evc, long ld, 
 void prec_rho(int nc, int npw, int gstart, const int* l2i, const z64* g, z64* h, long ld, const double* g2kin,
               const double* eprec, double* rho, hipStream_t s) {
   if (nc <= 0) return;
-  k_prec_rho<<<nc, NT, 0, s>>>(npw, gstart, l2i, g, h, ld, g2kin, eprec, rho);
+  k_prec_rho<double><<<nc, NT, 0, s>>>(npw, gstart, l2i, g, h, ld, g2kin, eprec, rho);
+  WH_HIP(hipGetLastError());
+}
+
+void prec_rho(int nc, int npw, int gstart, const int* l2i, const z32* g, z32* h, long ld, const double* g2kin,
+              const double* eprec, double* rho, hipStream_t s) {
+  if (nc <= 0) return;
+  k_prec_rho<float><<<nc, NT, 0, s>>>(npw, gstart, l2i, g, h, ld, g2kin, eprec, rho);
   WH_HIP(hipGetLastError());
 }
 
 void direction(int nc, int npw, const int* l2i, const double* beta, z64* h, const z64* hold, z64* bc, long ld,
                hipStream_t s) {
   if (nc <= 0) return;
-  k_direction<<<dim3(ceil_div(npw, NT), nc), NT, 0, s>>>(npw, l2i, beta, h, hold, bc, ld);
+  k_direction<double><<<dim3(ceil_div(npw, NT), nc), NT, 0, s>>>(npw, l2i, beta, h, hold, bc, ld, ld);
+  WH_HIP(hipGetLastError());
+}
+
+void direction(int nc, int npw, const int* l2i, const double* beta, z32* h, const z32* hold, z32* bc, long ld,
+               hipStream_t s) {
+  if (nc <= 0) return;
+  k_direction<float><<<dim3(ceil_div(npw, NT), nc), NT, 0, s>>>(npw, l2i, beta, h, hold, bc, ld, ld);
   WH_HIP(hipGetLastError());
 }
 
 void cg_update(int nc, int npw, int gstart, const int* l2i, const z64* h, z64* g, const z64* tc, z64* x, long ld,
                double* ac_out, hipStream_t s) {
   if (nc <= 0) return;
-  k_cg_update<<<nc, NT, 0, s>>>(npw, gstart, l2i, h, g, tc, x, ld, ac_out);
+  k_cg_update<double><<<nc, NT, 0, s>>>(npw, gstart, l2i, h, g, tc, x, ld, ld, ac_out);
+  WH_HIP(hipGetLastError());
+}
+
+void cg_update(int nc, int npw, int gstart, const int* l2i, const z32* h, z32* g, const z32* tc, z32* x, long ld,
+               double* ac_out, hipStream_t s) {
+  if (nc <= 0) return;
+  k_cg_update<float><<<nc, NT, 0, s>>>(npw, gstart, l2i, h, g, tc, x, ld, ld, ac_out);
   WH_HIP(hipGetLastError());
 }
 
@@ -244,7 +338,52 @@ void precondition(int m, int npw, const z64* f, z64* pf, long ld, const double* 
 void kinetic_minus_e(int m, int npw, const z64* x, z64* t, long ld, const double* g2kin, const double* e,
                      hipStream_t s) {
   if (m <= 0) return;
-  k_kinetic<<<dim3(ceil_div(npw, NT), m), NT, 0, s>>>(m, npw, x, t, ld, g2kin, e);
+  k_kinetic<double><<<dim3(ceil_div(npw, NT), m), NT, 0, s>>>(m, npw, x, t, ld, g2kin, e);
+  WH_HIP(hipGetLastError());
+}
+
+void kinetic_minus_e(int m, int npw, const z32* x, z32* t, long ld, const double* g2kin, const double* e,
+                     hipStream_t s) {
+  if (m <= 0) return;
+  k_kinetic<float><<<dim3(ceil_div(npw, NT), m), NT, 0, s>>>(m, npw, x, t, ld, g2kin, e);
+  WH_HIP(hipGetLastError());
+}
+
+void fold_increment(int nu, int npw, const int* u2i, z64* x, long ldx, z32* dx, long ld32, z64* xc, hipStream_t s) {
+  if (nu <= 0) return;
+  k_fold<<<dim3(ceil_div(npw, NT), nu), NT, 0, s>>>(npw, u2i, x, ldx, dx, ld32, xc);
+  WH_HIP(hipGetLastError());
+}
+
+void true_residual(int nu, int npw, int gstart, const int* u2i, const z64* tc, const z64* b, long ldx,
+                   const double* g2kin, const double* eprec, z32* g, long ld32, double* rho, hipStream_t s) {
+  if (nu <= 0) return;
+  k_true_residual<<<nu, NT, 0, s>>>(npw, gstart, u2i, tc, b, ldx, g2kin, eprec, g, ld32, rho);
+  WH_HIP(hipGetLastError());
+}
+
+void to_f64(int rows, int cols, const float* in, long ld_in, double* out, long ld_out, hipStream_t s) {
+  if (rows <= 0 || cols <= 0) return;
+  k_to_f64<<<dim3(ceil_div(cols, NT), rows), NT, 0, s>>>(cols, in, ld_in, out, ld_out);
+  WH_HIP(hipGetLastError());
+}
+
+void copy_rows(int n, int npw, const int* list, const z64* in, long ld_in, z64* out, long ld_out, bool scatter,
+               hipStream_t s) {
+  if (n <= 0) return;
+  k_copy_rows<<<dim3(ceil_div(npw, NT), n), NT, 0, s>>>(npw, list, in, ld_in, out, ld_out, scatter ? 1 : 0);
+  WH_HIP(hipGetLastError());
+}
+
+void gather_real(int n, const int* list, const double* in, double* out, hipStream_t s) {
+  if (n <= 0) return;
+  k_gather_real<<<ceil_div(n, NT), NT, 0, s>>>(n, list, in, out);
+  WH_HIP(hipGetLastError());
+}
+
+void to_f32(int rows, int cols, const double* in, long ld_in, float* out, long ld_out, hipStream_t s) {
+  if (rows <= 0 || cols <= 0) return;
+  k_to_f32<<<dim3(ceil_div(cols, NT), rows), NT, 0, s>>>(cols, in, ld_in, out, ld_out);
   WH_HIP(hipGetLastError());
 }
 

@@ -44,12 +44,14 @@ SIGNATURES = {
     "wh_set_states": ([I, P, P, P, D, D], I),
     "wh_set_coulomb": ([P], I),
     "wh_set_params": ([I, I], I),
+    "wh_set_precision": ([I, D], I),
     "wh_apply_sternheimerop_to_m_wfcs": ([I, P, P, P, P, I], I),
     "wh_linsolve_sternheimer_m_wfcts": ([I, I, P, P, P, P, D, P], I),
     "wh_dfpt": ([I, P, P, D, I, P], I),
     "wh_apply_operator": ([I, P, P, D, I], I),
     "wh_davidson_diago": ([P, P, P, P, P, I, P], I),
     "wh_get_counters": ([P, P], I),
+    "wh_get_precision_counters": ([P, P, P], I),
     "wh_profile": ([I], I),
     "wh_profile_query": ([C.c_char_p, P, P], I),
 }

@@ -166,11 +166,34 @@ def gpu_h_factory(gs):
     return h_psi
 
 
+def set_precision(mode: str = "fp64", refine_delta: float = 0.0):
+    """Arithmetic of the Sternheimer operator and solve: ``"fp64"`` (the
+    reference's), ``"mixed"`` (FP32 operator with 3-product split-BF16
+    projector GEMMs, CG in FP32 on an increment of the FP64 solution, FP64
+    residual recomputed when rho has dropped by ``refine_delta`` (0: library
+    default 1e-4); convergence is tested on the FP64 residual) or
+    ``"mixed6"`` (the same with 6-product split-BF16 GEMMs, about FP32
+    accurate)."""
+    modes = {"fp64": 0, "mixed": 1, "mixed6": 2}
+    if mode not in modes:
+        raise ValueError(f"precision mode {mode!r} is not one of {sorted(modes)}")
+    check(load().wh_set_precision(modes[mode], float(refine_delta)))
+
+
 def counters():
     lib = load()
     a, b = C.c_long(0), C.c_long(0)
     check(lib.wh_get_counters(C.addressof(a), C.addressof(b)))
     return int(a.value), int(b.value)
+
+
+def precision_counters():
+    """(FP32 operator applications, FP64 refinement residuals, bands finished
+    by the FP64 CG) so far."""
+    lib = load()
+    a, b, f = C.c_long(0), C.c_long(0), C.c_long(0)
+    check(lib.wh_get_precision_counters(C.addressof(a), C.addressof(b), C.addressof(f)))
+    return int(a.value), int(b.value), int(f.value)
 
 
 def profile(enable: bool):
