@@ -67,6 +67,13 @@ def main():
     ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"],
                     help="fp64: the FP64 solve; mixed: FP32 CG with FP64 refinement; both: the FP64 line "
                          "with the mixed-precision run of the same batch beside it")
+    ap.add_argument("--mixed-mode", default="mixed6", choices=["mixed", "mixed6"],
+                    help="mixed-precision variant: split-BF16 projector GEMMs with 3 (mixed) or 6 (mixed6) products")
+    ap.add_argument("--refine-delta", type=float, default=1e-4,
+                    help="mixed precision: FP64 residual recomputed when rho has dropped by this factor")
+    ap.add_argument("--hbm-calibration", action="store_true",
+                    help="run a plain streaming kernel of known size first (torch.neg over 1 GiB of float32: "
+                         "1 GiB read, 1 GiB written) to scale HBM byte counters of a profiler run")
     ap.add_argument("--no-davidson", action="store_true",
                     help="skip the Wstat Davidson wall-time run on the 6 H2 workload")
     ap.add_argument("--gs-cache", default=None,
@@ -83,6 +90,14 @@ def main():
     from west_amd.coulomb import sqvc_init
     from west_amd.groundstate import make_ground_state
     from west_amd.systems import make_si_system
+
+    if args.hbm_calibration:
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        torch.neg(a, out=b)
+        torch.cuda.synchronize()
+        del a, b
+        log("[bench] calibration kernel: neg over 2^28 float32 (1 GiB read + 1 GiB written)")
 
     kw, npert = WORKLOADS[args.workload]
     t_setup = time.perf_counter()
@@ -108,7 +123,7 @@ def main():
     nv = gs.nbndval
     pairs = npert * nv
 
-    head = _run_mode(wstat, torch, args, "mixed" if args.precision == "mixed" else "fp64", npert, dvg, dng)
+    head = _run_mode(wstat, torch, args, args.mixed_mode if args.precision == "mixed" else "fp64", npert, dvg, dng)
     value, ms_step = head["value"], head["ms_per_step"]
     stern_per_step, iters_per_step, kernels = head["stern_per_step"], head["iters_per_step"], head["kernels"]
 
@@ -124,14 +139,15 @@ def main():
     if args.precision == "both":
         dng64 = dng.clone()
         dng32 = torch.zeros_like(dng)
-        mx = _run_mode(wstat, torch, args, "mixed", npert, dvg, dng32)
+        mx = _run_mode(wstat, torch, args, args.mixed_mode, npert, dvg, dng32)
         mmodel = _byte_model(b, gs, npert, mx)
         mdom = max(mx["kernels"], key=lambda k: mx["kernels"][k]["ms_per_step"])
         l2 = float(torch.linalg.norm(dng32 - dng64) / torch.linalg.norm(dng64))
         mixed = {"value": round(mx["value"], 3), "unit": "pairs/s", "ms_per_step": round(mx["ms_per_step"], 3),
                  "speedup_vs_fp64": round(mx["value"] / value, 3),
                  "dtype": "f32 FFT/CG + split-bf16 MFMA projections (f32 accumulate) + f64 refinement",
-                 "refine_delta": 1e-4, "dng_l2_rel_err_vs_fp64": l2,
+                 "mode": args.mixed_mode, "bf16_products": 3 if args.mixed_mode == "mixed" else 6,
+                 "refine_delta": args.refine_delta, "dng_l2_rel_err_vs_fp64": l2,
                  "sternheimer_applications_per_step": mx["stern_per_step"],
                  "fp32_applications_per_step": mx["fp32_per_step"],
                  "fp64_refinements_per_step": mx["refine_per_step"],
@@ -203,7 +219,7 @@ def _davidson_h2(wstat, torch, args):
     log(f"[bench] {s.name}: npw={s.basis.npw} nr={s.basis.nr}; ground state in {time.perf_counter() - t:.1f}s")
     wstat.set_ground_state(gs, sqvc=sqvc_init(s.basis, gs.npwx))
     res = {}
-    for mode in ("fp64", "mixed"):
+    for mode in ("fp64", args.mixed_mode):
         wstat.set_precision(mode)
         torch.cuda.synchronize()
         c0 = wstat.counters()
@@ -212,7 +228,8 @@ def _davidson_h2(wstat, torch, args):
         torch.cuda.synchronize()
         wall = time.perf_counter() - t
         c1 = wstat.counters()
-        res[mode] = {"wall_s": round(wall, 3), "iterations": len(recs), "converged": int(conv.sum()),
+        res["fp64" if mode == "fp64" else "mixed"] = {
+                     "wall_s": round(wall, 3), "iterations": len(recs), "converged": int(conv.sum()),
                      "sternheimer_applications": c1[0] - c0[0], "ev": ev}
         log(f"[bench] davidson {mode}: {wall:.2f}s, {len(recs)} iterations, {int(conv.sum())}/32 converged")
     wstat.set_precision("fp64")
@@ -225,7 +242,7 @@ def _davidson_h2(wstat, torch, args):
 
 def _run_mode(wstat, torch, args, mode, npert, dvg, dng):
     """Warmup + timed chi0 batches in one precision mode; per-step counters and kernel times."""
-    wstat.set_precision(mode)
+    wstat.set_precision(mode, args.refine_delta if mode != "fp64" else 0.0)
     nv = wstat._state["gs"].nbndval
     pairs = npert * nv
     for i in range(args.warmup):

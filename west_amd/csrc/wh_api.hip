@@ -192,7 +192,7 @@ void stern_apply(const z64* x, z64* t, const double* d_e, int m, double alpha) {
 // Preconditioned CG  (DFPT_kernel/linsolve_sternheimer_m_wfcts.f90:107-332)
 // ---------------------------------------------------------------------------
 int linsolve_dev(int m, z64* b, z64* x, const std::vector<double>& e_col, const double* d_e_col,
-                 const double* d_eprec_col, double tr2, std::vector<int>* ierr_col) {
+                 const double* d_eprec_col, double tr2, std::vector<int>* ierr_col, int maxiter) {
   Context& c = ctx();
   hipStream_t s = c.stream;
   const int npw = c.npw;
@@ -215,7 +215,8 @@ int linsolve_dev(int m, z64* b, z64* x, const std::vector<double>& e_col, const 
   std::iota(l2i.begin(), l2i.end(), 0);
   std::vector<double> rho(m, 0.0), rhoold(m, 0.0), rho_c(m), beta(m), eu(m);
   int nc = m;
-  for (int iter = 1; iter <= c.n_dfpt_maxiter; ++iter) {
+  const int itmax = maxiter > 0 ? maxiter : c.n_dfpt_maxiter;
+  for (int iter = 1; iter <= itmax; ++iter) {
     WH_HIP(hipMemcpyAsync(d_l2i, l2i.data(), sizeof(int) * nc, hipMemcpyHostToDevice, s));
     WH_TIMED("cg_prec", prec_rho(nc, npw, c.gstart, d_l2i, g, h, ld, c.d_g2kin, d_eprec_col, d_rho, s));
     WH_HIP(hipMemcpyAsync(rho_c.data(), d_rho, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
@@ -374,14 +375,16 @@ int linsolve_mixed(int m, z64* b, z64* x, const std::vector<double>& e_col, cons
   WH_HIP(hipMemsetAsync(dx, 0, sizeof(z32) * v32, s));
 
   std::vector<char> is_conv(m, 0), to_fp64(m, 0);
-  std::vector<int> l2i(m), list, stall(m, 0);
+  std::vector<int> l2i(m), list, stall(m, 0), since(m, 0);
   std::vector<double> rho(m, 0.0), rhoold(m, 0.0), rho_ref(m, 0.0), tmp(m), beta(m), eu(m), eu64(m);
   list.reserve(m);
+  // FP32 iterations after which a refinement is forced
+  const int kforce = 40;
 
   // x += dx; r = A x - b in FP64; g = r in FP32; rho_ref = <M^-1 r|r>; converged if rho_ref < tr2.
-  // A band whose FP64 residual twice stays far above what the FP32 iteration
-  // reported (the FP32 operator is not accurate enough for its conditioning)
-  // is finished by the FP64 CG instead.
+  // A band whose FP64 residual twice drops less than tenfold between
+  // refinements (the FP32 operator is not accurate enough for its
+  // conditioning) is finished by the FP64 CG instead.
   auto refine = [&](const std::vector<int>& u, bool first) {
     const int nu = static_cast<int>(u.size());
     for (int k = 0; k < nu; ++k) eu64[k] = e_col[u[k]];
@@ -397,9 +400,10 @@ int linsolve_mixed(int m, z64* b, z64* x, const std::vector<double>& e_col, cons
       const int ib = u[k];
       const double prev = rho_ref[ib];
       rho_ref[ib] = tmp[k];
+      since[ib] = 0;
       if (tmp[k] < tr2) {
         is_conv[ib] = 1;
-      } else if (!first && tmp[k] > 0.1 * prev && tmp[k] > 100.0 * rho[ib]) {
+      } else if (!first && prev > 100.0 * tr2 && tmp[k] > 0.1 * prev) {
         if (++stall[ib] >= 2) to_fp64[ib] = 1;
       }
     }
@@ -421,12 +425,14 @@ int linsolve_mixed(int m, z64* b, z64* x, const std::vector<double>& e_col, cons
   std::vector<int> act;
   for (int ib = 0; ib < m; ++ib)
     if (!is_conv[ib]) act.push_back(ib);
+  int iter_done = 0;
   for (int iter = 1; iter <= c.n_dfpt_maxiter && !act.empty(); ++iter) {
+    iter_done = iter;
     precond(act, d_l2i);
     ++c.cg_iters;
     list.clear();
     for (int ib : act)
-      if (rho[ib] < tr2 || rho[ib] < c.refine_delta * rho_ref[ib]) list.push_back(ib);
+      if (rho[ib] < tr2 || rho[ib] < c.refine_delta * rho_ref[ib] || ++since[ib] >= kforce) list.push_back(ib);
     if (!list.empty()) {
       refine(list, false);
       std::vector<int> again;
@@ -439,6 +445,15 @@ int linsolve_mixed(int m, z64* b, z64* x, const std::vector<double>& e_col, cons
       if (!is_conv[ib] && !to_fp64[ib]) l2i[nc++] = ib;
     act.assign(l2i.begin(), l2i.begin() + nc);
     if (nc == 0) break;
+    // the last few slow bands: their iterations are latency-bound, and the
+    // FP64 CG restarted from the current FP64 iterate needs fewer of them
+    if (m >= 256 && nc <= m / 128 && iter >= 10) {
+      refine(act, true);
+      for (int ib : act)
+        if (!is_conv[ib]) to_fp64[ib] = 1;
+      act.clear();
+      break;
+    }
     for (int lb = 0; lb < nc; ++lb) {
       const int ib = l2i[lb];
       beta[lb] = (iter != 1) ? rho[ib] / rhoold[ib] : 0.0;
@@ -484,7 +499,7 @@ int linsolve_mixed(int m, z64* b, z64* x, const std::vector<double>& e_col, cons
     WH_HIP(hipStreamSynchronize(s));
     c.nfallback += nf;
     std::vector<int> ierr_f;
-    nonconv += linsolve_dev(nf, fb, fx, e_f, fe, fep, tr2, &ierr_f);
+    nonconv += linsolve_dev(nf, fb, fx, e_f, fe, fep, tr2, &ierr_f, std::max(1, c.n_dfpt_maxiter - iter_done));
     WH_HIP(hipMemcpyAsync(d_u2i, fl.data(), sizeof(int) * nf, hipMemcpyHostToDevice, s));
     copy_rows(nf, npw, d_u2i, fx, ld, x, ld, true, s);
     WH_HIP(hipStreamSynchronize(s));

@@ -135,8 +135,9 @@ void require_ready(bool need_states);
 
 // operations on device data in stick order (rows of npw complex, ld = npw)
 void stern_apply(const z64* x, z64* t, const double* d_e, int m, double alpha);
+// maxiter <= 0: n_dfpt_maxiter
 int linsolve_dev(int m, z64* b, z64* x, const std::vector<double>& e_col, const double* d_e_col,
-                 const double* d_eprec_col, double tr2, std::vector<int>* ierr_col);
+                 const double* d_eprec_col, double tr2, std::vector<int>* ierr_col, int maxiter = 0);
 int dfpt_dev(int m, const z64* dvg, z64* dng, double tr2, const double* out_weight, std::vector<int>* ierr_pert);
 void glbrak_q(int row0, int nq, const z64* f, int m, double* cmat);
 void stern_apply32(const z32* x, z32* t, const double* d_e, int m, double alpha);

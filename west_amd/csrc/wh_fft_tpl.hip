@@ -22,7 +22,12 @@ namespace wh {
 namespace {
 
 constexpr int NT = 256;
-constexpr int CPW = 8;
+// stored columns per workgroup of the z-kernels: one z row of a workgroup's
+// columns is a 128-byte line of the sticks buffer
+template <typename T>
+constexpr int cpw() {
+  return 128 / static_cast<int>(sizeof(cplx<T>));
+}
 
 // ---------------------------------------------------------------------------
 // compile-time trigonometry (Taylor series after reduction to [-pi/4, pi/4])
@@ -241,12 +246,24 @@ __device__ __forceinline__ void pass(int nlines, const cplx<T>* tw, int dir, Ld 
       const int i = bf - line * NBL;
       const int k = (P == 1) ? 0 : i % P;
       if (P > 1 && k != 0) {
+        cplx<T> w[R];
+        if constexpr (sizeof(T) == 4) {
+          // FP32: w^j from one LDS read of w = root^(k STEP), w^j = w^(j/2) w^(j - j/2)
+          // (at most log2(R) roundings deep; saves R - 2 LDS reads per butterfly)
+          w[1] = tw[k * STEP];
+          if (dir > 0) w[1].y = -w[1].y;
 #pragma unroll
-        for (int j = 1; j < R; ++j) {
-          cplx<T> w = tw[j * k * STEP];
-          if (dir > 0) w.y = -w.y;
-          v[t][j] = cmul(v[t][j], w);
+          for (int j = 2; j < R; ++j) w[j] = cmul(w[j / 2], w[j - j / 2]);
+        } else {
+          // FP64: table roots (FP64 products cost more than the LDS reads they save)
+#pragma unroll
+          for (int j = 1; j < R; ++j) {
+            w[j] = tw[j * k * STEP];
+            if (dir > 0) w[j].y = -w[j].y;
+          }
         }
+#pragma unroll
+        for (int j = 1; j < R; ++j) v[t][j] = cmul(v[t][j], w[j]);
       }
       dft<T, R>(v[t], dir);
       const int base = (i - k) * R + k;
@@ -328,6 +345,7 @@ constexpr int x_chunk(bool density, int occ = 1) {
 template <typename T, int N, typename V>
 __global__ __launch_bounds__(NT) void k_zinv_t(StickPlanDev P, const FftItem* items, const cplx<V>* psiA, long ldA,
                                                const cplx<V>* psiB, long ldB, cplx<T>* sticks, int mode) {
+  constexpr int CPW = cpw<T>();
   __shared__ cplx<T> tw[N];
   __shared__ cplx<T> lines[2 * CPW * N];
   const int it = blockIdx.y;
@@ -383,6 +401,7 @@ __global__ __launch_bounds__(NT) void k_zinv_t(StickPlanDev P, const FftItem* it
 template <typename T, int N, typename V>
 __global__ __launch_bounds__(NT) void k_zfwd_t(StickPlanDev P, const FftItem* items, const cplx<T>* sticks,
                                                const cplx<V>* in, long ldin, cplx<V>* out, long ldout, ZfwdArgs A) {
+  constexpr int CPW = cpw<T>();
   __shared__ cplx<T> tw[N];
   __shared__ cplx<T> lines[2 * CPW * N];
   const int it = blockIdx.y;
@@ -659,11 +678,11 @@ template <typename T, int N>
 struct Launch {
   static void zinv(const StickPlanDev& P, const FftItem* it, int n, const z64* a, long lda, const z64* b, long ldb,
                    void* st, int mode, hipStream_t s) {
-    k_zinv_t<T, N, double><<<dim3(ceil_div(P.ncs, CPW), n), NT, 0, s>>>(P, it, a, lda, b, ldb, static_cast<cplx<T>*>(st), mode);
+    k_zinv_t<T, N, double><<<dim3(ceil_div(P.ncs, cpw<T>()), n), NT, 0, s>>>(P, it, a, lda, b, ldb, static_cast<cplx<T>*>(st), mode);
   }
   static void zfwd(const StickPlanDev& P, const FftItem* it, int n, const void* st, const z64* in, long ldin, z64* out,
                    long ldout, const ZfwdArgs& A, hipStream_t s) {
-    k_zfwd_t<T, N, double><<<dim3(ceil_div(P.ncs, CPW), n), NT, 0, s>>>(P, it, static_cast<const cplx<T>*>(st), in, ldin, out,
+    k_zfwd_t<T, N, double><<<dim3(ceil_div(P.ncs, cpw<T>()), n), NT, 0, s>>>(P, it, static_cast<const cplx<T>*>(st), in, ldin, out,
                                                                  ldout, A);
   }
   static void vloc(const StickPlanDev& P, const FftItem* it, int n, const void* si, void* so, const double* v, long ldv,
@@ -690,12 +709,12 @@ template <int N>
 struct Launch32 {
   static void zinv(const StickPlanDev& P, const FftItem* it, int n, const z32* a, long lda, const z32* b, long ldb,
                    void* st, int mode, hipStream_t s) {
-    k_zinv_t<float, N, float><<<dim3(ceil_div(P.ncs, CPW), n), NT, 0, s>>>(P, it, a, lda, b, ldb,
+    k_zinv_t<float, N, float><<<dim3(ceil_div(P.ncs, cpw<float>()), n), NT, 0, s>>>(P, it, a, lda, b, ldb,
                                                                            static_cast<z32*>(st), mode);
   }
   static void zfwd(const StickPlanDev& P, const FftItem* it, int n, const void* st, const z32* in, long ldin, z32* out,
                    long ldout, const ZfwdArgs& A, hipStream_t s) {
-    k_zfwd_t<float, N, float><<<dim3(ceil_div(P.ncs, CPW), n), NT, 0, s>>>(P, it, static_cast<const z32*>(st), in,
+    k_zfwd_t<float, N, float><<<dim3(ceil_div(P.ncs, cpw<float>()), n), NT, 0, s>>>(P, it, static_cast<const z32*>(st), in,
                                                                            ldin, out, ldout, A);
   }
   static void vloc(const StickPlanDev& P, const FftItem* it, int n, const void* si, void* so, const float* v, long ldv,
