@@ -27,6 +27,11 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+
+# Measured HBM bytes per transform item (band pair) of the plane kernels on the
+# si64 workload: rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a streaming kernel
+# of known size (profiles/plane_vloc_si64_hbm_traffic.md).
+TRAFFIC_PER_ITEM = {"si64": {"plane_vloc": 985477 + 974592, "plane_vloc32": 497264 + 488448}}
 FP64_PEAK_TFLOPS = 78.6
 
 WORKLOADS = {
@@ -131,7 +136,7 @@ def main():
     n1, n2, n3 = b.nr
     model = _byte_model(b, gs, npert, head)
     dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
-    roof = _roofline(dom, kernels[dom], model, args.steps)
+    roof = _roofline(dom, kernels[dom], model, args.steps, args.workload, _item_bytes(b))
     e2e = _end_to_end(b, nv, value, head)
 
     # ---- mixed precision beside the FP64 line: same batch, same tr2
@@ -154,7 +159,8 @@ def main():
                  "fp64_fallback_bands_per_step": mx["fallback_per_step"],
                  "cg_iterations_per_step": mx["iters_per_step"],
                  "kernels_ms_per_step": {k: round(v["ms_per_step"], 3) for k, v in mx["kernels"].items()},
-                 "roofline": _roofline(mdom, mx["kernels"][mdom], mmodel, args.steps),
+                 "roofline": _roofline(mdom, mx["kernels"][mdom], mmodel, args.steps, args.workload,
+                                       _item_bytes(b)),
                  "roofline_end_to_end": _end_to_end(b, nv, mx["value"], mx)}
         log(f"[bench] mixed: {mx['value']:.1f} pairs/s ({mixed['speedup_vs_fp64']}x), dng L2 rel err {l2:.2e}")
 
@@ -303,6 +309,12 @@ def _ncols(b):
     return len({(int(m[0]), int(m[1])) for m in b.mill})
 
 
+def _item_bytes(b):
+    """algorithmic bytes per transform item of the plane kernels (sticks read + written, V read)"""
+    st64 = b.nr[2] * (2 * _ncols(b) - 1) * 16
+    return {"plane_vloc": 2 * st64 + 8 * b.nnr, "plane_vloc32": st64 + 4 * b.nnr}
+
+
 def _byte_model(b, gs, npert, run):
     """Compulsory HBM bytes (or MFMA flops) per step of each kernel class.
 
@@ -342,8 +354,9 @@ def _byte_model(b, gs, npert, run):
     }
 
 
-def _roofline(name, k, model, steps):
-    """achieved = algorithmic bytes (flops) per launch / average launch time."""
+def _roofline(name, k, model, steps, workload=None, item_bytes=None):
+    """achieved = algorithmic bytes (flops) per launch / average launch time;
+    traffic = measured HBM bytes per item x items per launch, where measured."""
     if name not in model:
         return {"kernel": name, "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": None, "traffic": None}
@@ -357,8 +370,12 @@ def _roofline(name, k, model, steps):
                 "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFLOPS, 4), "traffic": None,
                 "algorithmic_flops_per_launch": per_launch, "avg_launch_ms": k["avg_ms"]}
     ach = per_launch / avg_s / 1e9
+    traffic = None
+    meas = TRAFFIC_PER_ITEM.get(workload, {}).get(name)
+    if meas and item_bytes and item_bytes.get(name):
+        traffic = int(meas * per_launch / item_bytes[name])
     return {"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "algorithmic_bytes_per_launch": int(per_launch),
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "algorithmic_bytes_per_launch": int(per_launch),
             "avg_launch_ms": k["avg_ms"]}
 
 

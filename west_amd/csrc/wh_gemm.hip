@@ -22,9 +22,36 @@
 // holds D[l/16 + 4r][l%16].
 #include "wh_gemm.h"
 
+#include <rocblas/rocblas.h>
+
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 
 namespace wh {
+
+// Backend of the FP64 products. The bra products (small M, long K, fused
+// G = 0 correction and row scale) use the split-K kernel of this file; the
+// plain products (dgemm) use rocBLAS, which measured about twice as fast as
+// the kernel below for the ket shape (M = 2 npw, K = nq) on MI355X.
+// WH_GEMM=own or WH_GEMM=rocblas puts both on one backend.
+enum class Backend { Default, Own, Rocblas };
+static Backend backend() {
+  static const Backend v = [] {
+    const char* e = std::getenv("WH_GEMM");
+    if (e && std::strcmp(e, "rocblas") == 0) return Backend::Rocblas;
+    if (e && std::strcmp(e, "own") == 0) return Backend::Own;
+    return Backend::Default;
+  }();
+  return v;
+}
+
+static rocblas_handle blas(hipStream_t s) {
+  static rocblas_handle h = nullptr;
+  if (!h) WH_REQUIRE(rocblas_create_handle(&h) == rocblas_status_success, "rocblas_create_handle failed");
+  WH_REQUIRE(rocblas_set_stream(h, s) == rocblas_status_success, "rocblas_set_stream failed");
+  return h;
+}
 
 namespace {
 
@@ -234,6 +261,13 @@ Scratch& partials() {
 void dgemm(bool transa, int M, int N, int K, double alpha, const double* A, long lda, const double* B, long ldb,
            double beta, double* C, long ldc, hipStream_t s) {
   if (M <= 0 || N <= 0) return;
+  if (backend() != Backend::Own && K > 0) {
+    WH_REQUIRE(rocblas_dgemm(blas(s), transa ? rocblas_operation_transpose : rocblas_operation_none,
+                             rocblas_operation_none, M, N, K, &alpha, A, lda, B, ldb, &beta, C, ldc) ==
+                   rocblas_status_success,
+               "rocblas_dgemm failed");
+    return;
+  }
   dim3 grid(ceil_div(M, BM), ceil_div(N, BN), 1);
   const int kc = std::max(K, 1);
   if (transa)
@@ -246,6 +280,14 @@ void dgemm(bool transa, int M, int N, int K, double alpha, const double* A, long
 void dgemm_bra(int M, int N, int K, double alpha, const double* A, long lda, const double* B, long ldb, double g0,
                const double* scale, double* C, long ldc, hipStream_t s) {
   if (M <= 0 || N <= 0) return;
+  if (backend() == Backend::Rocblas && K > 0) {
+    const double zero = 0.0;
+    WH_REQUIRE(rocblas_dgemm(blas(s), rocblas_operation_transpose, rocblas_operation_none, M, N, K, &alpha, A, lda, B,
+                             ldb, &zero, C, ldc) == rocblas_status_success,
+               "rocblas_dgemm failed");
+    g0_fix_scale(M, N, g0, A, lda, B, ldb, scale, C, ldc, s);
+    return;
+  }
   const int tiles = ceil_div(M, BM) * ceil_div(N, BN);
   // enough workgroups for 256 compute units, each K chunk at least 64 tiles long
   int split = std::max(1, std::min(16, ceil_div(512, tiles)));
