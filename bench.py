@@ -38,6 +38,8 @@ WORKLOADS = {
     # name: (builder kwargs, perturbations per batch)
     "si64": ({"nrep": 2, "a": 10.26, "ecutwfc": 25.0, "sigma": 0.05, "seed": 2024}, 64),
     "si8": ({"nrep": 1, "a": 10.26, "ecutwfc": 25.0, "sigma": 0.05, "seed": 2024}, 16),
+    # largest single-GPU workload: 3x3x3 cell, 216 atoms, 432 bands, 100^3, 128 perturbations per batch
+    "si216": ({"nrep": 3, "a": 10.26, "ecutwfc": 25.0, "sigma": 0.05, "seed": 2024}, 128),
 }
 
 

@@ -202,3 +202,21 @@ def test_davidson(gpu, key, request):
     assert np.max(np.abs(ev - ev_ref) / np.abs(ev_ref)) < 1e-8
     assert [r["dfpt_dim"] for r in log] == [r["dfpt_dim"] for r in log_ref]
     assert [r["diago_dim"] for r in log] == [r["diago_dim"] for r in log_ref]
+
+
+def test_dfpt_density_large_grid(gpu):
+    """100^3 grid (the 216-atom workload's size): the response density is
+    accumulated in two x halves because an FP64 100 x 100 accumulator does not
+    fit in LDS beside the plane."""
+    from oracle.dfpt import dfpt, sqvc_init
+    from west_amd.groundstate import make_ground_state
+    from west_amd.systems import make_h2_system
+    s = make_h2_system(nmol=2, L=30.78, ecutwfc=25.0, seed=8)
+    assert tuple(s.basis.nr) == (100, 100, 100)
+    gs = make_ground_state(s, gpu.gpu_h_factory, nbnd=s.nbnd_occ + 2, scf_steps=0, tol=1e-6)
+    gpu.set_ground_state(gs, sqvc=sqvc_init(gs.basis, gs.npwx))
+    print(f"case 2H2 L=30.78 nr={s.basis.nr} npw={s.basis.npw}", flush=True)
+    dvg = random_wfc(gs, 3, seed=61)
+    ref = dfpt(gs, dvg, -1.0)
+    out = gpu.dfpt(3, dvg, -1.0)
+    assert _rel(out, ref) < 1e-12

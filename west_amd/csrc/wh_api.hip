@@ -514,9 +514,34 @@ int linsolve_mixed(int m, z64* b, z64* x, const std::vector<double>& e_col, cons
 //   dvg: m perturbations (rows, stick order); dng: m rows out.
 //   out_weight: per-G factor applied to dng (sqvc for apply_operator) or null.
 // ---------------------------------------------------------------------------
+static int dfpt_chunk(int m, const z64* dvg, z64* dng, double tr2, const double* out_weight,
+                      std::vector<int>* ierr_pert);
+
+// Perturbations are processed in chunks whose Sternheimer work vectors fit
+// the vector budget (seven npw-vectors per (perturbation, band) pair).
 int dfpt_dev(int m, const z64* dvg, z64* dng, double tr2, const double* out_weight, std::vector<int>* ierr_pert) {
   Context& c = ctx();
   require_ready(true);
+  if (ierr_pert) ierr_pert->assign(std::max(m, 0), 0);
+  if (m <= 0) return 0;
+  const double pair_bytes = 7.0 * 16.0 * c.npw;
+  const long max_pairs = std::max<long>(1, static_cast<long>(static_cast<double>(c.vectors_budget) / pair_bytes));
+  const int chunk = static_cast<int>(std::max<long>(1, std::min<long>(m, max_pairs / std::max(c.nbndval, 1))));
+  int nonconv = 0;
+  const long ld = c.npw;
+  for (int p0 = 0; p0 < m; p0 += chunk) {
+    const int n = std::min(chunk, m - p0);
+    std::vector<int> ie;
+    nonconv += dfpt_chunk(n, dvg + p0 * ld, dng + p0 * ld, tr2, out_weight, ierr_pert ? &ie : nullptr);
+    if (ierr_pert)
+      for (int k = 0; k < n; ++k) (*ierr_pert)[p0 + k] = ie[k];
+  }
+  return nonconv;
+}
+
+static int dfpt_chunk(int m, const z64* dvg, z64* dng, double tr2, const double* out_weight,
+                      std::vector<int>* ierr_pert) {
+  Context& c = ctx();
   hipStream_t s = c.stream;
   const int npw = c.npw;
   const long ld = npw;
@@ -626,7 +651,9 @@ int dfpt_dev(int m, const z64* dvg, z64* dng, double tr2, const double* out_weig
       WH_HIP(hipMemcpyAsync(di, hi.data(), sizeof(FftItem) * np_ * nv, hipMemcpyHostToDevice, s));
       WH_HIP(hipMemcpyAsync(dz, hz.data(), sizeof(FftItem) * np_, hipMemcpyHostToDevice, s));
       WH_TIMED("zinv", launch_zinv(c.plan, di, np_ * nv, c.d_q, ld, dpsi, ld, sa, ZINV_PAIR, s));
-      WH_TIMED("plane_density", launch_plane_density(c.plan, np_, nv, sa, c.d_occ, sb, s));
+      const long dsc = plane_density_scratch(c.plan);
+      double* dscr = dsc > 0 ? c.w_dscr.as<double>(static_cast<size_t>(dsc) * np_) : nullptr;
+      WH_TIMED("plane_density", launch_plane_density(c.plan, np_, nv, sa, c.d_occ, sb, dscr, s));
       WH_TIMED("zfwd", launch_zfwd(c.plan, dz, np_, sb, nullptr, 0, dng, ld, za, s));
       WH_HIP(hipStreamSynchronize(s));
     }
@@ -694,7 +721,7 @@ int wh_finalize(void) {
     if (p) (void)hipFree(p);
   for (DevBuf* b : {&c.evc_sticks, &c.w_items, &c.w_items2, &c.w_items_id, &c.w_sticks_a, &c.w_sticks_b,
                     &c.w_cmat, &c.w_x, &c.w_t, &c.w_g, &c.w_h, &c.w_hold, &c.w_bc, &c.w_tc, &c.w_b, &c.w_dvr,
-                    &c.w_in, &c.w_out, &c.w_dvin, &c.w_dng, &c.w_l2i, &c.w_beta, &c.w_eu, &c.w_rho, &c.w_ecol, &c.w_epcol,
+                    &c.w_in, &c.w_out, &c.w_dvin, &c.w_dng, &c.w_l2i, &c.w_beta, &c.w_eu, &c.w_rho, &c.w_ecol, &c.w_epcol, &c.w_dscr,
                     &c.m_vrs32, &c.m_q32, &c.m_qp[0], &c.m_qp[1], &c.m_qp[2], &c.m_qtp[0], &c.m_qtp[1], &c.m_qtp[2],
                     &c.m_g, &c.m_h, &c.m_hold, &c.m_dx, &c.m_bc, &c.m_tc, &c.m_cm, &c.m_work, &c.m_u2i, &c.m_rho64,
                     &c.m_eu64, &c.m_fb, &c.m_fx, &c.m_fe, &c.m_fep})

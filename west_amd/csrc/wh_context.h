@@ -90,10 +90,11 @@ struct Context {
   int n_dfpt_maxiter = 250;
   int l_kinetic_only = 0;
   size_t sticks_budget = size_t(4) << 30;   // bytes for transform intermediates
+  size_t vectors_budget = size_t(64) << 30;  // bytes for the CG vectors of one dfpt chunk
   // workspaces
   DevBuf w_items, w_items2, w_items_id, w_sticks_a, w_sticks_b, w_cmat;
   DevBuf w_x, w_t, w_g, w_h, w_hold, w_bc, w_tc, w_b, w_dvr, w_in, w_out, w_dvin, w_dng;
-  DevBuf w_l2i, w_beta, w_eu, w_rho, w_ecol, w_epcol;
+  DevBuf w_l2i, w_beta, w_eu, w_rho, w_ecol, w_epcol, w_dscr;
   // mixed precision: FP32 operator with split-BF16 projectors, FP64 refinement
   int precision = 0;              // 0: FP64, 1: mixed (3-product split BF16), 2: mixed (6-product)
   double refine_delta = 1e-4;     // refine when rho < refine_delta * rho of the last refinement

@@ -138,8 +138,10 @@ void launch_plane_vloc(const StickPlan& p, const FftItem* d_items, int nitems, c
                        const double* vpot, long ldpot, double scale, hipStream_t s);
 void launch_plane_to_real(const StickPlan& p, const FftItem* d_items, int nitems, const z64* sin,
                           double* out, long ldout, hipStream_t s);
+// scratch reals per perturbation launch_plane_density needs for this grid (0: none)
+long plane_density_scratch(const StickPlan& p);
 void launch_plane_density(const StickPlan& p, int npert, int nband, const z64* sin, const double* occ,
-                          z64* sout, hipStream_t s);
+                          z64* sout, double* scratch, hipStream_t s);
 void launch_zfwd(const StickPlan& p, const FftItem* d_items, int nitems, const z64* sticks,
                  const z64* in, long ldin, z64* out, long ldout, const ZfwdArgs& args, hipStream_t s);
 

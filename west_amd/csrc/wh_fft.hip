@@ -683,12 +683,16 @@ void launch_plane_to_real(const StickPlan& p, const FftItem* d_items, int nitems
   WH_HIP(hipGetLastError());
 }
 
+long plane_density_scratch(const StickPlan& p) {
+  return (p.tpl && p.tpl->density_scratch > 0) ? p.tpl->density_scratch : 0;
+}
+
 void launch_plane_density(const StickPlan& p, int npert, int nband, const z64* sin, const double* occ,
-                          z64* sout, hipStream_t s) {
+                          z64* sout, double* scratch, hipStream_t s) {
   if (npert <= 0) return;
   const StickPlanDev& P = p.dev;
-  if (p.tpl && p.tpl->density_fits) {
-    p.tpl->density(P, npert, nband, sin, occ, sout, s);
+  if (p.tpl && p.tpl->density_scratch >= 0) {
+    p.tpl->density(P, npert, nband, sin, occ, sout, scratch, s);
     WH_HIP(hipGetLastError());
     return;
   }

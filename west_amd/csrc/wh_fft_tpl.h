@@ -13,8 +13,10 @@ struct TplKernels {
   void (*vloc)(const StickPlanDev&, const FftItem*, int, const void*, void*, const double*, long, double, int,
                hipStream_t);
   void (*to_real)(const StickPlanDev&, const FftItem*, int, const void*, double*, long, hipStream_t);
-  void (*density)(const StickPlanDev&, int, int, const void*, const double*, void*, hipStream_t);
-  bool density_fits;
+  // density(P, npert, nband, sticks in, occ, sticks out, scratch, stream): scratch holds
+  // density_scratch reals per perturbation (may be null when that is 0)
+  void (*density)(const StickPlanDev&, int, int, const void*, const double*, void*, void*, hipStream_t);
+  long density_scratch;   // 0: none needed; -1: this grid has no density kernel
   int max_occ;
 };
 

@@ -625,6 +625,99 @@ __global__ __launch_bounds__(NTP) void k_plane_density_t(StickPlanDev P, int nba
   for (int c = threadIdx.x; c < P.nct; c += NTP) dst[c] = S.pl[P.col_pos[c]];
 }
 
+// Response density for grids whose N x N FP64 accumulator does not fit in LDS
+// beside the plane (N = 100, 108 in FP64): the x range is covered in two
+// halves, each a pass over all bands with an N x N/2 accumulator; the first
+// half is parked in global scratch (N x N/2 reals per plane, written and read
+// back by the same workgroup) until the forward transforms.
+template <typename T, int N>
+constexpr int xh_chunk() {
+  const int budget = 160 * 1024 - 2048;
+  const int fixed = static_cast<int>(sizeof(cplx<T>)) * (N + max_rows<N>() * N) +
+                    static_cast<int>(sizeof(T)) * N * (N / 2);
+  int jc = (budget - fixed) / (static_cast<int>(sizeof(cplx<T>)) * N);
+  if (jc > N) jc = N;
+  if (jc < 1) return 0;
+  const int nchunk = (N + jc - 1) / jc;
+  return (N + nchunk - 1) / nchunk;
+}
+
+template <typename T, int N>
+constexpr size_t big_density_smem() {
+  return sizeof(cplx<T>) * (static_cast<size_t>(N) + static_cast<size_t>(max_rows<N>()) * N +
+                            static_cast<size_t>(xh_chunk<T, N>()) * N) +
+         sizeof(T) * static_cast<size_t>(N) * (N / 2);
+}
+
+// LDS of k_plane_density_t (whole N x N accumulator)
+template <typename T, int N>
+constexpr bool density_fits_lds() {
+  return sizeof(cplx<T>) * (static_cast<size_t>(N) + static_cast<size_t>(max_rows<N>()) * N +
+                            static_cast<size_t>(x_chunk<T, N>(true)) * N) +
+             sizeof(T) * static_cast<size_t>(N) * N <=
+         160u * 1024u;
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(NTP) void k_plane_density_big_t(StickPlanDev P, int nband, const cplx<T>* sin,
+                                                            const double* occ, cplx<T>* sout, T* scratch) {
+  constexpr int JC = xh_chunk<T, N>();
+  constexpr int MR = max_rows<N>();
+  constexpr int H = N / 2;
+  static_assert(JC >= 1 && N % 2 == 0, "big density kernel: even N with room for one x line");
+  PlaneLds<T, N> S = plane_lds<T, N>();
+  T* acc = reinterpret_cast<T*>(S.x + JC * N);
+  const int z = blockIdx.x;
+  const int p = blockIdx.y;
+  const long sl = static_cast<long>(N) * P.nct;
+  const int imax = P.imax;
+  load_tw<T, N>(S.tw, P.tw1);
+  T* park = scratch + (static_cast<long>(p) * N + z) * N * H;   // [j][x < H]
+  cplx<T>* pl = S.pl;
+  for (int h = 0; h < 2; ++h) {
+    for (int t = threadIdx.x; t < N * H; t += NTP) acc[t] = T(0);
+    for (int v = 0; v < nband; ++v) {
+      plane_fill<T, N>(P, S, sin + (static_cast<long>(p) * nband + v) * sl + static_cast<long>(z) * P.nct);
+      fft_inplace<T, N, MR, NTP>(P.nrows, S.pl, S.tw, +1);
+      const T o = T(occ[v]);
+      for (int j0 = 0; j0 < N; j0 += JC) {
+        const int nj = min(JC, N - j0);
+        auto from_rows = [=](int jj, int x) {
+          const int r = row_of(x, N, imax);
+          return r >= 0 ? pl[r * N + j0 + jj] : mk<T>(T(0), T(0));
+        };
+        // each (jj, x) is produced by exactly one thread: no race on acc
+        auto accumulate = [=](int jj, int x, cplx<T> w) {
+          const int xl = x - h * H;
+          if (xl >= 0 && xl < H) acc[(j0 + jj) * H + xl] += o * w.x * w.y;
+        };
+        passes<T, N, 0, JC, NTP>(nj, S.x, S.tw, +1, from_rows, accumulate);
+      }
+    }
+    if (h == 0) {
+      for (int t = threadIdx.x; t < N * H; t += NTP) park[t] = acc[t];
+      __threadfence_block();
+      __syncthreads();
+    }
+  }
+  // forward of the real density: a1 then a2, keeping |i| <= imax and the columns
+  for (int j0 = 0; j0 < N; j0 += JC) {
+    const int nj = min(JC, N - j0);
+    auto from_acc = [=](int jj, int x) {
+      const T v = x < H ? park[(j0 + jj) * H + x] : acc[(j0 + jj) * H + x - H];
+      return mk<T>(v, T(0));
+    };
+    auto to_rows = [=](int jj, int x, cplx<T> w) {
+      const int r = row_of(x, N, imax);
+      if (r >= 0) pl[r * N + j0 + jj] = w;
+    };
+    passes<T, N, 0, JC, NTP>(nj, S.x, S.tw, -1, from_acc, to_rows);
+  }
+  fft_inplace<T, N, MR, NTP>(P.nrows, S.pl, S.tw, -1);
+  cplx<T>* dst = sout + static_cast<long>(p) * sl + static_cast<long>(z) * P.nct;
+  for (int c = threadIdx.x; c < P.nct; c += NTP) dst[c] = S.pl[P.col_pos[c]];
+}
+
 // ---------------------------------------------------------------------------
 // dispatch
 // ---------------------------------------------------------------------------
@@ -696,11 +789,24 @@ struct Launch {
     k_plane_to_real_t<T, N><<<dim3(N, n), NTP, sm, s>>>(P, it, static_cast<const cplx<T>*>(si), out, ldout);
   }
   static void density(const StickPlanDev& P, int np, int nb, const void* si, const double* occ, void* so,
-                      hipStream_t s) {
-    const size_t sm = plane_smem<T, N>(true);
-    allow_smem(k_plane_density_t<T, N>, sm);
-    k_plane_density_t<T, N><<<dim3(N, np), NTP, sm, s>>>(P, nb, static_cast<const cplx<T>*>(si), occ,
-                                                        static_cast<cplx<T>*>(so));
+                      void* scratch, hipStream_t s) {
+    if constexpr (density_fits_lds<T, N>()) {
+      const size_t sm = plane_smem<T, N>(true);
+      allow_smem(k_plane_density_t<T, N>, sm);
+      k_plane_density_t<T, N><<<dim3(N, np), NTP, sm, s>>>(P, nb, static_cast<const cplx<T>*>(si), occ,
+                                                          static_cast<cplx<T>*>(so));
+      return;
+    }
+    if constexpr (!density_fits_lds<T, N>() && N % 2 == 0 && xh_chunk<T, N>() >= 1) {
+      WH_REQUIRE(scratch != nullptr, "response density: scratch buffer required for this grid");
+      constexpr size_t smb = big_density_smem<T, N>();
+      static_assert(smb <= 160u * 1024u, "big density kernel exceeds LDS");
+      allow_smem(k_plane_density_big_t<T, N>, smb);
+      k_plane_density_big_t<T, N><<<dim3(N, np), NTP, smb, s>>>(P, nb, static_cast<const cplx<T>*>(si), occ,
+                                                               static_cast<cplx<T>*>(so), static_cast<T*>(scratch));
+    } else {
+      WH_REQUIRE(false, "response-density plane does not fit in LDS for this grid");
+    }
   }
 };
 
@@ -723,12 +829,25 @@ struct Launch32 {
   }
 };
 
+// scratch reals per perturbation the density launcher needs (0: the plane fits in LDS;
+// -1: no kernel for this size)
+template <typename T, int N>
+long density_scratch() {
+  if constexpr (density_fits_lds<T, N>()) {
+    return 0;
+  } else if constexpr (N % 2 == 0 && xh_chunk<T, N>() >= 1) {
+    return static_cast<long>(N) * N * (N / 2);
+  } else {
+    return -1;
+  }
+}
+
 template <typename T>
 const TplKernels* table_for(int n) {
 #define WH_TPL_CASE(NN)                                                                                       \
   case NN: {                                                                                                  \
     static const TplKernels k{&Launch<T, NN>::zinv, &Launch<T, NN>::zfwd, &Launch<T, NN>::vloc,             \
-                              &Launch<T, NN>::to_real, &Launch<T, NN>::density, plane_smem<T, NN>(true) <= 160u * 1024u, \
+                              &Launch<T, NN>::to_real, &Launch<T, NN>::density, density_scratch<T, NN>(), \
                               max_occ<T, NN>()};                                                              \
     return &k;                                                                                                \
   }
