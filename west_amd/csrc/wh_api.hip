@@ -90,7 +90,12 @@ static bool on_device(const void* p) {
 // device view of an input array (copied if it is host memory)
 template <typename T>
 static const T* dev_in(const T* p, size_t count, DevBuf& tmp) {
-  if (on_device(p)) return p;
+  if (on_device(p)) {
+    // the caller may have produced the array on any stream (the library's own
+    // stream does not synchronise with the null stream): wait for the device
+    WH_HIP(hipDeviceSynchronize());
+    return p;
+  }
   T* d = tmp.as<T>(count);
   WH_HIP(hipMemcpyAsync(d, p, sizeof(T) * count, hipMemcpyHostToDevice, ctx().stream));
   return d;
@@ -425,9 +430,7 @@ int linsolve_mixed(int m, z64* b, z64* x, const std::vector<double>& e_col, cons
   std::vector<int> act;
   for (int ib = 0; ib < m; ++ib)
     if (!is_conv[ib]) act.push_back(ib);
-  int iter_done = 0;
   for (int iter = 1; iter <= c.n_dfpt_maxiter && !act.empty(); ++iter) {
-    iter_done = iter;
     precond(act, d_l2i);
     ++c.cg_iters;
     list.clear();
@@ -499,7 +502,8 @@ int linsolve_mixed(int m, z64* b, z64* x, const std::vector<double>& e_col, cons
     WH_HIP(hipStreamSynchronize(s));
     c.nfallback += nf;
     std::vector<int> ierr_f;
-    nonconv += linsolve_dev(nf, fb, fx, e_f, fe, fep, tr2, &ierr_f, std::max(1, c.n_dfpt_maxiter - iter_done));
+    // a full n_dfpt_maxiter budget: these are the hardest bands, restarted
+    nonconv += linsolve_dev(nf, fb, fx, e_f, fe, fep, tr2, &ierr_f, c.n_dfpt_maxiter);
     WH_HIP(hipMemcpyAsync(d_u2i, fl.data(), sizeof(int) * nf, hipMemcpyHostToDevice, s));
     copy_rows(nf, npw, d_u2i, fx, ld, x, ld, true, s);
     WH_HIP(hipStreamSynchronize(s));

@@ -77,8 +77,8 @@ static_assert(near(sin2pi(7, 12), -0.5L) && near(cos2pi(2, 5), -0.80901699437494
 // ---------------------------------------------------------------------------
 // small DFTs in registers; dir = -1: exp(-i), +1: exp(+i)
 // ---------------------------------------------------------------------------
-template <typename T, int R>
-__device__ __forceinline__ void dft(cplx<T>* v, int dir);
+template <typename T, int R, int DIR>
+__device__ __forceinline__ void dft(cplx<T>* v);
 
 template <typename T>
 __device__ __forceinline__ void dft2(cplx<T>* v) {
@@ -86,30 +86,30 @@ __device__ __forceinline__ void dft2(cplx<T>* v) {
   v[1] = v[0] - v[1];
   v[0] = a;
 }
-template <typename T>
-__device__ __forceinline__ void dft3(cplx<T>* v, int dir) {
+template <typename T, int DIR>
+__device__ __forceinline__ void dft3(cplx<T>* v) {
   const cplx<T> t1 = v[1] + v[2];
   const cplx<T> t2 = v[1] - v[2];
   const cplx<T> m = v[0] + cscale(t1, T(-0.5));
-  const cplx<T> u = mul_i(cscale(t2, T(0.86602540378443864676)), dir);
+  const cplx<T> u = mul_i(cscale(t2, T(0.86602540378443864676)), DIR);
   v[0] = v[0] + t1;
   v[1] = m + u;
   v[2] = m - u;
 }
-template <typename T>
-__device__ __forceinline__ void dft4(cplx<T>* v, int dir) {
+template <typename T, int DIR>
+__device__ __forceinline__ void dft4(cplx<T>* v) {
   const cplx<T> t0 = v[0] + v[2];
   const cplx<T> t1 = v[0] - v[2];
   const cplx<T> t2 = v[1] + v[3];
   const cplx<T> t3 = v[1] - v[3];
-  const cplx<T> u = mul_i(t3, dir);
+  const cplx<T> u = mul_i(t3, DIR);
   v[0] = t0 + t2;
   v[2] = t0 - t2;
   v[1] = t1 + u;
   v[3] = t1 - u;
 }
-template <typename T>
-__device__ __forceinline__ void dft5(cplx<T>* v, int dir) {
+template <typename T, int DIR>
+__device__ __forceinline__ void dft5(cplx<T>* v) {
   constexpr T c1 = T(0.30901699437494742410), c2 = T(-0.80901699437494742410);
   constexpr T s1 = T(0.95105651629515357212), s2 = T(0.58778525229247312917);
   const cplx<T> t1 = v[1] + v[4];
@@ -118,8 +118,8 @@ __device__ __forceinline__ void dft5(cplx<T>* v, int dir) {
   const cplx<T> t4 = v[2] - v[3];
   const cplx<T> m1 = v[0] + cscale(t1, c1) + cscale(t2, c2);
   const cplx<T> m2 = v[0] + cscale(t1, c2) + cscale(t2, c1);
-  const cplx<T> u1 = mul_i(cscale(t3, s1) + cscale(t4, s2), dir);
-  const cplx<T> u2 = mul_i(cscale(t3, s2) - cscale(t4, s1), dir);
+  const cplx<T> u1 = mul_i(cscale(t3, s1) + cscale(t4, s2), DIR);
+  const cplx<T> u2 = mul_i(cscale(t3, s2) - cscale(t4, s1), DIR);
   v[0] = v[0] + t1 + t2;
   v[1] = m1 + u1;
   v[4] = m1 - u1;
@@ -128,8 +128,20 @@ __device__ __forceinline__ void dft5(cplx<T>* v, int dir) {
 }
 
 // two-level Cooley-Tukey: R = A*B, j = j1 + A j2, k = k2 + B k1
-template <typename T, int A, int B>
-__device__ __forceinline__ void dft_ct(cplx<T>* v, int dir) {
+// y * (c + i s) for constant c, s (folded at compile time after unrolling):
+// quarter turns are swaps, eighth turns two adds and two multiplies
+template <typename T>
+__device__ __forceinline__ cplx<T> twiddle_mul(cplx<T> y, T c, T s) {
+  const T tol = T(1e-6);
+  if (fabs(s) < tol) return c > T(0) ? y : mk<T>(-y.x, -y.y);
+  if (fabs(c) < tol) return s > T(0) ? mk<T>(-y.y, y.x) : mk<T>(y.y, -y.x);
+  if (fabs(c - s) < tol) return mk<T>(c * (y.x - y.y), c * (y.x + y.y));
+  if (fabs(c + s) < tol) return mk<T>(c * (y.x + y.y), c * (y.y - y.x));
+  return cmul(y, mk<T>(c, s));
+}
+
+template <typename T, int A, int B, int DIR>
+__device__ __forceinline__ void dft_ct(cplx<T>* v) {
   constexpr int R = A * B;
   cplx<T> y[A][B];
 #pragma unroll
@@ -137,40 +149,40 @@ __device__ __forceinline__ void dft_ct(cplx<T>* v, int dir) {
 #pragma unroll
     for (int j2 = 0; j2 < B; ++j2) y[j1][j2] = v[j1 + A * j2];
 #pragma unroll
-  for (int j1 = 0; j1 < A; ++j1) dft<T, B>(y[j1], dir);
+  for (int j1 = 0; j1 < A; ++j1) dft<T, B, DIR>(y[j1]);
 #pragma unroll
   for (int j1 = 1; j1 < A; ++j1)
 #pragma unroll
     for (int k2 = 1; k2 < B; ++k2) {
       const T c = T(cos2pi(j1 * k2, R));
       const T s = T(sin2pi(j1 * k2, R));
-      // exp(dir * 2 pi i j1 k2 / R)
-      y[j1][k2] = cmul(y[j1][k2], mk<T>(c, dir > 0 ? s : -s));
+      // exp(DIR * 2 pi i j1 k2 / R)
+      y[j1][k2] = twiddle_mul(y[j1][k2], c, DIR > 0 ? s : -s);
     }
 #pragma unroll
   for (int k2 = 0; k2 < B; ++k2) {
     cplx<T> col[A];
 #pragma unroll
     for (int j1 = 0; j1 < A; ++j1) col[j1] = y[j1][k2];
-    dft<T, A>(col, dir);
+    dft<T, A, DIR>(col);
 #pragma unroll
     for (int k1 = 0; k1 < A; ++k1) v[k2 + B * k1] = col[k1];
   }
 }
 
-template <typename T, int R>
-__device__ __forceinline__ void dft(cplx<T>* v, int dir) {
+template <typename T, int R, int DIR>
+__device__ __forceinline__ void dft(cplx<T>* v) {
   if constexpr (R == 2) dft2(v);
-  else if constexpr (R == 3) dft3(v, dir);
-  else if constexpr (R == 4) dft4(v, dir);
-  else if constexpr (R == 5) dft5(v, dir);
-  else if constexpr (R == 6) dft_ct<T, 2, 3>(v, dir);
-  else if constexpr (R == 8) dft_ct<T, 2, 4>(v, dir);
-  else if constexpr (R == 9) dft_ct<T, 3, 3>(v, dir);
-  else if constexpr (R == 10) dft_ct<T, 2, 5>(v, dir);
+  else if constexpr (R == 3) dft3<T, DIR>(v);
+  else if constexpr (R == 4) dft4<T, DIR>(v);
+  else if constexpr (R == 5) dft5<T, DIR>(v);
+  else if constexpr (R == 6) dft_ct<T, 2, 3, DIR>(v);
+  else if constexpr (R == 8) dft_ct<T, 2, 4, DIR>(v);
+  else if constexpr (R == 9) dft_ct<T, 3, 3, DIR>(v);
+  else if constexpr (R == 10) dft_ct<T, 2, 5, DIR>(v);
   else {
     static_assert(R == 12, "unsupported radix");
-    dft_ct<T, 4, 3>(v, dir);
+    dft_ct<T, 4, 3, DIR>(v);
   }
 }
 
@@ -218,8 +230,8 @@ struct Plan {
 // stores through st(line, idx, value); a barrier separates the two phases and
 // follows the stores.  MAXL bounds nlines (register staging).
 // ---------------------------------------------------------------------------
-template <typename T, int N, int S, int MAXL, int BT, class Ld, class St>
-__device__ __forceinline__ void pass(int nlines, const cplx<T>* tw, int dir, Ld ld, St st) {
+template <typename T, int N, int S, int MAXL, int BT, int DIR, class Ld, class St>
+__device__ __forceinline__ void pass(int nlines, const cplx<T>* tw, Ld ld, St st) {
   constexpr int R = Plan<N>::radix(S);
   constexpr int P = Plan<N>::stride(S);
   constexpr int NBL = N / R;
@@ -251,7 +263,7 @@ __device__ __forceinline__ void pass(int nlines, const cplx<T>* tw, int dir, Ld 
           // FP32: w^j from one LDS read of w = root^(k STEP), w^j = w^(j/2) w^(j - j/2)
           // (at most log2(R) roundings deep; saves R - 2 LDS reads per butterfly)
           w[1] = tw[k * STEP];
-          if (dir > 0) w[1].y = -w[1].y;
+          if (DIR > 0) w[1].y = -w[1].y;
 #pragma unroll
           for (int j = 2; j < R; ++j) w[j] = cmul(w[j / 2], w[j - j / 2]);
         } else {
@@ -259,13 +271,13 @@ __device__ __forceinline__ void pass(int nlines, const cplx<T>* tw, int dir, Ld 
 #pragma unroll
           for (int j = 1; j < R; ++j) {
             w[j] = tw[j * k * STEP];
-            if (dir > 0) w[j].y = -w[j].y;
+            if (DIR > 0) w[j].y = -w[j].y;
           }
         }
 #pragma unroll
         for (int j = 1; j < R; ++j) v[t][j] = cmul(v[t][j], w[j]);
       }
-      dft<T, R>(v[t], dir);
+      dft<T, R, DIR>(v[t]);
       const int base = (i - k) * R + k;
 #pragma unroll
       for (int j = 0; j < R; ++j) st(line, base + j * P, v[t][j]);
@@ -276,29 +288,29 @@ __device__ __forceinline__ void pass(int nlines, const cplx<T>* tw, int dir, Ld 
 
 // full transform: pass 0 loads through first(), the last pass stores through
 // last(), the passes in between work in place on buf (line stride N)
-template <typename T, int N, int S, int MAXL, int BT, class First, class Last>
-__device__ __forceinline__ void passes(int nlines, cplx<T>* buf, const cplx<T>* tw, int dir, First first, Last last) {
+template <typename T, int N, int S, int MAXL, int BT, int DIR, class First, class Last>
+__device__ __forceinline__ void passes(int nlines, cplx<T>* buf, const cplx<T>* tw, First first, Last last) {
   constexpr int NS = Plan<N>::nst;
   auto in_buf = [buf](int line, int idx) { return buf[line * N + idx]; };
   auto to_buf = [buf](int line, int idx, cplx<T> v) { buf[line * N + idx] = v; };
   if constexpr (NS == 1) {
-    pass<T, N, 0, MAXL, BT>(nlines, tw, dir, first, last);
+    pass<T, N, 0, MAXL, BT, DIR>(nlines, tw, first, last);
   } else if constexpr (S == 0) {
-    pass<T, N, 0, MAXL, BT>(nlines, tw, dir, first, to_buf);
-    passes<T, N, 1, MAXL, BT>(nlines, buf, tw, dir, first, last);
+    pass<T, N, 0, MAXL, BT, DIR>(nlines, tw, first, to_buf);
+    passes<T, N, 1, MAXL, BT, DIR>(nlines, buf, tw, first, last);
   } else if constexpr (S == NS - 1) {
-    pass<T, N, S, MAXL, BT>(nlines, tw, dir, in_buf, last);
+    pass<T, N, S, MAXL, BT, DIR>(nlines, tw, in_buf, last);
   } else {
-    pass<T, N, S, MAXL, BT>(nlines, tw, dir, in_buf, to_buf);
-    passes<T, N, S + 1, MAXL, BT>(nlines, buf, tw, dir, first, last);
+    pass<T, N, S, MAXL, BT, DIR>(nlines, tw, in_buf, to_buf);
+    passes<T, N, S + 1, MAXL, BT, DIR>(nlines, buf, tw, first, last);
   }
 }
 
-template <typename T, int N, int MAXL, int BT>
-__device__ __forceinline__ void fft_inplace(int nlines, cplx<T>* buf, const cplx<T>* tw, int dir) {
+template <typename T, int N, int MAXL, int BT, int DIR>
+__device__ __forceinline__ void fft_inplace(int nlines, cplx<T>* buf, const cplx<T>* tw) {
   auto in_buf = [buf](int line, int idx) { return buf[line * N + idx]; };
   auto to_buf = [buf](int line, int idx, cplx<T> v) { buf[line * N + idx] = v; };
-  passes<T, N, 0, MAXL, BT>(nlines, buf, tw, dir, in_buf, to_buf);
+  passes<T, N, 0, MAXL, BT, DIR>(nlines, buf, tw, in_buf, to_buf);
 }
 
 template <typename T, int N>
@@ -381,7 +393,7 @@ __global__ __launch_bounds__(NT) void k_zinv_t(StickPlanDev P, const FftItem* it
     lines[line * N + zm] = mk<T>(T(v.x), T(v.y));
   }
   __syncthreads();
-  fft_inplace<T, N, 2 * CPW, NT>(2 * ncl, lines, tw, +1);
+  fft_inplace<T, N, 2 * CPW, NT, +1>(2 * ncl, lines, tw);
   cplx<T>* out = sticks + static_cast<long>(it) * N * P.nct;
   for (int t = threadIdx.x; t < N * 2 * ncl; t += NT) {
     const int z = t / (2 * ncl);
@@ -427,7 +439,7 @@ __global__ __launch_bounds__(NT) void k_zfwd_t(StickPlanDev P, const FftItem* it
     lines[(single ? lc : 2 * lc + (own ? 0 : 1)) * N + z] = v;
   }
   __syncthreads();
-  fft_inplace<T, N, 2 * CPW, NT>(nl, lines, tw, -1);
+  fft_inplace<T, N, 2 * CPW, NT, -1>(nl, lines, tw);
   const int s0 = P.col_off[c0];
   const int s1 = P.col_off[c1];
   cplx<V>* oa = out + static_cast<long>(item.a) * ldout;
@@ -521,7 +533,7 @@ __global__ __launch_bounds__(NTP) __attribute__((amdgpu_waves_per_eu(2 * OCC))) 
   const int imax = P.imax;
   load_tw<T, N>(S.tw, P.tw1);
   plane_fill<T, N>(P, S, sin + static_cast<long>(item.a) * sl + static_cast<long>(z) * P.nct);
-  fft_inplace<T, N, MR, NTP>(P.nrows, S.pl, S.tw, +1);
+  fft_inplace<T, N, MR, NTP, +1>(P.nrows, S.pl, S.tw);
   const PT* V = vpot + (item.pot >= 0 ? static_cast<long>(item.pot) * ldpot : 0L) + static_cast<long>(z) * N * N;
   const T sc = T(scale);
   cplx<T>* pl = S.pl;
@@ -536,16 +548,16 @@ __global__ __launch_bounds__(NTP) __attribute__((amdgpu_waves_per_eu(2 * OCC))) 
     auto times_v = [=](int jj, int x, cplx<T> v) {
       xb[jj * N + x] = cscale(v, T(V[(j0 + jj) * N + x]) * sc);
     };
-    passes<T, N, 0, JC, NTP>(nj, xb, S.tw, +1, from_rows, times_v);
+    passes<T, N, 0, JC, NTP, +1>(nj, xb, S.tw, from_rows, times_v);
     // forward along a1: only |i| <= imax frequencies go back to the rows
     auto from_x = [=](int jj, int x) { return xb[jj * N + x]; };
     auto to_rows = [=](int jj, int x, cplx<T> v) {
       const int r = row_of(x, N, imax);
       if (r >= 0) pl[r * N + j0 + jj] = v;
     };
-    passes<T, N, 0, JC, NTP>(nj, xb, S.tw, -1, from_x, to_rows);
+    passes<T, N, 0, JC, NTP, -1>(nj, xb, S.tw, from_x, to_rows);
   }
-  fft_inplace<T, N, MR, NTP>(P.nrows, S.pl, S.tw, -1);
+  fft_inplace<T, N, MR, NTP, -1>(P.nrows, S.pl, S.tw);
   cplx<T>* dst = sout + static_cast<long>(it) * sl + static_cast<long>(z) * P.nct;
   for (int c = threadIdx.x; c < P.nct; c += NTP) dst[c] = S.pl[P.col_pos[c]];
 }
@@ -563,7 +575,7 @@ __global__ __launch_bounds__(NTP) void k_plane_to_real_t(StickPlanDev P, const F
   const int imax = P.imax;
   load_tw<T, N>(S.tw, P.tw1);
   plane_fill<T, N>(P, S, sin + static_cast<long>(it) * sl + static_cast<long>(z) * P.nct);
-  fft_inplace<T, N, MR, NTP>(P.nrows, S.pl, S.tw, +1);
+  fft_inplace<T, N, MR, NTP, +1>(P.nrows, S.pl, S.tw);
   double* oa = out + static_cast<long>(item.a) * ldout + static_cast<long>(z) * N * N;
   double* ob = item.b >= 0 ? out + static_cast<long>(item.b) * ldout + static_cast<long>(z) * N * N : nullptr;
   cplx<T>* pl = S.pl;
@@ -577,7 +589,7 @@ __global__ __launch_bounds__(NTP) void k_plane_to_real_t(StickPlanDev P, const F
       oa[(j0 + jj) * N + x] = static_cast<double>(v.x);
       if (ob) ob[(j0 + jj) * N + x] = static_cast<double>(v.y);
     };
-    passes<T, N, 0, JC, NTP>(nj, S.x, S.tw, +1, from_rows, to_out);
+    passes<T, N, 0, JC, NTP, +1>(nj, S.x, S.tw, from_rows, to_out);
   }
 }
 
@@ -597,7 +609,7 @@ __global__ __launch_bounds__(NTP) void k_plane_density_t(StickPlanDev P, int nba
   cplx<T>* pl = S.pl;
   for (int v = 0; v < nband; ++v) {
     plane_fill<T, N>(P, S, sin + (static_cast<long>(p) * nband + v) * sl + static_cast<long>(z) * P.nct);
-    fft_inplace<T, N, MR, NTP>(P.nrows, S.pl, S.tw, +1);
+    fft_inplace<T, N, MR, NTP, +1>(P.nrows, S.pl, S.tw);
     const T o = T(occ[v]);
     for (int j0 = 0; j0 < N; j0 += JC) {
       const int nj = min(JC, N - j0);
@@ -607,7 +619,7 @@ __global__ __launch_bounds__(NTP) void k_plane_density_t(StickPlanDev P, int nba
       };
       // each (jj, x) is produced by exactly one thread: no race on acc
       auto accumulate = [=](int jj, int x, cplx<T> w) { acc[(j0 + jj) * N + x] += o * w.x * w.y; };
-      passes<T, N, 0, JC, NTP>(nj, S.x, S.tw, +1, from_rows, accumulate);
+      passes<T, N, 0, JC, NTP, +1>(nj, S.x, S.tw, from_rows, accumulate);
     }
   }
   // forward of the real density: a1 then a2, keeping |i| <= imax and the columns
@@ -618,9 +630,9 @@ __global__ __launch_bounds__(NTP) void k_plane_density_t(StickPlanDev P, int nba
       const int r = row_of(x, N, imax);
       if (r >= 0) pl[r * N + j0 + jj] = w;
     };
-    passes<T, N, 0, JC, NTP>(nj, S.x, S.tw, -1, from_acc, to_rows);
+    passes<T, N, 0, JC, NTP, -1>(nj, S.x, S.tw, from_acc, to_rows);
   }
-  fft_inplace<T, N, MR, NTP>(P.nrows, S.pl, S.tw, -1);
+  fft_inplace<T, N, MR, NTP, -1>(P.nrows, S.pl, S.tw);
   cplx<T>* dst = sout + static_cast<long>(p) * sl + static_cast<long>(z) * P.nct;
   for (int c = threadIdx.x; c < P.nct; c += NTP) dst[c] = S.pl[P.col_pos[c]];
 }
@@ -678,7 +690,7 @@ __global__ __launch_bounds__(NTP) void k_plane_density_big_t(StickPlanDev P, int
     for (int t = threadIdx.x; t < N * H; t += NTP) acc[t] = T(0);
     for (int v = 0; v < nband; ++v) {
       plane_fill<T, N>(P, S, sin + (static_cast<long>(p) * nband + v) * sl + static_cast<long>(z) * P.nct);
-      fft_inplace<T, N, MR, NTP>(P.nrows, S.pl, S.tw, +1);
+      fft_inplace<T, N, MR, NTP, +1>(P.nrows, S.pl, S.tw);
       const T o = T(occ[v]);
       for (int j0 = 0; j0 < N; j0 += JC) {
         const int nj = min(JC, N - j0);
@@ -691,7 +703,7 @@ __global__ __launch_bounds__(NTP) void k_plane_density_big_t(StickPlanDev P, int
           const int xl = x - h * H;
           if (xl >= 0 && xl < H) acc[(j0 + jj) * H + xl] += o * w.x * w.y;
         };
-        passes<T, N, 0, JC, NTP>(nj, S.x, S.tw, +1, from_rows, accumulate);
+        passes<T, N, 0, JC, NTP, +1>(nj, S.x, S.tw, from_rows, accumulate);
       }
     }
     if (h == 0) {
@@ -711,9 +723,9 @@ __global__ __launch_bounds__(NTP) void k_plane_density_big_t(StickPlanDev P, int
       const int r = row_of(x, N, imax);
       if (r >= 0) pl[r * N + j0 + jj] = w;
     };
-    passes<T, N, 0, JC, NTP>(nj, S.x, S.tw, -1, from_acc, to_rows);
+    passes<T, N, 0, JC, NTP, -1>(nj, S.x, S.tw, from_acc, to_rows);
   }
-  fft_inplace<T, N, MR, NTP>(P.nrows, S.pl, S.tw, -1);
+  fft_inplace<T, N, MR, NTP, -1>(P.nrows, S.pl, S.tw);
   cplx<T>* dst = sout + static_cast<long>(p) * sl + static_cast<long>(z) * P.nct;
   for (int c = threadIdx.x; c < P.nct; c += NTP) dst[c] = S.pl[P.col_pos[c]];
 }

@@ -26,15 +26,19 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <map>
 #include <cstring>
 
 namespace wh {
 
-// Backend of the FP64 products. The bra products (small M, long K, fused
-// G = 0 correction and row scale) use the split-K kernel of this file; the
-// plain products (dgemm) use rocBLAS, which measured about twice as fast as
-// the kernel below for the ket shape (M = 2 npw, K = nq) on MI355X.
-// WH_GEMM=own or WH_GEMM=rocblas puts both on one backend.
+// Backend of the FP64 products. Plain products (dgemm) use rocBLAS, measured
+// about twice as fast as the kernel below for the ket shape (M = 2 npw,
+// K = nq) on MI355X. The bra products (small M = nq, long K = 2 npw, fused
+// G = 0 correction and row scale) use whichever of the split-K kernel of this
+// file and rocBLAS + k_g0_scale was faster when first timed for that (M, K)
+// on a call with N >= 512 columns (the kernel won at nq = 384, rocBLAS at
+// nq = 1296); smaller calls use the kernel until then.
+// WH_GEMM=own or WH_GEMM=rocblas puts every product on one backend.
 enum class Backend { Default, Own, Rocblas };
 static Backend backend() {
   static const Backend v = [] {
@@ -277,17 +281,8 @@ void dgemm(bool transa, int M, int N, int K, double alpha, const double* A, long
   WH_HIP(hipGetLastError());
 }
 
-void dgemm_bra(int M, int N, int K, double alpha, const double* A, long lda, const double* B, long ldb, double g0,
-               const double* scale, double* C, long ldc, hipStream_t s) {
-  if (M <= 0 || N <= 0) return;
-  if (backend() == Backend::Rocblas && K > 0) {
-    const double zero = 0.0;
-    WH_REQUIRE(rocblas_dgemm(blas(s), rocblas_operation_transpose, rocblas_operation_none, M, N, K, &alpha, A, lda, B,
-                             ldb, &zero, C, ldc) == rocblas_status_success,
-               "rocblas_dgemm failed");
-    g0_fix_scale(M, N, g0, A, lda, B, ldb, scale, C, ldc, s);
-    return;
-  }
+static void dgemm_bra_own(int M, int N, int K, double alpha, const double* A, long lda, const double* B, long ldb,
+                          double g0, const double* scale, double* C, long ldc, hipStream_t s) {
   const int tiles = ceil_div(M, BM) * ceil_div(N, BN);
   // enough workgroups for 256 compute units, each K chunk at least 64 tiles long
   int split = std::max(1, std::min(16, ceil_div(512, tiles)));
@@ -297,15 +292,55 @@ void dgemm_bra(int M, int N, int K, double alpha, const double* A, long lda, con
   const long pstride = static_cast<long>(ldc) * N;
   double* P = partials().get(static_cast<size_t>(split) * pstride);
   dim3 grid(ceil_div(M, BM), ceil_div(N, BN), split);
-  if (split > 1) {
-    k_dgemm<true><<<grid, NT, 0, s>>>(M, N, K, kchunk, 1.0, A, lda, B, ldb, 0.0, P, ldc, pstride);
-  } else {
-    k_dgemm<true><<<grid, NT, 0, s>>>(M, N, K, kchunk, 1.0, A, lda, B, ldb, 0.0, P, ldc, pstride);
-  }
+  k_dgemm<true><<<grid, NT, 0, s>>>(M, N, K, kchunk, 1.0, A, lda, B, ldb, 0.0, P, ldc, pstride);
   WH_HIP(hipGetLastError());
   const long n = static_cast<long>(M) * N;
   k_reduce_g0<<<ceil_div(n, 256), 256, 0, s>>>(M, N, split, P, pstride, alpha, g0, A, lda, B, ldb, scale, C, ldc);
   WH_HIP(hipGetLastError());
+}
+
+static void dgemm_bra_lib(int M, int N, int K, double alpha, const double* A, long lda, const double* B, long ldb,
+                          double g0, const double* scale, double* C, long ldc, hipStream_t s) {
+  const double zero = 0.0;
+  WH_REQUIRE(rocblas_dgemm(blas(s), rocblas_operation_transpose, rocblas_operation_none, M, N, K, &alpha, A, lda, B,
+                           ldb, &zero, C, ldc) == rocblas_status_success,
+             "rocblas_dgemm failed");
+  g0_fix_scale(M, N, g0, A, lda, B, ldb, scale, C, ldc, s);
+}
+
+void dgemm_bra(int M, int N, int K, double alpha, const double* A, long lda, const double* B, long ldb, double g0,
+               const double* scale, double* C, long ldc, hipStream_t s) {
+  if (M <= 0 || N <= 0) return;
+  const Backend be = backend();
+  if (be == Backend::Rocblas && K > 0) return dgemm_bra_lib(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s);
+  if (be == Backend::Own || K <= 0) return dgemm_bra_own(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s);
+  static std::map<std::pair<int, int>, int> chosen;   // (M, K) -> 0 kernel, 1 rocBLAS
+  const auto key = std::make_pair(M, K);
+  auto it = chosen.find(key);
+  if (it == chosen.end() && N >= 512) {
+    // time both on this call's arguments
+    hipEvent_t a, b;
+    WH_HIP(hipEventCreate(&a));
+    WH_HIP(hipEventCreate(&b));
+    float t[2] = {0.f, 0.f};
+    for (int rep = 0; rep < 2; ++rep)
+      for (int v = 0; v < 2; ++v) {
+        WH_HIP(hipEventRecord(a, s));
+        if (v == 0) dgemm_bra_own(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s);
+        else dgemm_bra_lib(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s);
+        WH_HIP(hipEventRecord(b, s));
+        WH_HIP(hipEventSynchronize(b));
+        float ms = 0.f;
+        WH_HIP(hipEventElapsedTime(&ms, a, b));
+        if (rep == 1) t[v] = ms;   // the first round warms both up
+      }
+    WH_HIP(hipEventDestroy(a));
+    WH_HIP(hipEventDestroy(b));
+    it = chosen.emplace(key, t[1] < t[0] ? 1 : 0).first;
+    // recompute with the chosen backend so the result does not depend on the order above
+  }
+  if (it != chosen.end() && it->second == 1) return dgemm_bra_lib(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s);
+  dgemm_bra_own(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s);
 }
 
 void g0_fix_scale(int M, int N, double g0, const double* A, long lda, const double* B, long ldb,

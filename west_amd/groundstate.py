@@ -261,3 +261,143 @@ def _psi_r(basis: GammaBasis, c: np.ndarray) -> np.ndarray:
     grid[basis.nl[:n] - 1] = c[:n]
     grid[basis.nlm[:n] - 1] = np.conj(c[:n])
     return grid_to_real(basis, grid)
+
+
+# ----------------------------------------------------------------------------
+# the same eigensolver on the device (large systems: the dense algebra of the
+# block Davidson on (nbnd x 2 npw) blocks moves to the GPU, FP64)
+# ----------------------------------------------------------------------------
+
+def _t_orthonormalize(X, cutoff: float = 1e-14):
+    import torch
+    if X.shape[0] == 0:
+        return X
+    n = torch.linalg.norm(X, dim=1)
+    X = X[n > 0] / n[n > 0, None]
+    for _ in range(2):
+        S = X @ X.T
+        w, V = torch.linalg.eigh(0.5 * (S + S.T))
+        keep = w > w.max() * cutoff
+        X = (V[:, keep] / torch.sqrt(w[keep])).T @ X
+    return X
+
+
+def _t_expand(W, V):
+    import torch
+    n = torch.linalg.norm(W, dim=1)
+    W = W[n > 0] / n[n > 0, None]
+    for _ in range(2):
+        W = W - (W @ V.T) @ V
+    n = torch.linalg.norm(W, dim=1)
+    W = W[n > 1e-8] / n[n > 1e-8, None]
+    W = _t_orthonormalize(W, cutoff=1e-10)
+    W = W - (W @ V.T) @ V
+    return W / torch.linalg.norm(W, dim=1)[:, None] if W.shape[0] else W
+
+
+def solve_bands_device(basis: GammaBasis, apply_h_dev, nbnd: int, npwx: int | None = None, tol: float = 1e-9,
+                       seed: int = 7, maxiter: int = 400, max_blocks: int = 3, verbose: bool = False,
+                       device: str = "cuda:0"):
+    """:func:`solve_bands` with the block Davidson on the device.
+
+    ``apply_h_dev`` maps a device complex128 tensor (m, npwx) to H psi on the
+    device.  Returns (et, evc) as numpy arrays."""
+    import torch
+    npw = basis.npw
+    npwx = npwx or npw
+    gstart = basis.gstart
+    dim = 2 * npw - 1 if gstart == 2 else 2 * npw
+    s2 = np.sqrt(2.0)
+    k = npw - 1
+
+    def to_real_t(c):
+        c = c[:, :npw]
+        if gstart == 2:
+            return torch.cat([c[:, :1].real, s2 * c[:, 1:].real, s2 * c[:, 1:].imag], dim=1)
+        return torch.cat([s2 * c.real, s2 * c.imag], dim=1)
+
+    def from_real_t(u):
+        out = torch.zeros((u.shape[0], npwx), dtype=torch.complex128, device=u.device)
+        if gstart == 2:
+            out[:, 0] = u[:, 0].to(torch.complex128)
+            out[:, 1:npw] = (u[:, 1:1 + k] + 1j * u[:, 1 + k:1 + 2 * k]) / s2
+        else:
+            out[:, :npw] = (u[:, :npw] + 1j * u[:, npw:2 * npw]) / s2
+        return out
+
+    def apply_real(U):
+        out = torch.empty_like(U)
+        bs = 512
+        for s in range(0, U.shape[0], bs):
+            c = from_real_t(U[s:s + bs]).contiguous()
+            if c.is_cuda:
+                torch.cuda.synchronize()
+            out[s:s + bs] = to_real_t(apply_h_dev(c))
+        return out
+
+    g2 = torch.from_numpy(np.asarray(basis.g2kin, dtype=np.float64)).to(device)
+    g2r = torch.cat([g2[:1], g2[1:], g2[1:]]) if gstart == 2 else torch.cat([g2, g2])
+
+    def precond(R, lam):
+        ek = torch.clamp(torch.abs(lam), min=1.0)[:, None]
+        return R / (1.0 + g2r[None, :] / ek)
+
+    nb = nbnd + max(4, nbnd // 8)
+    rng = np.random.default_rng(seed)
+    X0 = torch.from_numpy(rng.normal(size=(nb, dim))).to(device) / (1.0 + g2r[None, :])
+    V = _t_orthonormalize(X0)
+    HV = apply_real(V)
+    maxdim = max_blocks * nb
+    for it in range(maxiter):
+        T = V @ HV.T
+        w, C = torch.linalg.eigh(0.5 * (T + T.T))
+        X = C[:, :nb].T @ V
+        HX = C[:, :nb].T @ HV
+        lam = w[:nb]
+        R = HX - lam[:, None] * X
+        rn = torch.linalg.norm(R, dim=1)
+        rmax = float(rn[:nbnd].max())
+        if verbose and it % 5 == 0:
+            print(f"  davidson (device) it {it}: dim {V.shape[0]} max res {rmax:.3e}", flush=True)
+        if rmax < tol:
+            break
+        active = rn > tol * 0.1
+        W = precond(R[active], lam[active])
+        if V.shape[0] + W.shape[0] > maxdim:
+            V, HV = X, HX
+        W = _t_expand(W, V)
+        if W.shape[0] == 0:
+            break
+        V = torch.cat([V, W])
+        HV = torch.cat([HV, apply_real(W)])
+    X = _t_orthonormalize(X[:nbnd])
+    HX = apply_real(X)
+    T = X @ HX.T
+    w, Vv = torch.linalg.eigh(0.5 * (T + T.T))
+    X = Vv.T @ X
+    evc = from_real_t(X).cpu().numpy()
+    return w[:nbnd].cpu().numpy(), evc[:nbnd]
+
+
+def make_ground_state_device(system: System, apply_h_factory, nbnd: int | None = None, tol: float = 1e-6,
+                             verbose: bool = False, npwx: int | None = None) -> GroundState:
+    """Non-self-consistent ground state (LDA potential of the superposed atomic
+    densities) with the bands found on the device: for the largest workloads,
+    where the host eigensolver of :func:`make_ground_state` is too slow.
+    ``apply_h_factory(gs)`` must return a callable accepting device tensors."""
+    b = system.basis
+    npwx = npwx or b.npw
+    nbndval = system.nbnd_occ
+    nbnd = nbnd or nbndval
+    vkb, dvan = kb_projectors(system)
+    vkbx = np.zeros((vkb.shape[1], npwx), dtype=np.complex128)
+    vkbx[:, :b.npw] = vkb.T
+    vrs = ks_potential(system, atomic_density_g(system), local_ionic_potential_g(system))
+    gs = GroundState(basis=b, npwx=npwx, vrs=vrs, vkb=vkbx, dvan=dvan,
+                     evc=np.zeros((nbnd, npwx), dtype=np.complex128), et=np.zeros(nbnd),
+                     nbndval=nbndval, occ=np.ones(nbndval), meta={"system": system.name, "scf_steps": 0})
+    et, evc = solve_bands_device(b, apply_h_factory(gs), nbnd, npwx, tol=tol, verbose=verbose)
+    gs.evc = evc
+    gs.et = et
+    gs.alpha_pv = max(2.0 * (float(et[:nbndval].max()) - float(et.min())), 0.01)
+    return gs

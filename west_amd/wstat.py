@@ -160,7 +160,8 @@ def gpu_h_factory(gs):
     check(lib.wh_set_potential(ptr(np.ascontiguousarray(gs.vrs, dtype=np.float64))))
 
     def h_psi(psi):
-        psi = np.ascontiguousarray(psi, dtype=np.complex128)
+        if not hasattr(psi, "data_ptr"):
+            psi = np.ascontiguousarray(psi, dtype=np.complex128)
         return apply_sternheimerop_to_m_wfcs(0, psi, np.zeros(psi.shape[0]), 0.0)
 
     return h_psi
