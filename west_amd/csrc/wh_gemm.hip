@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <map>
+#include <tuple>
 #include <cstring>
 
 namespace wh {
@@ -36,8 +37,8 @@ namespace wh {
 // K = nq) on MI355X. The bra products (small M = nq, long K = 2 npw, fused
 // G = 0 correction and row scale) use whichever of the split-K kernel of this
 // file and rocBLAS + k_g0_scale was faster when first timed for that (M, K)
-// on a call with N >= 512 columns (the kernel won at nq = 384, rocBLAS at
-// nq = 1296); smaller calls use the kernel until then.
+// and the power-of-two bucket of N, for N >= 512 columns (at 8192 columns the
+// kernel won at nq = 384, rocBLAS at nq = 1296); smaller calls use the kernel.
 // WH_GEMM=own or WH_GEMM=rocblas puts every product on one backend.
 enum class Backend { Default, Own, Rocblas };
 static Backend backend() {
@@ -314,8 +315,10 @@ void dgemm_bra(int M, int N, int K, double alpha, const double* A, long lda, con
   const Backend be = backend();
   if (be == Backend::Rocblas && K > 0) return dgemm_bra_lib(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s);
   if (be == Backend::Own || K <= 0) return dgemm_bra_own(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s);
-  static std::map<std::pair<int, int>, int> chosen;   // (M, K) -> 0 kernel, 1 rocBLAS
-  const auto key = std::make_pair(M, K);
+  static std::map<std::tuple<int, int, int>, int> chosen;   // (M, K, log2 N) -> 0 kernel, 1 rocBLAS
+  int lg = 0;
+  while ((2 << lg) <= N) ++lg;
+  const auto key = std::make_tuple(M, K, lg);
   auto it = chosen.find(key);
   if (it == chosen.end() && N >= 512) {
     // time both on this call's arguments
