@@ -84,8 +84,10 @@ int wh_set_params(int n_dfpt_maxiter, int l_kinetic_only);
  * the matrix cores in split BF16 with FP32 accumulation: 3 bf16 products per
  * product in mode 1, about 16 bits; 6 in mode 2, about FP32). The CG iterates
  * in FP32 on an increment of the FP64 solution; the FP64 residual b - A x is
- * recomputed whenever the FP32 rho has dropped by refine_delta (<= 0: 1e-4)
- * since the last recomputation, and a band converges only when that FP64 rho
+ * recomputed whenever the FP32 rho has dropped by refine_delta (<= 0: 1e-6 in
+ * mode 1, 1e-8 in mode 2)
+ * since the last recomputation (or every 40 FP32 iterations; the first
+ * residual is formed with the FP32 operator), and a band converges only when that FP64 rho
  * is below tr2 (the reference's test). A band whose FP64 residual twice stays
  * far above the FP32 estimate is finished by the FP64 CG. The reference has
  * no such mode; call surfaces and convergence test are unchanged. Modes 1-2

@@ -22,6 +22,8 @@ void fold_increment(int nu, int npw, const int* u2i, z64* x, long ldx, z32* dx, 
 //   r = tc[u] - b[ib]; rho[u] = <M^-1 r|r>; g[ib] = r (FP32)
 void true_residual(int nu, int npw, int gstart, const int* u2i, const z64* tc, const z64* b, long ldx,
                    const double* g2kin, const double* eprec, z32* g, long ld32, double* rho, hipStream_t s);
+// g[r] = t[r] - b[r] (FP32 operator output minus the FP64 right-hand side), rows 0..m-1
+void residual32(int m, int npw, const z32* t, long ld32, const z64* b, long ldb, z32* g, hipStream_t s);
 // out[r][c] = in[r][c] for c < cols, FP64 -> FP32, rows of ld_out (pad left untouched)
 void to_f32(int rows, int cols, const double* in, long ld_in, float* out, long ld_out, hipStream_t s);
 void to_f64(int rows, int cols, const float* in, long ld_in, double* out, long ld_out, hipStream_t s);

@@ -200,6 +200,16 @@ __global__ __launch_bounds__(NT) void k_true_residual(int npw, int gstart, const
   if (threadIdx.x == 0) rho[u] = 2.0 * s - (gstart == 2 ? r0 : 0.0);
 }
 
+__global__ void k_residual32(int npw, const z32* t, long ld32, const z64* b, long ldb, z32* g) {
+  const int r = blockIdx.y;
+  const int q = blockIdx.x * NT + threadIdx.x;
+  if (q >= npw) return;
+  const z32 tv = t[static_cast<long>(r) * ld32 + q];
+  const z64 bv = b[static_cast<long>(r) * ldb + q];
+  g[static_cast<long>(r) * ld32 + q] = mk(static_cast<float>(static_cast<double>(tv.x) - bv.x),
+                                          static_cast<float>(static_cast<double>(tv.y) - bv.y));
+}
+
 __global__ void k_to_f64(int cols, const float* in, long ld_in, double* out, long ld_out) {
   const int r = blockIdx.y;
   const long c = static_cast<long>(blockIdx.x) * NT + threadIdx.x;
@@ -359,6 +369,12 @@ void true_residual(int nu, int npw, int gstart, const int* u2i, const z64* tc, c
                    const double* g2kin, const double* eprec, z32* g, long ld32, double* rho, hipStream_t s) {
   if (nu <= 0) return;
   k_true_residual<<<nu, NT, 0, s>>>(npw, gstart, u2i, tc, b, ldx, g2kin, eprec, g, ld32, rho);
+  WH_HIP(hipGetLastError());
+}
+
+void residual32(int m, int npw, const z32* t, long ld32, const z64* b, long ldb, z32* g, hipStream_t s) {
+  if (m <= 0) return;
+  k_residual32<<<dim3(ceil_div(npw, NT), m), NT, 0, s>>>(npw, t, ld32, b, ldb, g);
   WH_HIP(hipGetLastError());
 }
 

@@ -172,7 +172,8 @@ def set_precision(mode: str = "fp64", refine_delta: float = 0.0):
     reference's), ``"mixed"`` (FP32 operator with 3-product split-BF16
     projector GEMMs, CG in FP32 on an increment of the FP64 solution, FP64
     residual recomputed when rho has dropped by ``refine_delta`` (0: library
-    default 1e-4); convergence is tested on the FP64 residual) or
+    default, 1e-6 for "mixed" and 1e-8 for "mixed6"); convergence is tested
+    on the FP64 residual) or
     ``"mixed6"`` (the same with 6-product split-BF16 GEMMs, about FP32
     accurate)."""
     modes = {"fp64": 0, "mixed": 1, "mixed6": 2}
