@@ -83,14 +83,16 @@ int wh_set_params(int n_dfpt_maxiter, int l_kinetic_only);
  * modes 1 and 2: the operator runs in FP32 (FFTs, vectors; projector GEMMs on
  * the matrix cores in split BF16 with FP32 accumulation: 3 bf16 products per
  * product in mode 1, about 16 bits; 6 in mode 2, about FP32). The CG iterates
- * in FP32 on an increment of the FP64 solution; the FP64 residual b - A x is
- * recomputed whenever the FP32 rho has dropped by refine_delta (<= 0: 1e-6 in
- * mode 1, 1e-8 in mode 2)
- * since the last recomputation (or every 40 FP32 iterations; the first
- * residual is formed with the FP32 operator), and a band converges only when that FP64 rho
- * is below tr2 (the reference's test). A band whose FP64 residual twice stays
- * far above the FP32 estimate is finished by the FP64 CG. The reference has
- * no such mode; call surfaces and convergence test are unchanged. Modes 1-2
+ * in FP32 on an increment of the FP64 solution (its first residual formed with
+ * the FP32 operator); the FP64 residual b - A x is recomputed whenever the
+ * FP32 rho has dropped by refine_delta (<= 0: 1e-6 in mode 1, 1e-8 in mode 2)
+ * since the last recomputation, or after 40 FP32 iterations, and a band
+ * converges only when that FP64 rho is below tr2 (the reference's test).
+ * Bands whose FP64 residual twice drops less than tenfold between
+ * recomputations, bands still unconverged after n_dfpt_maxiter/2 FP32
+ * iterations and the last 1/128 of a large batch are finished by the FP64 CG.
+ * The reference has no such mode; call surfaces and convergence test are
+ * unchanged. Modes 1-2
  * need a cubic FFT grid of a size with compiled kernels (18, 20, 24, 30, 32,
  * 36, 40, 45, 48, 54, 60, 64, 72, 80, 90, 96, 100, 108). */
 int wh_set_precision(int mode, double refine_delta);
