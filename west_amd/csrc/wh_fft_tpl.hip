@@ -23,10 +23,11 @@ namespace {
 
 constexpr int NT = 256;
 // stored columns per workgroup of the z-kernels: one z row of a workgroup's
-// columns is a 128-byte line of the sticks buffer
-template <typename T>
+// columns is one (zfwd) or two (zinv) 128-byte lines of the sticks buffer
+// (measured best on the 72^3 grid in both precisions)
+template <typename T, bool INV>
 constexpr int cpw() {
-  return 128 / static_cast<int>(sizeof(cplx<T>));
+  return (INV ? 256 : 128) / static_cast<int>(sizeof(cplx<T>));
 }
 
 // ---------------------------------------------------------------------------
@@ -223,6 +224,15 @@ struct Plan {
     for (int i = 0; i < s; ++i) p *= P.r[i];
     return p;
   }
+  // twiddles of pass s >= 1 are stored [k][j] (k < stride(s), 1 <= j < radix(s))
+  // from twoff(s): the R - 1 roots one butterfly needs are adjacent, and
+  // neighbouring butterflies read 16 (R - 1) bytes apart (no LDS bank clashes)
+  static constexpr int twoff(int s) {
+    int o = 0;
+    for (int i = 1; i < s; ++i) o += stride(i) * (radix(i) - 1);
+    return o;
+  }
+  static constexpr int twsize() { return twoff(nst) > 0 ? twoff(nst) : 1; }
 };
 
 // ---------------------------------------------------------------------------
@@ -236,7 +246,6 @@ __device__ __forceinline__ void pass(int nlines, const cplx<T>* tw, Ld ld, St st
   constexpr int P = Plan<N>::stride(S);
   constexpr int NBL = N / R;
   constexpr int NB = (MAXL * NBL + BT - 1) / BT;
-  constexpr int STEP = N / (P * R);
   const int nbf = nlines * NBL;
   cplx<T> v[NB][R];
 #pragma unroll
@@ -258,24 +267,13 @@ __device__ __forceinline__ void pass(int nlines, const cplx<T>* tw, Ld ld, St st
       const int i = bf - line * NBL;
       const int k = (P == 1) ? 0 : i % P;
       if (P > 1 && k != 0) {
-        cplx<T> w[R];
-        if constexpr (sizeof(T) == 4) {
-          // FP32: w^j from one LDS read of w = root^(k STEP), w^j = w^(j/2) w^(j - j/2)
-          // (at most log2(R) roundings deep; saves R - 2 LDS reads per butterfly)
-          w[1] = tw[k * STEP];
-          if (DIR > 0) w[1].y = -w[1].y;
+        const cplx<T>* wk = tw + Plan<N>::twoff(S) + k * (R - 1);
 #pragma unroll
-          for (int j = 2; j < R; ++j) w[j] = cmul(w[j / 2], w[j - j / 2]);
-        } else {
-          // FP64: table roots (FP64 products cost more than the LDS reads they save)
-#pragma unroll
-          for (int j = 1; j < R; ++j) {
-            w[j] = tw[j * k * STEP];
-            if (DIR > 0) w[j].y = -w[j].y;
-          }
+        for (int j = 1; j < R; ++j) {
+          cplx<T> w = wk[j - 1];
+          if (DIR > 0) w.y = -w.y;
+          v[t][j] = cmul(v[t][j], w);
         }
-#pragma unroll
-        for (int j = 1; j < R; ++j) v[t][j] = cmul(v[t][j], w[j]);
       }
       dft<T, R, DIR>(v[t]);
       const int base = (i - k) * R + k;
@@ -288,34 +286,49 @@ __device__ __forceinline__ void pass(int nlines, const cplx<T>* tw, Ld ld, St st
 
 // full transform: pass 0 loads through first(), the last pass stores through
 // last(), the passes in between work in place on buf (line stride N)
-template <typename T, int N, int S, int MAXL, int BT, int DIR, class First, class Last>
+template <typename T, int N, int S, int MAXL, int BT, int DIR, class First, class Last, int LS = N>
 __device__ __forceinline__ void passes(int nlines, cplx<T>* buf, const cplx<T>* tw, First first, Last last) {
   constexpr int NS = Plan<N>::nst;
-  auto in_buf = [buf](int line, int idx) { return buf[line * N + idx]; };
-  auto to_buf = [buf](int line, int idx, cplx<T> v) { buf[line * N + idx] = v; };
+  auto in_buf = [buf](int line, int idx) { return buf[line * LS + idx]; };
+  auto to_buf = [buf](int line, int idx, cplx<T> v) { buf[line * LS + idx] = v; };
   if constexpr (NS == 1) {
     pass<T, N, 0, MAXL, BT, DIR>(nlines, tw, first, last);
   } else if constexpr (S == 0) {
     pass<T, N, 0, MAXL, BT, DIR>(nlines, tw, first, to_buf);
-    passes<T, N, 1, MAXL, BT, DIR>(nlines, buf, tw, first, last);
+    passes<T, N, 1, MAXL, BT, DIR, First, Last, LS>(nlines, buf, tw, first, last);
   } else if constexpr (S == NS - 1) {
     pass<T, N, S, MAXL, BT, DIR>(nlines, tw, in_buf, last);
   } else {
     pass<T, N, S, MAXL, BT, DIR>(nlines, tw, in_buf, to_buf);
-    passes<T, N, S + 1, MAXL, BT, DIR>(nlines, buf, tw, first, last);
+    passes<T, N, S + 1, MAXL, BT, DIR, First, Last, LS>(nlines, buf, tw, first, last);
   }
 }
 
-template <typename T, int N, int MAXL, int BT, int DIR>
+// LS: distance between lines in buf (N, or N + 1 for the plane rows, whose
+// column reads would otherwise fall on one or two LDS bank groups)
+template <typename T, int N, int MAXL, int BT, int DIR, int LS = N>
 __device__ __forceinline__ void fft_inplace(int nlines, cplx<T>* buf, const cplx<T>* tw) {
-  auto in_buf = [buf](int line, int idx) { return buf[line * N + idx]; };
-  auto to_buf = [buf](int line, int idx, cplx<T> v) { buf[line * N + idx] = v; };
-  passes<T, N, 0, MAXL, BT, DIR>(nlines, buf, tw, in_buf, to_buf);
+  auto in_buf = [buf](int line, int idx) { return buf[line * LS + idx]; };
+  auto to_buf = [buf](int line, int idx, cplx<T> v) { buf[line * LS + idx] = v; };
+  passes<T, N, 0, MAXL, BT, DIR, decltype(in_buf), decltype(to_buf), LS>(nlines, buf, tw, in_buf, to_buf);
 }
 
+// pass twiddle tables (Plan<N>::twoff) from the forward roots g[m] = exp(-2 pi i m / N)
 template <typename T, int N>
 __device__ __forceinline__ void load_tw(cplx<T>* tw, const double* g) {
-  for (int t = threadIdx.x; t < N; t += blockDim.x) tw[t] = mk<T>(T(g[2 * t]), T(g[2 * t + 1]));
+  using PL = Plan<N>;
+#pragma unroll
+  for (int s = 1; s < PL::nst; ++s) {
+    const int R = PL::radix(s);
+    const int step = N / (PL::stride(s) * R);
+    const int n = PL::stride(s) * (R - 1);
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+      const int k = t / (R - 1);
+      const int j = t - k * (R - 1) + 1;
+      const int m = (j * k * step) % N;
+      tw[PL::twoff(s) + t] = mk<T>(T(g[2 * m]), T(g[2 * m + 1]));
+    }
+  }
 }
 
 // threads of the per-plane kernels (two waves per SIMD)
@@ -327,12 +340,18 @@ template <int N>
 constexpr int max_rows() {
   return N / 2 + 2;
 }
+// row stride of the plane buffer (one element of padding: column accesses
+// stride over all LDS banks)
+template <int N>
+constexpr int prow() {
+  return N + 1;
+}
 // x-line chunk of the plane kernel when `occ` workgroups share the 160 KB of
 // LDS of a CU (more resident workgroups hide the barriers of the LDS passes)
 template <typename T, int N>
 constexpr int x_budget(bool density, int occ) {
   const int budget = 160 * 1024 / occ - 2048;
-  const int fixed = static_cast<int>(sizeof(cplx<T>)) * (N + max_rows<N>() * N) +
+  const int fixed = static_cast<int>(sizeof(cplx<T>)) * (Plan<N>::twsize() + max_rows<N>() * prow<N>()) +
                     (density ? static_cast<int>(sizeof(T)) * N * N : 0);
   return (budget - fixed) / (static_cast<int>(sizeof(cplx<T>)) * N);
 }
@@ -357,9 +376,10 @@ constexpr int x_chunk(bool density, int occ = 1) {
 template <typename T, int N, typename V>
 __global__ __launch_bounds__(NT) void k_zinv_t(StickPlanDev P, const FftItem* items, const cplx<V>* psiA, long ldA,
                                                const cplx<V>* psiB, long ldB, cplx<T>* sticks, int mode) {
-  constexpr int CPW = cpw<T>();
-  __shared__ cplx<T> tw[N];
-  __shared__ cplx<T> lines[2 * CPW * N];
+  constexpr int CPW = cpw<T, true>();
+  __shared__ cplx<T> tw[Plan<N>::twsize()];
+  constexpr int LS = N + 1;   // line stride: column-order accesses spread over the LDS banks
+  __shared__ cplx<T> lines[2 * CPW * LS];
   const int it = blockIdx.y;
   const int c0 = blockIdx.x * CPW;
   const int c1 = min(c0 + CPW, P.ncs);
@@ -368,7 +388,7 @@ __global__ __launch_bounds__(NT) void k_zinv_t(StickPlanDev P, const FftItem* it
   const cplx<V>* A = psiA + static_cast<long>(item.a) * ldA;
   const cplx<V>* B = item.b >= 0 ? psiB + static_cast<long>(item.b) * ldB : nullptr;
   load_tw<T, N>(tw, P.tw3);
-  for (int t = threadIdx.x; t < 2 * ncl * N; t += NT) lines[t] = mk<T>(T(0), T(0));
+  for (int t = threadIdx.x; t < 2 * ncl * LS; t += NT) lines[t] = mk<T>(T(0), T(0));
   __syncthreads();
   const int s0 = P.col_off[c0];
   const int s1 = P.col_off[c1];
@@ -378,7 +398,7 @@ __global__ __launch_bounds__(NT) void k_zinv_t(StickPlanDev P, const FftItem* it
     if (z < 0) z += N;
     const cplx<V> a1 = A[s];
     const cplx<V> a2 = B ? B[s] : mk<V>(V(0), V(0));
-    lines[(2 * lc) * N + z] = mk<T>(T(a1.x - a2.y), T(a1.y + a2.x));
+    lines[(2 * lc) * LS + z] = mk<T>(T(a1.x - a2.y), T(a1.y + a2.x));
   }
   __syncthreads();
   for (int s = s0 + threadIdx.x; s < s1; s += NT) {
@@ -390,10 +410,10 @@ __global__ __launch_bounds__(NT) void k_zinv_t(StickPlanDev P, const FftItem* it
     cplx<V> v = mk<V>(a1.x + a2.y, a2.x - a1.y);
     if (mode == ZINV_PAIR_REALG0 && s == 0) v = mk<V>(a1.x, a2.x);
     const int line = (c0 + lc == 0) ? 0 : 2 * lc + 1;
-    lines[line * N + zm] = mk<T>(T(v.x), T(v.y));
+    lines[line * LS + zm] = mk<T>(T(v.x), T(v.y));
   }
   __syncthreads();
-  fft_inplace<T, N, 2 * CPW, NT, +1>(2 * ncl, lines, tw);
+  fft_inplace<T, N, 2 * CPW, NT, +1, LS>(2 * ncl, lines, tw);
   cplx<T>* out = sticks + static_cast<long>(it) * N * P.nct;
   for (int t = threadIdx.x; t < N * 2 * ncl; t += NT) {
     const int z = t / (2 * ncl);
@@ -403,7 +423,7 @@ __global__ __launch_bounds__(NT) void k_zinv_t(StickPlanDev P, const FftItem* it
     const int c = c0 + lc;
     if (!own && c == 0) continue;
     const int col = own ? c : c + P.ncs - 1;
-    out[static_cast<long>(z) * P.nct + col] = lines[(2 * lc + (own ? 0 : 1)) * N + z];
+    out[static_cast<long>(z) * P.nct + col] = lines[(2 * lc + (own ? 0 : 1)) * LS + z];
   }
 }
 
@@ -413,9 +433,10 @@ __global__ __launch_bounds__(NT) void k_zinv_t(StickPlanDev P, const FftItem* it
 template <typename T, int N, typename V>
 __global__ __launch_bounds__(NT) void k_zfwd_t(StickPlanDev P, const FftItem* items, const cplx<T>* sticks,
                                                const cplx<V>* in, long ldin, cplx<V>* out, long ldout, ZfwdArgs A) {
-  constexpr int CPW = cpw<T>();
-  __shared__ cplx<T> tw[N];
-  __shared__ cplx<T> lines[2 * CPW * N];
+  constexpr int CPW = cpw<T, false>();
+  __shared__ cplx<T> tw[Plan<N>::twsize()];
+  constexpr int LS = N + 1;   // line stride: column-order accesses spread over the LDS banks
+  __shared__ cplx<T> lines[2 * CPW * LS];
   const int it = blockIdx.y;
   const int c0 = blockIdx.x * CPW;
   const int c1 = min(c0 + CPW, P.ncs);
@@ -436,10 +457,10 @@ __global__ __launch_bounds__(NT) void k_zfwd_t(StickPlanDev P, const FftItem* it
       const int col = own ? c : c + P.ncs - 1;
       v = src[static_cast<long>(z) * P.nct + col];
     }
-    lines[(single ? lc : 2 * lc + (own ? 0 : 1)) * N + z] = v;
+    lines[(single ? lc : 2 * lc + (own ? 0 : 1)) * LS + z] = v;
   }
   __syncthreads();
-  fft_inplace<T, N, 2 * CPW, NT, -1>(nl, lines, tw);
+  fft_inplace<T, N, 2 * CPW, NT, -1, LS>(nl, lines, tw);
   const int s0 = P.col_off[c0];
   const int s1 = P.col_off[c1];
   cplx<V>* oa = out + static_cast<long>(item.a) * ldout;
@@ -450,7 +471,7 @@ __global__ __launch_bounds__(NT) void k_zfwd_t(StickPlanDev P, const FftItem* it
     int z = k % N;
     if (z < 0) z += N;
     if (single) {
-      const cplx<T> F = lines[lc * N + z];
+      const cplx<T> F = lines[lc * LS + z];
       const double w = A.scale * (A.weight ? A.weight[s] : 1.0);
       cplx<V> v = mk<V>(V(static_cast<double>(F.x) * w), V(static_cast<double>(F.y) * w));
       if (A.zero_g0 && s == 0) v = mk<V>(V(0), V(0));
@@ -459,9 +480,9 @@ __global__ __launch_bounds__(NT) void k_zfwd_t(StickPlanDev P, const FftItem* it
     }
     int zm = (-k) % N;
     if (zm < 0) zm += N;
-    const cplx<T> Ft = lines[(2 * lc) * N + z];
+    const cplx<T> Ft = lines[(2 * lc) * LS + z];
     const int ml = (c0 + lc == 0) ? 0 : 2 * lc + 1;
-    const cplx<T> Fmt = lines[ml * N + zm];
+    const cplx<T> Fmt = lines[ml * LS + zm];
     const cplx<V> F = mk<V>(V(Ft.x), V(Ft.y));
     const cplx<V> Fm = mk<V>(V(Fmt.x), V(Fmt.y));
     const cplx<V> fp = cscale(F + Fm, V(0.5));
@@ -498,16 +519,23 @@ __device__ __forceinline__ PlaneLds<T, N> plane_lds() {
   extern __shared__ char smem_raw[];
   PlaneLds<T, N> s;
   s.tw = reinterpret_cast<cplx<T>*>(smem_raw);
-  s.pl = s.tw + N;
-  s.x = s.pl + max_rows<N>() * N;
+  s.pl = s.tw + Plan<N>::twsize();
+  s.x = s.pl + max_rows<N>() * prow<N>();
   return s;
+}
+
+// plane-buffer index of a column position r * N + j (col_pos)
+template <int N>
+__device__ __forceinline__ int pl_pos(int pos) {
+  const int r = pos / N;
+  return pos + r * (prow<N>() - N);
 }
 
 template <typename T, int N>
 __device__ __forceinline__ void plane_fill(const StickPlanDev& P, PlaneLds<T, N>& S, const cplx<T>* src) {
-  for (int t = threadIdx.x; t < P.nrows * N; t += NTP) S.pl[t] = mk<T>(T(0), T(0));
+  for (int t = threadIdx.x; t < P.nrows * prow<N>(); t += NTP) S.pl[t] = mk<T>(T(0), T(0));
   __syncthreads();
-  for (int c = threadIdx.x; c < P.nct; c += NTP) S.pl[P.col_pos[c]] = src[c];
+  for (int c = threadIdx.x; c < P.nct; c += NTP) S.pl[pl_pos<N>(P.col_pos[c])] = src[c];
   __syncthreads();
 }
 
@@ -533,7 +561,7 @@ __global__ __launch_bounds__(NTP) __attribute__((amdgpu_waves_per_eu(2 * OCC))) 
   const int imax = P.imax;
   load_tw<T, N>(S.tw, P.tw1);
   plane_fill<T, N>(P, S, sin + static_cast<long>(item.a) * sl + static_cast<long>(z) * P.nct);
-  fft_inplace<T, N, MR, NTP, +1>(P.nrows, S.pl, S.tw);
+  fft_inplace<T, N, MR, NTP, +1, prow<N>()>(P.nrows, S.pl, S.tw);
   const PT* V = vpot + (item.pot >= 0 ? static_cast<long>(item.pot) * ldpot : 0L) + static_cast<long>(z) * N * N;
   const T sc = T(scale);
   cplx<T>* pl = S.pl;
@@ -543,7 +571,7 @@ __global__ __launch_bounds__(NTP) __attribute__((amdgpu_waves_per_eu(2 * OCC))) 
     // inverse along a1: zero-padded rows in, V(r) applied on the way out
     auto from_rows = [=](int jj, int x) {
       const int r = row_of(x, N, imax);
-      return r >= 0 ? pl[r * N + j0 + jj] : mk<T>(T(0), T(0));
+      return r >= 0 ? pl[r * prow<N>() + j0 + jj] : mk<T>(T(0), T(0));
     };
     auto times_v = [=](int jj, int x, cplx<T> v) {
       xb[jj * N + x] = cscale(v, T(V[(j0 + jj) * N + x]) * sc);
@@ -553,13 +581,13 @@ __global__ __launch_bounds__(NTP) __attribute__((amdgpu_waves_per_eu(2 * OCC))) 
     auto from_x = [=](int jj, int x) { return xb[jj * N + x]; };
     auto to_rows = [=](int jj, int x, cplx<T> v) {
       const int r = row_of(x, N, imax);
-      if (r >= 0) pl[r * N + j0 + jj] = v;
+      if (r >= 0) pl[r * prow<N>() + j0 + jj] = v;
     };
     passes<T, N, 0, JC, NTP, -1>(nj, xb, S.tw, from_x, to_rows);
   }
-  fft_inplace<T, N, MR, NTP, -1>(P.nrows, S.pl, S.tw);
+  fft_inplace<T, N, MR, NTP, -1, prow<N>()>(P.nrows, S.pl, S.tw);
   cplx<T>* dst = sout + static_cast<long>(it) * sl + static_cast<long>(z) * P.nct;
-  for (int c = threadIdx.x; c < P.nct; c += NTP) dst[c] = S.pl[P.col_pos[c]];
+  for (int c = threadIdx.x; c < P.nct; c += NTP) dst[c] = S.pl[pl_pos<N>(P.col_pos[c])];
 }
 
 template <typename T, int N>
@@ -575,7 +603,7 @@ __global__ __launch_bounds__(NTP) void k_plane_to_real_t(StickPlanDev P, const F
   const int imax = P.imax;
   load_tw<T, N>(S.tw, P.tw1);
   plane_fill<T, N>(P, S, sin + static_cast<long>(it) * sl + static_cast<long>(z) * P.nct);
-  fft_inplace<T, N, MR, NTP, +1>(P.nrows, S.pl, S.tw);
+  fft_inplace<T, N, MR, NTP, +1, prow<N>()>(P.nrows, S.pl, S.tw);
   double* oa = out + static_cast<long>(item.a) * ldout + static_cast<long>(z) * N * N;
   double* ob = item.b >= 0 ? out + static_cast<long>(item.b) * ldout + static_cast<long>(z) * N * N : nullptr;
   cplx<T>* pl = S.pl;
@@ -583,7 +611,7 @@ __global__ __launch_bounds__(NTP) void k_plane_to_real_t(StickPlanDev P, const F
     const int nj = min(JC, N - j0);
     auto from_rows = [=](int jj, int x) {
       const int r = row_of(x, N, imax);
-      return r >= 0 ? pl[r * N + j0 + jj] : mk<T>(T(0), T(0));
+      return r >= 0 ? pl[r * prow<N>() + j0 + jj] : mk<T>(T(0), T(0));
     };
     auto to_out = [=](int jj, int x, cplx<T> v) {
       oa[(j0 + jj) * N + x] = static_cast<double>(v.x);
@@ -609,13 +637,13 @@ __global__ __launch_bounds__(NTP) void k_plane_density_t(StickPlanDev P, int nba
   cplx<T>* pl = S.pl;
   for (int v = 0; v < nband; ++v) {
     plane_fill<T, N>(P, S, sin + (static_cast<long>(p) * nband + v) * sl + static_cast<long>(z) * P.nct);
-    fft_inplace<T, N, MR, NTP, +1>(P.nrows, S.pl, S.tw);
+    fft_inplace<T, N, MR, NTP, +1, prow<N>()>(P.nrows, S.pl, S.tw);
     const T o = T(occ[v]);
     for (int j0 = 0; j0 < N; j0 += JC) {
       const int nj = min(JC, N - j0);
       auto from_rows = [=](int jj, int x) {
         const int r = row_of(x, N, imax);
-        return r >= 0 ? pl[r * N + j0 + jj] : mk<T>(T(0), T(0));
+        return r >= 0 ? pl[r * prow<N>() + j0 + jj] : mk<T>(T(0), T(0));
       };
       // each (jj, x) is produced by exactly one thread: no race on acc
       auto accumulate = [=](int jj, int x, cplx<T> w) { acc[(j0 + jj) * N + x] += o * w.x * w.y; };
@@ -628,13 +656,13 @@ __global__ __launch_bounds__(NTP) void k_plane_density_t(StickPlanDev P, int nba
     auto from_acc = [=](int jj, int x) { return mk<T>(acc[(j0 + jj) * N + x], T(0)); };
     auto to_rows = [=](int jj, int x, cplx<T> w) {
       const int r = row_of(x, N, imax);
-      if (r >= 0) pl[r * N + j0 + jj] = w;
+      if (r >= 0) pl[r * prow<N>() + j0 + jj] = w;
     };
     passes<T, N, 0, JC, NTP, -1>(nj, S.x, S.tw, from_acc, to_rows);
   }
-  fft_inplace<T, N, MR, NTP, -1>(P.nrows, S.pl, S.tw);
+  fft_inplace<T, N, MR, NTP, -1, prow<N>()>(P.nrows, S.pl, S.tw);
   cplx<T>* dst = sout + static_cast<long>(p) * sl + static_cast<long>(z) * P.nct;
-  for (int c = threadIdx.x; c < P.nct; c += NTP) dst[c] = S.pl[P.col_pos[c]];
+  for (int c = threadIdx.x; c < P.nct; c += NTP) dst[c] = S.pl[pl_pos<N>(P.col_pos[c])];
 }
 
 // Response density for grids whose N x N FP64 accumulator does not fit in LDS
@@ -645,7 +673,7 @@ __global__ __launch_bounds__(NTP) void k_plane_density_t(StickPlanDev P, int nba
 template <typename T, int N>
 constexpr int xh_chunk() {
   const int budget = 160 * 1024 - 2048;
-  const int fixed = static_cast<int>(sizeof(cplx<T>)) * (N + max_rows<N>() * N) +
+  const int fixed = static_cast<int>(sizeof(cplx<T>)) * (Plan<N>::twsize() + max_rows<N>() * prow<N>()) +
                     static_cast<int>(sizeof(T)) * N * (N / 2);
   int jc = (budget - fixed) / (static_cast<int>(sizeof(cplx<T>)) * N);
   if (jc > N) jc = N;
@@ -656,7 +684,7 @@ constexpr int xh_chunk() {
 
 template <typename T, int N>
 constexpr size_t big_density_smem() {
-  return sizeof(cplx<T>) * (static_cast<size_t>(N) + static_cast<size_t>(max_rows<N>()) * N +
+  return sizeof(cplx<T>) * (static_cast<size_t>(Plan<N>::twsize()) + static_cast<size_t>(max_rows<N>()) * prow<N>() +
                             static_cast<size_t>(xh_chunk<T, N>()) * N) +
          sizeof(T) * static_cast<size_t>(N) * (N / 2);
 }
@@ -664,7 +692,7 @@ constexpr size_t big_density_smem() {
 // LDS of k_plane_density_t (whole N x N accumulator)
 template <typename T, int N>
 constexpr bool density_fits_lds() {
-  return sizeof(cplx<T>) * (static_cast<size_t>(N) + static_cast<size_t>(max_rows<N>()) * N +
+  return sizeof(cplx<T>) * (static_cast<size_t>(Plan<N>::twsize()) + static_cast<size_t>(max_rows<N>()) * prow<N>() +
                             static_cast<size_t>(x_chunk<T, N>(true)) * N) +
              sizeof(T) * static_cast<size_t>(N) * N <=
          160u * 1024u;
@@ -690,13 +718,13 @@ __global__ __launch_bounds__(NTP) void k_plane_density_big_t(StickPlanDev P, int
     for (int t = threadIdx.x; t < N * H; t += NTP) acc[t] = T(0);
     for (int v = 0; v < nband; ++v) {
       plane_fill<T, N>(P, S, sin + (static_cast<long>(p) * nband + v) * sl + static_cast<long>(z) * P.nct);
-      fft_inplace<T, N, MR, NTP, +1>(P.nrows, S.pl, S.tw);
+      fft_inplace<T, N, MR, NTP, +1, prow<N>()>(P.nrows, S.pl, S.tw);
       const T o = T(occ[v]);
       for (int j0 = 0; j0 < N; j0 += JC) {
         const int nj = min(JC, N - j0);
         auto from_rows = [=](int jj, int x) {
           const int r = row_of(x, N, imax);
-          return r >= 0 ? pl[r * N + j0 + jj] : mk<T>(T(0), T(0));
+          return r >= 0 ? pl[r * prow<N>() + j0 + jj] : mk<T>(T(0), T(0));
         };
         // each (jj, x) is produced by exactly one thread: no race on acc
         auto accumulate = [=](int jj, int x, cplx<T> w) {
@@ -721,13 +749,13 @@ __global__ __launch_bounds__(NTP) void k_plane_density_big_t(StickPlanDev P, int
     };
     auto to_rows = [=](int jj, int x, cplx<T> w) {
       const int r = row_of(x, N, imax);
-      if (r >= 0) pl[r * N + j0 + jj] = w;
+      if (r >= 0) pl[r * prow<N>() + j0 + jj] = w;
     };
     passes<T, N, 0, JC, NTP, -1>(nj, S.x, S.tw, from_acc, to_rows);
   }
-  fft_inplace<T, N, MR, NTP, -1>(P.nrows, S.pl, S.tw);
+  fft_inplace<T, N, MR, NTP, -1, prow<N>()>(P.nrows, S.pl, S.tw);
   cplx<T>* dst = sout + static_cast<long>(p) * sl + static_cast<long>(z) * P.nct;
-  for (int c = threadIdx.x; c < P.nct; c += NTP) dst[c] = S.pl[P.col_pos[c]];
+  for (int c = threadIdx.x; c < P.nct; c += NTP) dst[c] = S.pl[pl_pos<N>(P.col_pos[c])];
 }
 
 // ---------------------------------------------------------------------------
@@ -736,7 +764,7 @@ __global__ __launch_bounds__(NTP) void k_plane_density_big_t(StickPlanDev P, int
 template <typename T, int N>
 size_t plane_smem(bool density, int occ = 1) {
   const int jc = occ == 1 ? x_chunk<T, N>(density, 1) : occ == 2 ? x_chunk<T, N>(density, 2) : x_chunk<T, N>(density, 3);
-  size_t b = sizeof(cplx<T>) * (static_cast<size_t>(N) + static_cast<size_t>(max_rows<N>()) * N +
+  size_t b = sizeof(cplx<T>) * (static_cast<size_t>(Plan<N>::twsize()) + static_cast<size_t>(max_rows<N>()) * prow<N>() +
                                 static_cast<size_t>(jc) * N);
   if (density) b += sizeof(T) * static_cast<size_t>(N) * N;
   return b;
@@ -783,11 +811,11 @@ template <typename T, int N>
 struct Launch {
   static void zinv(const StickPlanDev& P, const FftItem* it, int n, const z64* a, long lda, const z64* b, long ldb,
                    void* st, int mode, hipStream_t s) {
-    k_zinv_t<T, N, double><<<dim3(ceil_div(P.ncs, cpw<T>()), n), NT, 0, s>>>(P, it, a, lda, b, ldb, static_cast<cplx<T>*>(st), mode);
+    k_zinv_t<T, N, double><<<dim3(ceil_div(P.ncs, cpw<T, true>()), n), NT, 0, s>>>(P, it, a, lda, b, ldb, static_cast<cplx<T>*>(st), mode);
   }
   static void zfwd(const StickPlanDev& P, const FftItem* it, int n, const void* st, const z64* in, long ldin, z64* out,
                    long ldout, const ZfwdArgs& A, hipStream_t s) {
-    k_zfwd_t<T, N, double><<<dim3(ceil_div(P.ncs, cpw<T>()), n), NT, 0, s>>>(P, it, static_cast<const cplx<T>*>(st), in, ldin, out,
+    k_zfwd_t<T, N, double><<<dim3(ceil_div(P.ncs, cpw<T, false>()), n), NT, 0, s>>>(P, it, static_cast<const cplx<T>*>(st), in, ldin, out,
                                                                  ldout, A);
   }
   static void vloc(const StickPlanDev& P, const FftItem* it, int n, const void* si, void* so, const double* v, long ldv,
@@ -827,12 +855,12 @@ template <int N>
 struct Launch32 {
   static void zinv(const StickPlanDev& P, const FftItem* it, int n, const z32* a, long lda, const z32* b, long ldb,
                    void* st, int mode, hipStream_t s) {
-    k_zinv_t<float, N, float><<<dim3(ceil_div(P.ncs, cpw<float>()), n), NT, 0, s>>>(P, it, a, lda, b, ldb,
+    k_zinv_t<float, N, float><<<dim3(ceil_div(P.ncs, cpw<float, true>()), n), NT, 0, s>>>(P, it, a, lda, b, ldb,
                                                                            static_cast<z32*>(st), mode);
   }
   static void zfwd(const StickPlanDev& P, const FftItem* it, int n, const void* st, const z32* in, long ldin, z32* out,
                    long ldout, const ZfwdArgs& A, hipStream_t s) {
-    k_zfwd_t<float, N, float><<<dim3(ceil_div(P.ncs, cpw<float>()), n), NT, 0, s>>>(P, it, static_cast<const z32*>(st), in,
+    k_zfwd_t<float, N, float><<<dim3(ceil_div(P.ncs, cpw<float, false>()), n), NT, 0, s>>>(P, it, static_cast<const z32*>(st), in,
                                                                            ldin, out, ldout, A);
   }
   static void vloc(const StickPlanDev& P, const FftItem* it, int n, const void* si, void* so, const float* v, long ldv,
