@@ -223,7 +223,7 @@ int linsolve_dev(int m, z64* b, z64* x, const std::vector<double>& e_col, const 
   const int itmax = maxiter > 0 ? maxiter : c.n_dfpt_maxiter;
   for (int iter = 1; iter <= itmax; ++iter) {
     WH_HIP(hipMemcpyAsync(d_l2i, l2i.data(), sizeof(int) * nc, hipMemcpyHostToDevice, s));
-    WH_TIMED("cg_prec", prec_rho(nc, npw, c.gstart, d_l2i, g, h, ld, c.d_g2kin, d_eprec_col, d_rho, s));
+    WH_TIMED("cg_prec", prec_rho(nc, npw, c.gstart, d_l2i, g, ld, c.d_g2kin, d_eprec_col, d_rho, s));
     WH_HIP(hipMemcpyAsync(rho_c.data(), d_rho, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
     WH_HIP(hipStreamSynchronize(s));
     ++c.cg_iters;
@@ -245,7 +245,7 @@ int linsolve_dev(int m, z64* b, z64* x, const std::vector<double>& e_col, const 
     WH_HIP(hipMemcpyAsync(d_l2i, l2i.data(), sizeof(int) * nc, hipMemcpyHostToDevice, s));
     WH_HIP(hipMemcpyAsync(d_beta, beta.data(), sizeof(double) * nc, hipMemcpyHostToDevice, s));
     WH_HIP(hipMemcpyAsync(d_eu, eu.data(), sizeof(double) * nc, hipMemcpyHostToDevice, s));
-    WH_TIMED("cg_dir", direction(nc, npw, d_l2i, d_beta, h, hold, bc, ld, s));
+    WH_TIMED("cg_dir", direction(nc, npw, d_l2i, d_beta, g, c.d_g2kin, d_eprec_col, h, hold, bc, ld, s));
     stern_apply(bc, tc, d_eu, nc, c.alpha_pv);
     WH_TIMED("cg_update", cg_update(nc, npw, c.gstart, d_l2i, h, g, tc, x, ld, nullptr, s));
     std::swap(h, hold);   // hold <- h  (linsolve 319)
@@ -418,7 +418,7 @@ int linsolve_mixed(int m, z64* b, z64* x, const std::vector<double>& e_col, cons
   auto precond = [&](const std::vector<int>& idx, int* d_idx) {
     const int n = static_cast<int>(idx.size());
     WH_HIP(hipMemcpyAsync(d_idx, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
-    WH_TIMED("cg_prec32", prec_rho(n, npw, c.gstart, d_idx, g, h, l32, c.d_g2kin, d_eprec_col, d_rho, s));
+    WH_TIMED("cg_prec32", prec_rho(n, npw, c.gstart, d_idx, g, l32, c.d_g2kin, d_eprec_col, d_rho, s));
     WH_HIP(hipMemcpyAsync(tmp.data(), d_rho, sizeof(double) * n, hipMemcpyDeviceToHost, s));
     WH_HIP(hipStreamSynchronize(s));
     for (int k = 0; k < n; ++k) rho[idx[k]] = tmp[k];
@@ -480,7 +480,7 @@ int linsolve_mixed(int m, z64* b, z64* x, const std::vector<double>& e_col, cons
     WH_HIP(hipMemcpyAsync(d_l2i, l2i.data(), sizeof(int) * nc, hipMemcpyHostToDevice, s));
     WH_HIP(hipMemcpyAsync(d_beta, beta.data(), sizeof(double) * nc, hipMemcpyHostToDevice, s));
     WH_HIP(hipMemcpyAsync(d_eu, eu.data(), sizeof(double) * nc, hipMemcpyHostToDevice, s));
-    WH_TIMED("cg_dir32", direction(nc, npw, d_l2i, d_beta, h, hold, bc, l32, s));
+    WH_TIMED("cg_dir32", direction(nc, npw, d_l2i, d_beta, g, c.d_g2kin, d_eprec_col, h, hold, bc, l32, s));
     stern_apply32(bc, tc, d_eu, nc, c.alpha_pv);
     WH_TIMED("cg_update32", cg_update(nc, npw, c.gstart, d_l2i, h, g, tc, dx, l32, nullptr, s));
     std::swap(h, hold);

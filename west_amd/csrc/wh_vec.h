@@ -6,12 +6,16 @@
 
 namespace wh {
 
-void prec_rho(int nc, int npw, int gstart, const int* l2i, const z64* g, z64* h, long ld, const double* g2kin,
+// rho[lb] = <M^-1 g|g> for the columns l2i[0..nc)
+void prec_rho(int nc, int npw, int gstart, const int* l2i, const z64* g, long ld, const double* g2kin,
               const double* eprec, double* rho, hipStream_t s);
-void prec_rho(int nc, int npw, int gstart, const int* l2i, const z32* g, z32* h, long ld, const double* g2kin,
+void prec_rho(int nc, int npw, int gstart, const int* l2i, const z32* g, long ld, const double* g2kin,
               const double* eprec, double* rho, hipStream_t s);
-void direction(int nc, int npw, const int* l2i, const double* beta, z32* h, const z32* hold, z32* bc, long ld,
-               hipStream_t s);
+// h = -M^-1 g + beta hold, bc = h (compact)
+void direction(int nc, int npw, const int* l2i, const double* beta, const z64* g, const double* g2kin,
+               const double* eprec, z64* h, const z64* hold, z64* bc, long ld, hipStream_t s);
+void direction(int nc, int npw, const int* l2i, const double* beta, const z32* g, const double* g2kin,
+               const double* eprec, z32* h, const z32* hold, z32* bc, long ld, hipStream_t s);
 void cg_update(int nc, int npw, int gstart, const int* l2i, const z32* h, z32* g, const z32* tc, z32* x, long ld,
                double* ac_out, hipStream_t s);
 void kinetic_minus_e(int m, int npw, const z32* x, z32* t, long ld, const double* g2kin, const double* e,
@@ -31,8 +35,6 @@ void to_f64(int rows, int cols, const float* in, long ld_in, double* out, long l
 void copy_rows(int n, int npw, const int* list, const z64* in, long ld_in, z64* out, long ld_out, bool scatter,
                hipStream_t s);
 void gather_real(int n, const int* list, const double* in, double* out, hipStream_t s);
-void direction(int nc, int npw, const int* l2i, const double* beta, z64* h, const z64* hold, z64* bc, long ld,
-               hipStream_t s);
 void cg_update(int nc, int npw, int gstart, const int* l2i, const z64* h, z64* g, const z64* tc, z64* x, long ld,
                double* ac_out, hipStream_t s);
 void gather_sticks(int m, int npw, const int* perm, const z64* in, long ldin, z64* out, long ldout, const double* w,

@@ -11,8 +11,10 @@ namespace {
 
 constexpr int NT = 256;
 
+// sum over the block (BT threads): wavefront reduction, then across the
+// wavefronts in a fixed order
+template <int BT = NT>
 __device__ __forceinline__ double block_sum(double v, double* red) {
-  // wavefront reduction then across the 4 wavefronts of the block
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
@@ -20,7 +22,7 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   if (lane == 0) red[w] = v;
   __syncthreads();
   double s = 0.0;
-  for (int i = 0; i < NT / 64; ++i) s += red[i];
+  for (int i = 0; i < BT / 64; ++i) s += red[i];
   return s;
 }
 
@@ -30,36 +32,35 @@ __device__ __forceinline__ double gamma_dot_part(const z64* a, const z64* b, int
   return s;
 }
 
-// h[ib] = g[ib] / max(1, g2kin/eprec[ib]);  rho[lb] = <h|g>   (linsolve 150-181)
+// rho[lb] = <M^-1 g|g> with M^-1 g = g[ib] / max(1, g2kin/eprec[ib])   (linsolve 150-181);
+// M^-1 g itself is formed again by k_direction (one read of g here instead of
+// a read of g and a write and re-read of M^-1 g).
 // V is the vector precision; the inner products are summed in FP64.
 template <typename V>
-__global__ __launch_bounds__(NT) void k_prec_rho(int npw, int gstart, const int* l2i, const cplx<V>* g, cplx<V>* h,
-                                                 long ld, const double* g2kin, const double* eprec, double* rho) {
+__global__ __launch_bounds__(NT) void k_prec_rho(int npw, int gstart, const int* l2i, const cplx<V>* g, long ld,
+                                                 const double* g2kin, const double* eprec, double* rho) {
   __shared__ double red[NT / 64];
   const int lb = blockIdx.x;
   const int ib = l2i[lb];
   const cplx<V>* gi = g + static_cast<long>(ib) * ld;
-  cplx<V>* hi = h + static_cast<long>(ib) * ld;
   const double ep = eprec[ib];
-  double s = 0.0;
+  double s = 0.0, r0 = 0.0;
   for (int q = threadIdx.x; q < npw; q += NT) {
     const V f = V(1.0 / fmax(1.0, g2kin[q] / ep));
     const cplx<V> gv = gi[q];
     const cplx<V> hv = mk<V>(gv.x * f, gv.y * f);
-    hi[q] = hv;
     s += static_cast<double>(hv.x) * gv.x + static_cast<double>(hv.y) * gv.y;
+    if (q == 0) r0 = static_cast<double>(hv.x) * gv.x;
   }
   s = block_sum(s, red);
-  if (threadIdx.x == 0) {
-    double r = 2.0 * s;
-    if (gstart == 2) r -= static_cast<double>(hi[0].x) * gi[0].x;
-    rho[lb] = r;
-  }
+  r0 = block_sum(r0, red);
+  if (threadIdx.x == 0) rho[lb] = 2.0 * s - (gstart == 2 ? r0 : 0.0);
 }
 
-// h[ib] = -h[ib] + beta[lb] hold[ib];  bc[lb] = h[ib]   (linsolve 226-240)
+// h[ib] = -M^-1 g[ib] + beta[lb] hold[ib];  bc[lb] = h[ib]   (linsolve 150-157, 226-240)
 template <typename V>
-__global__ __launch_bounds__(NT) void k_direction(int npw, const int* l2i, const double* beta, cplx<V>* h,
+__global__ __launch_bounds__(NT) void k_direction(int npw, const int* l2i, const double* beta, const cplx<V>* g,
+                                                  const double* g2kin, const double* eprec, cplx<V>* h,
                                                   const cplx<V>* hold, cplx<V>* bc, long ld, long ldc) {
   const int lb = blockIdx.y;
   const int ib = l2i[lb];
@@ -67,8 +68,9 @@ __global__ __launch_bounds__(NT) void k_direction(int npw, const int* l2i, const
   if (q >= npw) return;
   const V bt = V(beta[lb]);
   const long o = static_cast<long>(ib) * ld + q;
-  cplx<V> v = h[o];
-  v = mk<V>(-v.x, -v.y);
+  const V f = V(1.0 / fmax(1.0, g2kin[q] / eprec[ib]));
+  const cplx<V> gv = g[o];
+  cplx<V> v = mk<V>(-(gv.x * f), -(gv.y * f));
   if (bt != V(0)) {
     const cplx<V> w = hold[o];
     v = mk<V>(v.x + bt * w.x, v.y + bt * w.y);
@@ -116,6 +118,58 @@ __global__ __launch_bounds__(NT) void k_cg_update(int npw, int gstart, const int
     cplx<V> gv = gi[q];
     xi[q] = mk<V>(xv.x - step * hv.x, xv.y - step * hv.y);
     gi[q] = mk<V>(gv.x - step * tv.x, gv.y - step * tv.y);
+  }
+}
+
+// The same update with the band's h and t (and g when CG_ is set) held in
+// registers between the inner products and the update (one read of each
+// instead of two): 1024 threads, up to E elements each.
+constexpr int NTU = 1024;
+template <typename V, int E, bool CG_>
+__global__ __launch_bounds__(NTU) void k_cg_update_reg(int npw, int gstart, const int* l2i, const cplx<V>* h,
+                                                      cplx<V>* g, const cplx<V>* tc, cplx<V>* x, long ld, long ldc) {
+  __shared__ double red[NTU / 64];
+  const int lb = blockIdx.x;
+  const int ib = l2i[lb];
+  const cplx<V>* hi = h + static_cast<long>(ib) * ld;
+  cplx<V>* gi = g + static_cast<long>(ib) * ld;
+  const cplx<V>* ti = tc + static_cast<long>(lb) * ldc;
+  cplx<V>* xi = x + static_cast<long>(ib) * ld;
+  cplx<V> hv[E], gv[CG_ ? E : 1], tv[E];
+  double sa = 0.0, sc = 0.0;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int q = threadIdx.x + e * NTU;
+    if (q < npw) {
+      hv[e] = hi[q];
+      const cplx<V> gq = gi[q];
+      if constexpr (CG_) gv[e] = gq;
+      tv[e] = ti[q];
+      sa += static_cast<double>(hv[e].x) * gq.x + static_cast<double>(hv[e].y) * gq.y;
+      sc += static_cast<double>(hv[e].x) * tv[e].x + static_cast<double>(hv[e].y) * tv[e].y;
+    }
+  }
+  sa = block_sum<NTU>(sa, red);
+  sc = block_sum<NTU>(sc, red);
+  double a = 2.0 * sa, c = 2.0 * sc;
+  if (gstart == 2) {
+    // element 0 belongs to thread 0; the block sums above end with a barrier
+    a -= static_cast<double>(hi[0].x) * gi[0].x;
+    c -= static_cast<double>(hi[0].x) * ti[0].x;
+  }
+  __syncthreads();   // every thread has read gi[0] before it is updated
+  const V step = V(a / c);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int q = threadIdx.x + e * NTU;
+    if (q < npw) {
+      const cplx<V> xv = xi[q];
+      cplx<V> gq;
+      if constexpr (CG_) gq = gv[e];
+      else gq = gi[q];
+      xi[q] = mk<V>(xv.x - step * hv[e].x, xv.y - step * hv[e].y);
+      gi[q] = mk<V>(gq.x - step * tv[e].x, gq.y - step * tv[e].y);
+    }
   }
 }
 
@@ -282,46 +336,59 @@ __global__ __launch_bounds__(NT) void k_eprec(int npw, const z64* evc, long ld, 
 
 }  // namespace
 
-void prec_rho(int nc, int npw, int gstart, const int* l2i, const z64* g, z64* h, long ld, const double* g2kin,
+void prec_rho(int nc, int npw, int gstart, const int* l2i, const z64* g, long ld, const double* g2kin,
               const double* eprec, double* rho, hipStream_t s) {
   if (nc <= 0) return;
-  k_prec_rho<double><<<nc, NT, 0, s>>>(npw, gstart, l2i, g, h, ld, g2kin, eprec, rho);
+  k_prec_rho<double><<<nc, NT, 0, s>>>(npw, gstart, l2i, g, ld, g2kin, eprec, rho);
   WH_HIP(hipGetLastError());
 }
 
-void prec_rho(int nc, int npw, int gstart, const int* l2i, const z32* g, z32* h, long ld, const double* g2kin,
+void prec_rho(int nc, int npw, int gstart, const int* l2i, const z32* g, long ld, const double* g2kin,
               const double* eprec, double* rho, hipStream_t s) {
   if (nc <= 0) return;
-  k_prec_rho<float><<<nc, NT, 0, s>>>(npw, gstart, l2i, g, h, ld, g2kin, eprec, rho);
+  k_prec_rho<float><<<nc, NT, 0, s>>>(npw, gstart, l2i, g, ld, g2kin, eprec, rho);
   WH_HIP(hipGetLastError());
 }
 
-void direction(int nc, int npw, const int* l2i, const double* beta, z64* h, const z64* hold, z64* bc, long ld,
-               hipStream_t s) {
+void direction(int nc, int npw, const int* l2i, const double* beta, const z64* g, const double* g2kin,
+               const double* eprec, z64* h, const z64* hold, z64* bc, long ld, hipStream_t s) {
   if (nc <= 0) return;
-  k_direction<double><<<dim3(ceil_div(npw, NT), nc), NT, 0, s>>>(npw, l2i, beta, h, hold, bc, ld, ld);
+  k_direction<double><<<dim3(ceil_div(npw, NT), nc), NT, 0, s>>>(npw, l2i, beta, g, g2kin, eprec, h, hold, bc, ld,
+                                                                  ld);
   WH_HIP(hipGetLastError());
 }
 
-void direction(int nc, int npw, const int* l2i, const double* beta, z32* h, const z32* hold, z32* bc, long ld,
-               hipStream_t s) {
+void direction(int nc, int npw, const int* l2i, const double* beta, const z32* g, const double* g2kin,
+               const double* eprec, z32* h, const z32* hold, z32* bc, long ld, hipStream_t s) {
   if (nc <= 0) return;
-  k_direction<float><<<dim3(ceil_div(npw, NT), nc), NT, 0, s>>>(npw, l2i, beta, h, hold, bc, ld, ld);
+  k_direction<float><<<dim3(ceil_div(npw, NT), nc), NT, 0, s>>>(npw, l2i, beta, g, g2kin, eprec, h, hold, bc, ld,
+                                                                 ld);
+  WH_HIP(hipGetLastError());
+}
+
+template <typename V>
+static void cg_update_any(int nc, int npw, int gstart, const int* l2i, const cplx<V>* h, cplx<V>* g,
+                          const cplx<V>* tc, cplx<V>* x, long ld, double* ac_out, hipStream_t s) {
+  if (nc <= 0) return;
+  // FP64 at 9 elements per thread keeps h and t only (g too would spill)
+  if (!ac_out && npw <= 4 * NTU) {
+    k_cg_update_reg<V, 4, true><<<nc, NTU, 0, s>>>(npw, gstart, l2i, h, g, tc, x, ld, ld);
+  } else if (!ac_out && npw <= 9 * NTU) {
+    k_cg_update_reg<V, 9, (sizeof(V) == 4)><<<nc, NTU, 0, s>>>(npw, gstart, l2i, h, g, tc, x, ld, ld);
+  } else {
+    k_cg_update<V><<<nc, NT, 0, s>>>(npw, gstart, l2i, h, g, tc, x, ld, ld, ac_out);
+  }
   WH_HIP(hipGetLastError());
 }
 
 void cg_update(int nc, int npw, int gstart, const int* l2i, const z64* h, z64* g, const z64* tc, z64* x, long ld,
                double* ac_out, hipStream_t s) {
-  if (nc <= 0) return;
-  k_cg_update<double><<<nc, NT, 0, s>>>(npw, gstart, l2i, h, g, tc, x, ld, ld, ac_out);
-  WH_HIP(hipGetLastError());
+  cg_update_any<double>(nc, npw, gstart, l2i, h, g, tc, x, ld, ac_out, s);
 }
 
 void cg_update(int nc, int npw, int gstart, const int* l2i, const z32* h, z32* g, const z32* tc, z32* x, long ld,
                double* ac_out, hipStream_t s) {
-  if (nc <= 0) return;
-  k_cg_update<float><<<nc, NT, 0, s>>>(npw, gstart, l2i, h, g, tc, x, ld, ld, ac_out);
-  WH_HIP(hipGetLastError());
+  cg_update_any<float>(nc, npw, gstart, l2i, h, g, tc, x, ld, ac_out, s);
 }
 
 void gather_sticks(int m, int npw, const int* perm, const z64* in, long ldin, z64* out, long ldout, const double* w,
