@@ -36,9 +36,10 @@ namespace wh {
 // about twice as fast as the kernel below for the ket shape (M = 2 npw,
 // K = nq) on MI355X. The bra products (small M = nq, long K = 2 npw, fused
 // G = 0 correction and row scale) use whichever of the split-K kernel of this
-// file and rocBLAS + k_g0_scale was faster when first timed for that (M, K)
-// and the power-of-two bucket of N, for N >= 512 columns (at 8192 columns the
-// kernel won at nq = 384, rocBLAS at nq = 1296); smaller calls use the kernel.
+// file (three K-split depths) and rocBLAS + k_g0_scale was fastest when first
+// timed for that (M, K) and the power-of-two bucket of N, for N >= 512 columns
+// (at 8192 columns the kernel won at nq = 384, rocBLAS at nq = 1296); smaller
+// calls use the kernel.
 // WH_GEMM=own or WH_GEMM=rocblas puts every product on one backend.
 enum class Backend { Default, Own, Rocblas };
 static Backend backend() {
@@ -282,11 +283,12 @@ void dgemm(bool transa, int M, int N, int K, double alpha, const double* A, long
   WH_HIP(hipGetLastError());
 }
 
+// wg_target: workgroups the K split aims at (512: two per compute unit)
 static void dgemm_bra_own(int M, int N, int K, double alpha, const double* A, long lda, const double* B, long ldb,
-                          double g0, const double* scale, double* C, long ldc, hipStream_t s) {
+                          double g0, const double* scale, double* C, long ldc, hipStream_t s, int wg_target = 512) {
   const int tiles = ceil_div(M, BM) * ceil_div(N, BN);
-  // enough workgroups for 256 compute units, each K chunk at least 64 tiles long
-  int split = std::max(1, std::min(16, ceil_div(512, tiles)));
+  // enough workgroups for the compute units, each K chunk at least 64 tiles long
+  int split = std::max(1, std::min(16, ceil_div(wg_target, tiles)));
   split = std::max(1, std::min(split, K / (64 * BK)));
   const int kchunk = ceil_div(ceil_div(std::max(K, 1), split), BK) * BK;
   split = ceil_div(std::max(K, 1), kchunk);
@@ -315,35 +317,41 @@ void dgemm_bra(int M, int N, int K, double alpha, const double* A, long lda, con
   const Backend be = backend();
   if (be == Backend::Rocblas && K > 0) return dgemm_bra_lib(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s);
   if (be == Backend::Own || K <= 0) return dgemm_bra_own(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s);
-  static std::map<std::tuple<int, int, int>, int> chosen;   // (M, K, log2 N) -> 0 kernel, 1 rocBLAS
+  // variants: 0..2 the kernel with its K split aimed at 512, 1024, 2048 workgroups; 3 rocBLAS
+  static std::map<std::tuple<int, int, int>, int> chosen;   // (M, K, log2 N) -> variant
   int lg = 0;
   while ((2 << lg) <= N) ++lg;
   const auto key = std::make_tuple(M, K, lg);
+  auto run = [&](int v) {
+    if (v == 3) dgemm_bra_lib(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s);
+    else dgemm_bra_own(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s, 512 << v);
+  };
   auto it = chosen.find(key);
   if (it == chosen.end() && N >= 512) {
-    // time both on this call's arguments
+    // time every variant on this call's arguments (a first round warms them up)
     hipEvent_t a, b;
     WH_HIP(hipEventCreate(&a));
     WH_HIP(hipEventCreate(&b));
-    float t[2] = {0.f, 0.f};
+    float t[4] = {0.f, 0.f, 0.f, 0.f};
     for (int rep = 0; rep < 2; ++rep)
-      for (int v = 0; v < 2; ++v) {
+      for (int v = 0; v < 4; ++v) {
         WH_HIP(hipEventRecord(a, s));
-        if (v == 0) dgemm_bra_own(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s);
-        else dgemm_bra_lib(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s);
+        run(v);
         WH_HIP(hipEventRecord(b, s));
         WH_HIP(hipEventSynchronize(b));
         float ms = 0.f;
         WH_HIP(hipEventElapsedTime(&ms, a, b));
-        if (rep == 1) t[v] = ms;   // the first round warms both up
+        if (rep == 1) t[v] = ms;
       }
     WH_HIP(hipEventDestroy(a));
     WH_HIP(hipEventDestroy(b));
-    it = chosen.emplace(key, t[1] < t[0] ? 1 : 0).first;
-    // recompute with the chosen backend so the result does not depend on the order above
+    int best = 0;
+    for (int v = 1; v < 4; ++v)
+      if (t[v] < t[best]) best = v;
+    it = chosen.emplace(key, best).first;
   }
-  if (it != chosen.end() && it->second == 1) return dgemm_bra_lib(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s);
-  dgemm_bra_own(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s);
+  // recomputed with the chosen variant, so the result does not depend on the timing order
+  run(it != chosen.end() ? it->second : 0);
 }
 
 void g0_fix_scale(int M, int N, double g0, const double* A, long lda, const double* B, long ldb,
