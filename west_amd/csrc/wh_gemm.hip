@@ -134,7 +134,7 @@ __device__ __forceinline__ void store_b(double (*Bs)[BN + PAD], const double (&r
 // blockIdx.z = split index; k range [kz*kchunk, min(K, (kz+1)*kchunk)).
 // split > 1: raw partial sums go to C + kz*pstride (alpha, beta ignored).
 template <bool TA>
-__global__ __launch_bounds__(NT) void k_dgemm(int M, int N, int K, int kchunk, double alpha,
+__global__ __launch_bounds__(NT, 2) void k_dgemm(int M, int N, int K, int kchunk, double alpha,
                                               const double* __restrict__ A, long lda, const double* __restrict__ B,
                                               long ldb, double beta, double* __restrict__ C, long ldc, long pstride) {
   __shared__ double As[2][BK][BM + PAD];
