@@ -213,26 +213,39 @@ constexpr RadixPlan make_plan(int N) {
   return p;
 }
 
-template <int N>
+// REV: the radices in reverse order. A forward transform with the reversed
+// plan can start from the output of an inverse transform's last pass without
+// an LDS round trip: that pass leaves butterfly i holding positions
+// i + j N/R, exactly what butterfly i of the reversed plan's first pass reads.
+template <int N, bool REV = false>
 struct Plan {
   static constexpr RadixPlan P = make_plan(N);
   static_assert(P.n > 0, "N must factor into 2, 3, 5");
   static constexpr int nst = P.n;
-  static constexpr int radix(int s) { return P.r[s]; }
+  static constexpr int radix(int s) { return REV ? P.r[P.n - 1 - s] : P.r[s]; }
   static constexpr int stride(int s) {   // product of the radices of passes before s
     int p = 1;
-    for (int i = 0; i < s; ++i) p *= P.r[i];
+    for (int i = 0; i < s; ++i) p *= radix(i);
     return p;
   }
-  // twiddles of pass s >= 1 are stored [k][j] (k < stride(s), 1 <= j < radix(s))
-  // from twoff(s): the R - 1 roots one butterfly needs are adjacent, and
-  // neighbouring butterflies read 16 (R - 1) bytes apart (no LDS bank clashes)
-  static constexpr int twoff(int s) {
+  static constexpr int twtotal() {
     int o = 0;
+    for (int i = 1; i < nst; ++i) o += stride(i) * (radix(i) - 1);
+    return o;
+  }
+  // twiddles of pass s >= 1 are stored [k][j] (k < stride(s), 1 <= j < radix(s))
+  // from twoff(s) (the reversed plan's tables follow the forward plan's): the
+  // R - 1 roots one butterfly needs are adjacent, and neighbouring butterflies
+  // read 16 (R - 1) bytes apart (no LDS bank clashes)
+  static constexpr int twoff(int s) {
+    int o = REV ? Plan<N, false>::twtotal() : 0;
     for (int i = 1; i < s; ++i) o += stride(i) * (radix(i) - 1);
     return o;
   }
-  static constexpr int twsize() { return twoff(nst) > 0 ? twoff(nst) : 1; }
+  static constexpr int twsize() {
+    const int t = Plan<N, false>::twtotal() + Plan<N, true>::twtotal();
+    return t > 0 ? t : 1;
+  }
 };
 
 // ---------------------------------------------------------------------------
@@ -240,10 +253,11 @@ struct Plan {
 // stores through st(line, idx, value); a barrier separates the two phases and
 // follows the stores.  MAXL bounds nlines (register staging).
 // ---------------------------------------------------------------------------
-template <typename T, int N, int S, int MAXL, int BT, int DIR, class Ld, class St>
+template <typename T, int N, int S, int MAXL, int BT, int DIR, bool REV = false, class Ld, class St>
 __device__ __forceinline__ void pass(int nlines, const cplx<T>* tw, Ld ld, St st) {
-  constexpr int R = Plan<N>::radix(S);
-  constexpr int P = Plan<N>::stride(S);
+  using PL = Plan<N, REV>;
+  constexpr int R = PL::radix(S);
+  constexpr int P = PL::stride(S);
   constexpr int NBL = N / R;
   constexpr int NB = (MAXL * NBL + BT - 1) / BT;
   const int nbf = nlines * NBL;
@@ -267,7 +281,7 @@ __device__ __forceinline__ void pass(int nlines, const cplx<T>* tw, Ld ld, St st
       const int i = bf - line * NBL;
       const int k = (P == 1) ? 0 : i % P;
       if (P > 1 && k != 0) {
-        const cplx<T>* wk = tw + Plan<N>::twoff(S) + k * (R - 1);
+        const cplx<T>* wk = tw + PL::twoff(S) + k * (R - 1);
 #pragma unroll
         for (int j = 1; j < R; ++j) {
           cplx<T> w = wk[j - 1];
@@ -313,10 +327,11 @@ __device__ __forceinline__ void fft_inplace(int nlines, cplx<T>* buf, const cplx
   passes<T, N, 0, MAXL, BT, DIR, decltype(in_buf), decltype(to_buf), LS>(nlines, buf, tw, in_buf, to_buf);
 }
 
-// pass twiddle tables (Plan<N>::twoff) from the forward roots g[m] = exp(-2 pi i m / N)
-template <typename T, int N>
-__device__ __forceinline__ void load_tw(cplx<T>* tw, const double* g) {
-  using PL = Plan<N>;
+// pass twiddle tables of both plans (Plan<N, REV>::twoff) from the forward
+// roots g[m] = exp(-2 pi i m / N)
+template <typename T, int N, bool REV>
+__device__ __forceinline__ void load_tw_plan(cplx<T>* tw, const double* g) {
+  using PL = Plan<N, REV>;
 #pragma unroll
   for (int s = 1; s < PL::nst; ++s) {
     const int R = PL::radix(s);
@@ -329,6 +344,66 @@ __device__ __forceinline__ void load_tw(cplx<T>* tw, const double* g) {
       tw[PL::twoff(s) + t] = mk<T>(T(g[2 * m]), T(g[2 * m + 1]));
     }
   }
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void load_tw(cplx<T>* tw, const double* g) {
+  load_tw_plan<T, N, false>(tw, g);
+  load_tw_plan<T, N, true>(tw, g);
+}
+
+// The turn between an inverse transform (forward-order plan) and a forward
+// transform (reversed plan) along the same lines: the inverse's last pass,
+// the pointwise op mid(line, position, value), and the reversed plan's first
+// pass, in registers.  Loads through ld like a pass; stores pass-0 outputs of
+// the reversed plan through st.
+template <typename T, int N, int MAXL, int BT, class Ld, class Mid, class St>
+__device__ __forceinline__ void pass_turn(int nlines, const cplx<T>* tw, Ld ld, Mid mid, St st) {
+  using PF = Plan<N, false>;
+  constexpr int S = PF::nst - 1;
+  constexpr int R = PF::radix(S);
+  constexpr int P = PF::stride(S);
+  constexpr int NBL = N / R;
+  static_assert(NBL == P, "last pass: one butterfly per residue");
+  static_assert(Plan<N, true>::radix(0) == R, "reversed plan starts with the last radix");
+  constexpr int NB = (MAXL * NBL + BT - 1) / BT;
+  const int nbf = nlines * NBL;
+  cplx<T> v[NB][R];
+#pragma unroll
+  for (int t = 0; t < NB; ++t) {
+    const int bf = threadIdx.x + t * BT;
+    if (bf < nbf) {
+      const int line = bf / NBL;
+      const int i = bf - line * NBL;
+#pragma unroll
+      for (int j = 0; j < R; ++j) v[t][j] = ld(line, i + j * NBL);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < NB; ++t) {
+    const int bf = threadIdx.x + t * BT;
+    if (bf < nbf) {
+      const int line = bf / NBL;
+      const int i = bf - line * NBL;   // = k of the last pass (i < P)
+      if (P > 1 && i != 0) {
+        const cplx<T>* wk = tw + PF::twoff(S) + i * (R - 1);
+#pragma unroll
+        for (int j = 1; j < R; ++j) {
+          cplx<T> w = wk[j - 1];
+          w.y = -w.y;   // inverse: exp(+i ...)
+          v[t][j] = cmul(v[t][j], w);
+        }
+      }
+      dft<T, R, +1>(v[t]);
+#pragma unroll
+      for (int j = 0; j < R; ++j) v[t][j] = mid(line, i + j * P, v[t][j]);
+      dft<T, R, -1>(v[t]);
+#pragma unroll
+      for (int j = 0; j < R; ++j) st(line, i * R + j, v[t][j]);
+    }
+  }
+  __syncthreads();
 }
 
 // threads of the per-plane kernels (two waves per SIMD)
@@ -355,10 +430,32 @@ constexpr int x_budget(bool density, int occ) {
                     (density ? static_cast<int>(sizeof(T)) * N * N : 0);
   return (budget - fixed) / (static_cast<int>(sizeof(cplx<T>)) * N);
 }
+// x lines of the potential kernel: one element of padding after every 8, so
+// the stride-8 stores of a radix-8 pass spread over the LDS banks
+template <int N>
+constexpr int xline() {
+  return N + N / 8 + 1;
+}
+__device__ __forceinline__ int xpad(int x) { return x + (x >> 3); }
+
+template <typename T, int N>
+constexpr int vloc_budget(int occ) {
+  const int budget = 160 * 1024 / occ - 2048;
+  const int fixed = static_cast<int>(sizeof(cplx<T>)) * (Plan<N>::twsize() + max_rows<N>() * prow<N>());
+  return (budget - fixed) / (static_cast<int>(sizeof(cplx<T>)) * xline<N>());
+}
+template <typename T, int N>
+constexpr int vloc_chunk(int occ) {
+  int jc = vloc_budget<T, N>(occ);
+  if (jc > N) jc = N;
+  if (jc < 1) return 1;
+  const int nchunk = (N + jc - 1) / jc;
+  return (N + nchunk - 1) / nchunk;
+}
 // occ workgroups per CU leave room for at least 4 x-lines per chunk
 template <typename T, int N>
 constexpr bool occ_fits(int occ) {
-  return occ == 1 || x_budget<T, N>(false, occ) >= 4;
+  return occ == 1 || vloc_budget<T, N>(occ) >= 4;
 }
 template <typename T, int N>
 constexpr int x_chunk(bool density, int occ = 1) {
@@ -551,7 +648,8 @@ __device__ __forceinline__ int row_of(int x, int N, int imax) {
 template <typename T, int N, typename PT, int OCC>
 __global__ __launch_bounds__(NTP) __attribute__((amdgpu_waves_per_eu(2 * OCC))) void k_plane_vloc_t(StickPlanDev P, const FftItem* items, const cplx<T>* sin,
                                                      cplx<T>* sout, const PT* vpot, long ldpot, double scale) {
-  constexpr int JC = x_chunk<T, N>(false, OCC);
+  constexpr int JC = vloc_chunk<T, N>(OCC);
+  constexpr int XL = xline<N>();
   constexpr int MR = max_rows<N>();
   PlaneLds<T, N> S = plane_lds<T, N>();
   const int z = blockIdx.x;
@@ -573,17 +671,19 @@ __global__ __launch_bounds__(NTP) __attribute__((amdgpu_waves_per_eu(2 * OCC))) 
       const int r = row_of(x, N, imax);
       return r >= 0 ? pl[r * prow<N>() + j0 + jj] : mk<T>(T(0), T(0));
     };
-    auto times_v = [=](int jj, int x, cplx<T> v) {
-      xb[jj * N + x] = cscale(v, T(V[(j0 + jj) * N + x]) * sc);
-    };
-    passes<T, N, 0, JC, NTP, +1>(nj, xb, S.tw, from_rows, times_v);
-    // forward along a1: only |i| <= imax frequencies go back to the rows
-    auto from_x = [=](int jj, int x) { return xb[jj * N + x]; };
+    auto in_x = [=](int jj, int x) { return xb[jj * XL + xpad(x)]; };
+    auto to_x = [=](int jj, int x, cplx<T> v) { xb[jj * XL + xpad(x)] = v; };
+    auto times_v = [=](int jj, int x, cplx<T> v) { return cscale(v, T(V[(j0 + jj) * N + x]) * sc); };
+    // forward along a1 (reversed radix order): only |i| <= imax frequencies go back to the rows
     auto to_rows = [=](int jj, int x, cplx<T> v) {
       const int r = row_of(x, N, imax);
       if (r >= 0) pl[r * prow<N>() + j0 + jj] = v;
     };
-    passes<T, N, 0, JC, NTP, -1>(nj, xb, S.tw, from_x, to_rows);
+    static_assert(Plan<N>::nst == 2, "two-pass plans");
+    pass<T, N, 0, JC, NTP, +1>(nj, S.tw, from_rows, to_x);
+    // inverse's last pass, V(r), forward's first pass in registers
+    pass_turn<T, N, JC, NTP>(nj, S.tw, in_x, times_v, to_x);
+    pass<T, N, 1, JC, NTP, -1, true>(nj, S.tw, in_x, to_rows);
   }
   fft_inplace<T, N, MR, NTP, -1, prow<N>()>(P.nrows, S.pl, S.tw);
   cplx<T>* dst = sout + static_cast<long>(it) * sl + static_cast<long>(z) * P.nct;
@@ -780,10 +880,17 @@ void allow_smem(K kernel, size_t bytes) {
   done[key] = bytes;
 }
 
+template <typename T, int N, int OCC>
+constexpr size_t vloc_smem() {
+  return sizeof(cplx<T>) * (static_cast<size_t>(Plan<N>::twsize()) + static_cast<size_t>(max_rows<N>()) * prow<N>() +
+                            static_cast<size_t>(vloc_chunk<T, N>(OCC)) * xline<N>());
+}
+
 template <typename T, int N, typename PT, int OCC>
 void vloc_occ(const StickPlanDev& P, const FftItem* it, int n, const void* si, void* so, const PT* v, long ldv,
               double scale, hipStream_t s) {
-  const size_t sm = plane_smem<T, N>(false, OCC);
+  constexpr size_t sm = vloc_smem<T, N, OCC>();
+  static_assert(sm <= 160u * 1024u, "plane kernel exceeds LDS");
   allow_smem(k_plane_vloc_t<T, N, PT, OCC>, sm);
   k_plane_vloc_t<T, N, PT, OCC><<<dim3(N, n), NTP, sm, s>>>(P, it, static_cast<const cplx<T>*>(si),
                                                             static_cast<cplx<T>*>(so), v, ldv, scale);
