@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 # Measured HBM bytes per transform item (band pair) of the plane kernels on the
 # si64 workload: rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a streaming kernel
 # of known size (profiles/plane_vloc_si64_hbm_traffic.md).
-TRAFFIC_PER_ITEM = {"si64": {"plane_vloc": 985477 + 974592, "plane_vloc32": 497264 + 488448}}
+TRAFFIC_PER_ITEM = {"si64": {"plane_vloc": 985465 + 974592, "plane_vloc32": 499557 + 564293}}
 FP64_PEAK_TFLOPS = 78.6
 
 WORKLOADS = {
