@@ -40,6 +40,8 @@ WORKLOADS = {
     "si8": ({"nrep": 1, "a": 10.26, "ecutwfc": 25.0, "sigma": 0.05, "seed": 2024}, 16),
     # largest single-GPU workload: 3x3x3 cell, 216 atoms, 432 bands, 100^3, 128 perturbations per batch
     "si216": ({"nrep": 3, "a": 10.26, "ecutwfc": 25.0, "sigma": 0.05, "seed": 2024}, 128),
+    # batch of 24 uneven H2 clusters (2-24 molecules, 12-24 bohr cells, 16-64 PDEP each): Davidson wall time
+    "c4": (None, None),
 }
 
 
@@ -106,6 +108,10 @@ def main():
         torch.cuda.synchronize()
         del a, b
         log("[bench] calibration kernel: neg over 2^28 float32 (1 GiB read + 1 GiB written)")
+
+    if args.workload == "c4":
+        _batch_c4(wstat, torch, args)
+        return
 
     kw, npert = WORKLOADS[args.workload]
     t_setup = time.perf_counter()
@@ -220,6 +226,80 @@ def main():
     print(json.dumps(rec), flush=True)
 
 
+def _batch_c4(wstat, torch, args):
+    """24 independent H2 clusters of uneven size (seeded: 2-24 molecules in
+    12-24 bohr cubic cells, 25 Ry, 16-64 PDEP each, tr2_dfpt 1e-12): Wstat
+    Davidson on all of them, one after the other and concurrently (one host
+    thread and HIP stream per system), in FP64 and in mixed precision."""
+    from west_amd.coulomb import sqvc_init
+    from west_amd.groundstate import make_ground_state
+    from west_amd.systems import make_h2_clusters
+    clusters = make_h2_clusters(nsys=24, seed=2024)
+    wstat.init(0)
+    handles = []
+    t = time.perf_counter()
+    for k, (s, npdep) in enumerate(clusters):
+        h = 0 if k == 0 else wstat.system_create()
+        wstat.system_select(h)
+        gs = make_ground_state(s, wstat.gpu_h_factory, nbnd=s.nbnd_occ + 4, scf_steps=args.scf, tol=1e-7)
+        wstat.set_ground_state(gs, sqvc=sqvc_init(s.basis, gs.npwx))
+        handles.append(h)
+        log(f"[bench] {s.name}: nr={s.basis.nr[0]} npw={s.basis.npw} bands={gs.nbndval} pdep={npdep}")
+    wstat.system_select(0)
+    log(f"[bench] 24 ground states in {time.perf_counter() - t:.1f}s")
+    jobs = [(h, {"n_pdep_eigen": npdep, "n_pdep_times": 4, "tr2_dfpt": 1e-12, "want_vectors": False})
+            for h, (s, npdep) in zip(handles, clusters)]
+    res = {}
+    evs = {}
+    for mode in ("fp64", args.mixed_mode):
+        for h in handles:
+            wstat.system_select(h)
+            wstat.set_precision(mode)
+        wstat.system_select(0)
+        wstat.davidson_batch(jobs, concurrent=True)   # warm-up: kernel loads, occupancy and GEMM timing
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        seq = wstat.davidson_batch(jobs, concurrent=False)
+        t_seq = time.perf_counter() - t
+        t = time.perf_counter()
+        con = wstat.davidson_batch(jobs, concurrent=True)
+        t_con = time.perf_counter() - t
+        dseq = max(float(np.max(np.abs(a[0] - b[0]) / np.abs(a[0]))) for a, b in zip(seq, con))
+        key = "fp64" if mode == "fp64" else "mixed"
+        evs[key] = [r[0] for r in con]
+        res[key] = {"wall_s_sequential": round(t_seq, 3), "wall_s_concurrent": round(t_con, 3),
+                    "speedup_concurrent": round(t_seq / t_con, 3),
+                    "converged": int(sum(int(r[1].sum()) for r in con)),
+                    "davidson_iterations": [len(r[3]) for r in con],
+                    "ev_max_rel_diff_sequential_vs_concurrent": dseq}
+        log(f"[bench] c4 {mode}: sequential {t_seq:.2f}s, concurrent {t_con:.2f}s")
+    for h in handles:
+        wstat.system_select(h)
+        wstat.set_precision("fp64")
+    wstat.system_select(0)
+    # reference eigenvalues: FP64 with tr2_dfpt 1e-20, so that both runs' errors show
+    tight = wstat.davidson_batch([(h, dict(kw, tr2_dfpt=1e-20)) for h, kw in jobs], concurrent=True)
+    evt = [r[0] for r in tight]
+
+    def rel(a, b):
+        return float(np.max(np.abs(a - b) / np.abs(b)))
+
+    err = max(rel(a, b) for a, b in zip(evs["mixed"], evs["fp64"]))
+    res["fp64"]["pdep_ev_max_rel_err_vs_tight_fp64"] = max(rel(a, b) for a, b in zip(evs["fp64"], evt))
+    res["mixed"]["pdep_ev_max_rel_err_vs_tight_fp64"] = max(rel(a, b) for a, b in zip(evs["mixed"], evt))
+    npdep_total = sum(n for _, n in clusters)
+    rec = {"metric": "Wstat Davidson wall time, batch of 24 uneven H2 clusters", "value": res["fp64"]["wall_s_concurrent"],
+           "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "f64", "data": "synthetic (seeded clusters)",
+           "config": {"workload": "c4: 24 H2 clusters, 2-24 molecules, 12-24 bohr cells, 25 Ry, 16-64 PDEP each, "
+                                  "tr2_dfpt 1e-12, n_pdep_times 4",
+                      "systems": [{"name": s.name, "nr": s.basis.nr[0], "npw": s.basis.npw, "bands": s.nbnd_occ,
+                                   "pdep": n} for s, n in clusters],
+                      "pdep_total": npdep_total},
+           "fp64": res["fp64"], "mixed_precision": dict(res["mixed"], mode=args.mixed_mode,
+                                                         pdep_ev_max_rel_err_vs_fp64=err)}
+    print(json.dumps(rec), flush=True)
+
+
 def _davidson_h2(wstat, torch, args):
     """Wstat Davidson on 6 H2 molecules (bond 1.40 bohr, random orientations,
     >= 5 bohr apart, 18 bohr cell, 25 Ry, 6 occupied bands): 32 PDEP,
@@ -259,7 +339,7 @@ def _davidson_h2(wstat, torch, args):
 def _run_mode(wstat, torch, args, mode, npert, dvg, dng):
     """Warmup + timed chi0 batches in one precision mode; per-step counters and kernel times."""
     wstat.set_precision(mode, args.refine_delta if mode != "fp64" else 0.0)
-    nv = wstat._state["gs"].nbndval
+    nv = wstat.current_ground_state().nbndval
     pairs = npert * nv
     for i in range(args.warmup):
         s0 = wstat.counters()

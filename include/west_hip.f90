@@ -86,6 +86,21 @@ MODULE west_hip
        INTEGER(C_INT), VALUE :: n_dfpt_maxiter, l_kinetic_only
      END FUNCTION
      !
+     INTEGER(C_INT) FUNCTION wh_system_create(handle) BIND(C, NAME="wh_system_create")
+       IMPORT :: C_INT
+       INTEGER(C_INT), INTENT(OUT) :: handle
+     END FUNCTION
+     !
+     INTEGER(C_INT) FUNCTION wh_system_select(handle) BIND(C, NAME="wh_system_select")
+       IMPORT :: C_INT
+       INTEGER(C_INT), VALUE :: handle
+     END FUNCTION
+     !
+     INTEGER(C_INT) FUNCTION wh_system_destroy(handle) BIND(C, NAME="wh_system_destroy")
+       IMPORT :: C_INT
+       INTEGER(C_INT), VALUE :: handle
+     END FUNCTION
+     !
      INTEGER(C_INT) FUNCTION wh_set_precision(mode, refine_delta) BIND(C, NAME="wh_set_precision")
        IMPORT :: C_INT, C_DOUBLE
        INTEGER(C_INT), VALUE :: mode

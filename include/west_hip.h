@@ -94,8 +94,22 @@ int wh_set_params(int n_dfpt_maxiter, int l_kinetic_only);
  * The reference has no such mode; call surfaces and convergence test are
  * unchanged. Modes 1-2
  * need a cubic FFT grid of a size with compiled kernels (18, 20, 24, 30, 32,
- * 36, 40, 45, 48, 54, 60, 64, 72, 80, 90, 96, 100, 108). */
+ * 36, 40, 45, 48, 50, 54, 60, 64, 72, 75, 80, 90, 96, 100, 108). */
 int wh_set_precision(int mode, double refine_delta);
+
+/* ---- several systems ------------------------------------------------- */
+
+/* Independent ground states (systems) in one process on one GPU. wh_init
+ * creates system 0; wh_system_create adds one (own HIP stream, workspaces and
+ * counters) and returns its handle; wh_system_select makes it the target of
+ * every following call from the calling host thread (each thread starts on
+ * system 0). Systems selected in different threads run concurrently. No
+ * counterpart in the reference (WEST runs one system per process); it serves
+ * batches of small, unevenly sized systems that one system alone cannot fill
+ * the GPU with. */
+int wh_system_create(int* handle);
+int wh_system_select(int handle);
+int wh_system_destroy(int handle);
 
 /* ---- the hot path --------------------------------------------------- */
 

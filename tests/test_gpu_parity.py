@@ -220,3 +220,34 @@ def test_dfpt_density_large_grid(gpu):
     ref = dfpt(gs, dvg, -1.0)
     out = gpu.dfpt(3, dvg, -1.0)
     assert _rel(out, ref) < 1e-12
+
+
+def test_systems_concurrent(gpu, gs_h2, gs_3h2, gs_si8):
+    """Three systems registered side by side, chi0 applied from three host
+    threads at once: each matches the oracle, and system 0 keeps its state."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle.dfpt import dfpt, sqvc_init
+    cases = [gs_h2, gs_3h2, gs_si8]
+    handles = [gpu.system_create() for _ in cases]
+    try:
+        for h, gs in zip(handles, cases):
+            gpu.system_select(h)
+            gpu.set_ground_state(gs, sqvc=sqvc_init(gs.basis, gs.npwx))
+        gpu.system_select(0)
+        dvgs = [random_wfc(gs, 3, seed=71 + i) for i, gs in enumerate(cases)]
+
+        def work(i):
+            gpu.system_select(handles[i])
+            return gpu.dfpt(3, dvgs[i], 1e-22)
+
+        with ThreadPoolExecutor(max_workers=3) as ex:
+            outs = list(ex.map(work, range(3)))
+        for gs, dvg, out in zip(cases, dvgs, outs):
+            assert _rel(out, dfpt(gs, dvg, 1e-22)) < 1e-9
+        res = gpu.davidson_batch([(handles[i], {"n_pdep_eigen": 2, "tr2_dfpt": 1e-18, "want_vectors": False})
+                                  for i in range(3)])
+        assert all(len(r[0]) == 2 for r in res)
+    finally:
+        gpu.system_select(0)
+        for h in handles:
+            gpu.system_destroy(h)

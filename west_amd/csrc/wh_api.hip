@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 
 #include "../../include/west_hip.h"
@@ -17,14 +18,60 @@ namespace wh {
 // ---------------------------------------------------------------------------
 // error state
 // ---------------------------------------------------------------------------
-static std::string g_last_error;
+static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
 const char* last_error() { return g_last_error.c_str(); }
 
-static Context* g_ctx = nullptr;
+// ---------------------------------------------------------------------------
+// systems: each registered ground state lives in its own Context (own stream,
+// workspaces, counters); system 0 is created by wh_init. A host thread works
+// on the system it selected last (system 0 by default), so independent
+// systems can run concurrently from several threads on one GPU.
+// ---------------------------------------------------------------------------
+static std::mutex g_sys_mutex;
+static std::vector<Context*> g_sys;   // index = handle; null after destroy
+static int g_device = -1;
+static thread_local int t_current = 0;
+
 Context& ctx() {
-  if (!g_ctx) throw Error{-1, "wh_init has not been called"};
-  return *g_ctx;
+  Context* c = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_sys_mutex);
+    if (g_sys.empty()) throw Error{-1, "wh_init has not been called"};
+    if (t_current >= 0 && t_current < static_cast<int>(g_sys.size())) c = g_sys[t_current];
+  }
+  if (!c) throw Error{-1, "the selected system does not exist"};
+  return *c;
+}
+
+static Context* new_context(int device) {
+  Context* c = new Context();
+  c->device = device;
+  WH_HIP(hipSetDevice(device));
+  WH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  return c;
+}
+
+static void free_context(Context* cp) {
+  Context& c = *cp;
+  (void)hipStreamSynchronize(c.stream);
+  c.prof.flush();
+  for (auto e : c.prof.pool) (void)hipEventDestroy(e);
+  c.plan.release();
+  for (void* p : {static_cast<void*>(c.d_perm), static_cast<void*>(c.d_g2kin), static_cast<void*>(c.d_vrs),
+                  static_cast<void*>(c.d_q), static_cast<void*>(c.d_qscale), static_cast<void*>(c.d_occ),
+                  static_cast<void*>(c.d_eprec), static_cast<void*>(c.d_sqvc)})
+    if (p) (void)hipFree(p);
+  for (DevBuf* b : {&c.evc_sticks, &c.w_items, &c.w_items2, &c.w_items_id, &c.w_sticks_a, &c.w_sticks_b,
+                    &c.w_cmat, &c.w_x, &c.w_t, &c.w_g, &c.w_h, &c.w_hold, &c.w_bc, &c.w_tc, &c.w_b, &c.w_dvr,
+                    &c.w_in, &c.w_out, &c.w_dvin, &c.w_dng, &c.w_l2i, &c.w_beta, &c.w_eu, &c.w_rho, &c.w_ecol,
+                    &c.w_epcol, &c.w_dscr, &c.m_vrs32, &c.m_q32, &c.m_qp[0], &c.m_qp[1], &c.m_qp[2], &c.m_qtp[0],
+                    &c.m_qtp[1], &c.m_qtp[2], &c.m_g, &c.m_h, &c.m_hold, &c.m_dx, &c.m_bc, &c.m_tc, &c.m_cm,
+                    &c.m_work, &c.m_u2i, &c.m_rho64, &c.m_eu64, &c.m_fb, &c.m_fx, &c.m_fe, &c.m_fep})
+    b->release();
+  for (DevBuf& b : c.dav) b.release();
+  (void)hipStreamDestroy(c.stream);
+  delete cp;
 }
 
 void require_ready(bool need_states) {
@@ -275,8 +322,8 @@ static void prepare_mixed() {
   Context& c = ctx();
   if (c.mixed_valid) return;
   WH_REQUIRE(c.plan.tpl32 != nullptr,
-             "mixed precision needs a cubic FFT grid of a compiled size (18, 20, 24, 30, 32, 36, 40, 45, 48, 54, 60, "
-             "64, 72, 80, 90, 96, 100, 108)");
+             "mixed precision needs a cubic FFT grid of a compiled size (18, 20, 24, 30, 32, 36, 40, 45, 48, 50, 54, 60, "
+             "64, 72, 75, 80, 90, 96, 100, 108)");
   hipStream_t s = c.stream;
   c.ld32 = round_up(c.npw, 16);
   const long lda = 2 * c.ld32;
@@ -714,40 +761,58 @@ int wh_init(int device) {
   int n = 0;
   WH_HIP(hipGetDeviceCount(&n));
   WH_REQUIRE(device >= 0 && device < n, "device index out of range");
-  if (g_ctx) {
-    WH_REQUIRE(g_ctx->device == device, "the library is already initialised on another device");
+  std::lock_guard<std::mutex> lk(g_sys_mutex);
+  if (!g_sys.empty()) {
+    WH_REQUIRE(g_device == device, "the library is already initialised on another device");
     return 0;
   }
-  WH_HIP(hipSetDevice(device));
-  g_ctx = new Context();
-  g_ctx->device = device;
-  WH_HIP(hipStreamCreateWithFlags(&g_ctx->stream, hipStreamNonBlocking));
+  g_sys.push_back(new_context(device));
+  g_device = device;
   return 0;
   WH_API_END
 }
 
 int wh_finalize(void) {
   WH_API_BEGIN
-  if (!g_ctx) return 0;
-  Context& c = *g_ctx;
-  (void)hipStreamSynchronize(c.stream);
-  c.prof.flush();
-  for (auto e : c.prof.pool) (void)hipEventDestroy(e);
-  c.plan.release();
-  for (void* p : {static_cast<void*>(c.d_perm), static_cast<void*>(c.d_g2kin), static_cast<void*>(c.d_vrs),
-                  static_cast<void*>(c.d_q), static_cast<void*>(c.d_qscale), static_cast<void*>(c.d_occ),
-                  static_cast<void*>(c.d_eprec), static_cast<void*>(c.d_sqvc)})
-    if (p) (void)hipFree(p);
-  for (DevBuf* b : {&c.evc_sticks, &c.w_items, &c.w_items2, &c.w_items_id, &c.w_sticks_a, &c.w_sticks_b,
-                    &c.w_cmat, &c.w_x, &c.w_t, &c.w_g, &c.w_h, &c.w_hold, &c.w_bc, &c.w_tc, &c.w_b, &c.w_dvr,
-                    &c.w_in, &c.w_out, &c.w_dvin, &c.w_dng, &c.w_l2i, &c.w_beta, &c.w_eu, &c.w_rho, &c.w_ecol, &c.w_epcol, &c.w_dscr,
-                    &c.m_vrs32, &c.m_q32, &c.m_qp[0], &c.m_qp[1], &c.m_qp[2], &c.m_qtp[0], &c.m_qtp[1], &c.m_qtp[2],
-                    &c.m_g, &c.m_h, &c.m_hold, &c.m_dx, &c.m_bc, &c.m_tc, &c.m_cm, &c.m_work, &c.m_u2i, &c.m_rho64,
-                    &c.m_eu64, &c.m_fb, &c.m_fx, &c.m_fe, &c.m_fep})
-    b->release();
-  (void)hipStreamDestroy(c.stream);
-  delete g_ctx;
-  g_ctx = nullptr;
+  std::lock_guard<std::mutex> lk(g_sys_mutex);
+  for (Context* c : g_sys)
+    if (c) free_context(c);
+  g_sys.clear();
+  g_device = -1;
+  t_current = 0;
+  return 0;
+  WH_API_END
+}
+
+int wh_system_create(int* handle) {
+  WH_API_BEGIN
+  WH_REQUIRE(handle != nullptr, "handle is null");
+  std::lock_guard<std::mutex> lk(g_sys_mutex);
+  WH_REQUIRE(!g_sys.empty(), "wh_init has not been called");
+  g_sys.push_back(new_context(g_device));
+  *handle = static_cast<int>(g_sys.size()) - 1;
+  return 0;
+  WH_API_END
+}
+
+int wh_system_select(int handle) {
+  WH_API_BEGIN
+  std::lock_guard<std::mutex> lk(g_sys_mutex);
+  WH_REQUIRE(handle >= 0 && handle < static_cast<int>(g_sys.size()) && g_sys[handle],
+             "no such system");
+  t_current = handle;
+  WH_HIP(hipSetDevice(g_device));
+  return 0;
+  WH_API_END
+}
+
+int wh_system_destroy(int handle) {
+  WH_API_BEGIN
+  std::lock_guard<std::mutex> lk(g_sys_mutex);
+  WH_REQUIRE(handle > 0 && handle < static_cast<int>(g_sys.size()) && g_sys[handle],
+             "no such system (system 0 ends with wh_finalize)");
+  free_context(g_sys[handle]);
+  g_sys[handle] = nullptr;
   return 0;
   WH_API_END
 }

@@ -104,6 +104,7 @@ struct Context {
   DevBuf m_vrs32, m_q32, m_qp[3], m_qtp[3];   // Q in FP32; its bf16 pieces; pieces of Q^T
   DevBuf m_g, m_h, m_hold, m_dx, m_bc, m_tc, m_cm, m_work, m_u2i, m_rho64, m_eu64;
   DevBuf m_fb, m_fx, m_fe, m_fep;   // bands finished by the FP64 CG (right sides, solutions, e, eprec)
+  DevBuf dav[9];                    // Davidson basis, products and subspace arrays
   // counters
   long nstern = 0;     // operator applications (all precisions)
   long nstern32 = 0;   // of which FP32

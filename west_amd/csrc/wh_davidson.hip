@@ -101,7 +101,7 @@ struct Solver {
 };
 
 Solver& solver() {
-  static Solver s;
+  static thread_local Solver s;
   if (!s.h) {
     WH_REQUIRE(rocblas_create_handle(&s.h) == rocblas_status_success, "rocblas_create_handle failed");
   }
@@ -180,7 +180,9 @@ extern "C" int wh_davidson_diago(const wh_davidson_params* prm, double* ev_out, 
     const long ld = npw;
     if (log_len) *log_len = 0;
     // device basis
-    static DevBuf b_dvg, b_dng, b_tmp, b_hr, b_w, b_e, b_info, b_vr, b_sq;
+    DevBuf &b_dvg = c.dav[0], &b_dng = c.dav[1], &b_tmp = c.dav[2], &b_hr = c.dav[3], &b_w = c.dav[4],
+           &b_e = c.dav[5], &b_info = c.dav[6], &b_vr = c.dav[7], &b_sq = c.dav[8];
+    (void)b_sq;
     z64* dvg = b_dvg.as<z64>(static_cast<size_t>(nvecx) * npw);
     z64* dng = b_dng.as<z64>(static_cast<size_t>(nvecx) * npw);
     z64* tmp = b_tmp.as<z64>(static_cast<size_t>(nvecx) * npw);

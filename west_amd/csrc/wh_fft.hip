@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <map>
+#include <mutex>
 
 #include "wh_fft.h"
 #include "wh_fft_tpl.h"
@@ -507,6 +508,8 @@ size_t plane_smem_bytes(const StickPlanDev& P, int njc, bool density) {
 template <typename K>
 void set_smem(K kernel, size_t bytes) {
   static std::map<const void*, size_t> done;
+  static std::mutex done_mutex;
+  std::lock_guard<std::mutex> lk(done_mutex);
   auto key = reinterpret_cast<const void*>(kernel);
   auto itr = done.find(key);
   if (itr != done.end() && itr->second >= bytes) return;

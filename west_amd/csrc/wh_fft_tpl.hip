@@ -13,6 +13,7 @@
 //    selection into the last forward pass, so the padded lines are never
 //    materialised and no separate scatter/gather sweeps remain.
 #include <map>
+#include <mutex>
 
 #include "wh_fft.h"
 #include "wh_fft_tpl.h"
@@ -679,11 +680,18 @@ __global__ __launch_bounds__(NTP) __attribute__((amdgpu_waves_per_eu(2 * OCC))) 
       const int r = row_of(x, N, imax);
       if (r >= 0) pl[r * prow<N>() + j0 + jj] = v;
     };
-    static_assert(Plan<N>::nst == 2, "two-pass plans");
-    pass<T, N, 0, JC, NTP, +1>(nj, S.tw, from_rows, to_x);
-    // inverse's last pass, V(r), forward's first pass in registers
-    pass_turn<T, N, JC, NTP>(nj, S.tw, in_x, times_v, to_x);
-    pass<T, N, 1, JC, NTP, -1, true>(nj, S.tw, in_x, to_rows);
+    if constexpr (Plan<N>::nst == 2) {
+      pass<T, N, 0, JC, NTP, +1>(nj, S.tw, from_rows, to_x);
+      // inverse's last pass, V(r), forward's first pass in registers
+      pass_turn<T, N, JC, NTP>(nj, S.tw, in_x, times_v, to_x);
+      pass<T, N, 1, JC, NTP, -1, true>(nj, S.tw, in_x, to_rows);
+    } else {
+      // other plans: inverse with V(r) applied on the way out, then the forward
+      auto store_v = [=](int jj, int x, cplx<T> v) { xb[jj * XL + x] = times_v(jj, x, v); };
+      auto in_xl = [=](int jj, int x) { return xb[jj * XL + x]; };
+      passes<T, N, 0, JC, NTP, +1, decltype(from_rows), decltype(store_v), XL>(nj, xb, S.tw, from_rows, store_v);
+      passes<T, N, 0, JC, NTP, -1, decltype(in_xl), decltype(to_rows), XL>(nj, xb, S.tw, in_xl, to_rows);
+    }
   }
   fft_inplace<T, N, MR, NTP, -1, prow<N>()>(P.nrows, S.pl, S.tw);
   cplx<T>* dst = sout + static_cast<long>(it) * sl + static_cast<long>(z) * P.nct;
@@ -873,6 +881,8 @@ size_t plane_smem(bool density, int occ = 1) {
 template <class K>
 void allow_smem(K kernel, size_t bytes) {
   static std::map<const void*, size_t> done;
+  static std::mutex done_mutex;
+  std::lock_guard<std::mutex> lk(done_mutex);
   auto key = reinterpret_cast<const void*>(kernel);
   auto itr = done.find(key);
   if (itr != done.end() && itr->second >= bytes) return;
@@ -1008,10 +1018,12 @@ const TplKernels* table_for(int n) {
     WH_TPL_CASE(40)
     WH_TPL_CASE(45)
     WH_TPL_CASE(48)
+    WH_TPL_CASE(50)
     WH_TPL_CASE(54)
     WH_TPL_CASE(60)
     WH_TPL_CASE(64)
     WH_TPL_CASE(72)
+    WH_TPL_CASE(75)
     WH_TPL_CASE(80)
     WH_TPL_CASE(90)
     WH_TPL_CASE(96)
@@ -1039,10 +1051,12 @@ const TplKernels32* table32_for(int n) {
     WH_TPL32_CASE(40)
     WH_TPL32_CASE(45)
     WH_TPL32_CASE(48)
+    WH_TPL32_CASE(50)
     WH_TPL32_CASE(54)
     WH_TPL32_CASE(60)
     WH_TPL32_CASE(64)
     WH_TPL32_CASE(72)
+    WH_TPL32_CASE(75)
     WH_TPL32_CASE(80)
     WH_TPL32_CASE(90)
     WH_TPL32_CASE(96)

@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <map>
+#include <mutex>
 #include <tuple>
 #include <cstring>
 
@@ -52,11 +53,18 @@ static Backend backend() {
   return v;
 }
 
+// one rocBLAS handle per host thread (handles are not shared between threads)
+struct BlasHandle {
+  rocblas_handle h = nullptr;
+  ~BlasHandle() {
+    if (h) rocblas_destroy_handle(h);
+  }
+};
 static rocblas_handle blas(hipStream_t s) {
-  static rocblas_handle h = nullptr;
-  if (!h) WH_REQUIRE(rocblas_create_handle(&h) == rocblas_status_success, "rocblas_create_handle failed");
-  WH_REQUIRE(rocblas_set_stream(h, s) == rocblas_status_success, "rocblas_set_stream failed");
-  return h;
+  static thread_local BlasHandle bh;
+  if (!bh.h) WH_REQUIRE(rocblas_create_handle(&bh.h) == rocblas_status_success, "rocblas_create_handle failed");
+  WH_REQUIRE(rocblas_set_stream(bh.h, s) == rocblas_status_success, "rocblas_set_stream failed");
+  return bh.h;
 }
 
 namespace {
@@ -257,8 +265,9 @@ struct Scratch {
   }
 };
 
+// split-K partial sums, per host thread
 Scratch& partials() {
-  static Scratch s;
+  static thread_local Scratch s;
   return s;
 }
 
@@ -319,6 +328,8 @@ void dgemm_bra(int M, int N, int K, double alpha, const double* A, long lda, con
   if (be == Backend::Own || K <= 0) return dgemm_bra_own(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s);
   // variants: 0..2 the kernel with its K split aimed at 512, 1024, 2048 workgroups; 3 rocBLAS
   static std::map<std::tuple<int, int, int>, int> chosen;   // (M, K, log2 N) -> variant
+  static std::mutex chosen_mutex;
+  std::unique_lock<std::mutex> lk(chosen_mutex);
   int lg = 0;
   while ((2 << lg) <= N) ++lg;
   const auto key = std::make_tuple(M, K, lg);
@@ -350,8 +361,10 @@ void dgemm_bra(int M, int N, int K, double alpha, const double* A, long lda, con
       if (t[v] < t[best]) best = v;
     it = chosen.emplace(key, best).first;
   }
+  const int pick = it != chosen.end() ? it->second : 0;
+  lk.unlock();
   // recomputed with the chosen variant, so the result does not depend on the timing order
-  run(it != chosen.end() ? it->second : 0);
+  run(pick);
 }
 
 void g0_fix_scale(int M, int N, double g0, const double* A, long lda, const double* B, long ldb,

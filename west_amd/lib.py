@@ -38,6 +38,9 @@ SIGNATURES = {
     "wh_init": ([I], I),
     "wh_finalize": ([], I),
     "wh_last_error": ([], C.c_char_p),
+    "wh_system_create": ([P], I),
+    "wh_system_select": ([I], I),
+    "wh_system_destroy": ([I], I),
     "wh_set_grid": ([I, I, I, I, I, I, P, P, D, D, P], I),
     "wh_set_potential": ([P], I),
     "wh_set_nonlocal": ([I, P, P], I),

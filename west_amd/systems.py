@@ -277,3 +277,28 @@ def kb_projectors(system: System):
     if b.gstart == 2:
         vkb[0, :] = vkb[0, :].real
     return np.ascontiguousarray(vkb), np.array(dvals)
+
+
+def make_h2_clusters(nsys: int = 24, seed: int = 2024, ecutwfc: float = 25.0, min_sep: float = 4.0,
+                     nmol_range=(2, 24), cell_range=(12.0, 24.0), pdep_range=(16, 64)):
+    """A batch of independent H2 clusters of uneven size: for each, a seeded
+    molecule count (uniform in ``nmol_range``), cubic cell edge (uniform in
+    ``cell_range`` bohr, enlarged by 5 % steps while the molecules cannot be
+    placed ``min_sep`` apart) and PDEP count (uniform in ``pdep_range``).
+    Returns a list of (System, n_pdep)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(nsys):
+        nmol = int(rng.integers(nmol_range[0], nmol_range[1] + 1))
+        L = float(rng.uniform(*cell_range))
+        npdep = int(rng.integers(pdep_range[0], pdep_range[1] + 1))
+        s_seed = int(rng.integers(1 << 30))
+        while True:
+            try:
+                s = make_h2_system(nmol=nmol, L=L, ecutwfc=ecutwfc, seed=s_seed, min_sep=min_sep,
+                                   name=f"cluster{k}_{nmol}H2_L{L:.2f}")
+                break
+            except RuntimeError:
+                L *= 1.05
+        out.append((s, npdep))
+    return out

@@ -18,13 +18,23 @@ device.  Non-convergence is reported like the reference: a warning count
 from __future__ import annotations
 
 import ctypes as C
+import threading
 import warnings
 
 import numpy as np
 
 from .lib import DavidsonParams, DavidsonStep, check, load, ptr
 
-_state = {"device": None, "gs": None}
+_state = {"device": None}
+# per system: the registered ground state and the grid registered by gpu_h_factory;
+# the system a thread works on is the one it selected last (0 by default)
+_gs_by_sys: dict = {}
+_grid_key_by_sys: dict = {}
+_tls = threading.local()
+
+
+def _cur() -> int:
+    return getattr(_tls, "handle", 0)
 
 
 def init(device: int = 0):
@@ -32,6 +42,28 @@ def init(device: int = 0):
     check(lib.wh_init(int(device)))
     _state["device"] = device
     return lib
+
+
+def system_create() -> int:
+    """A new, empty system (own stream and workspaces); returns its handle."""
+    h = C.c_int(-1)
+    check(load().wh_system_create(C.addressof(h)))
+    return int(h.value)
+
+
+def system_select(handle: int):
+    """Target the system ``handle`` with every later call from this thread."""
+    check(load().wh_system_select(int(handle)))
+    _tls.handle = int(handle)
+
+
+def current_ground_state():
+    """The ground state registered on this thread's system."""
+    return _gs_by_sys[_cur()]
+
+
+def system_destroy(handle: int):
+    check(load().wh_system_destroy(int(handle)))
 
 
 def set_ground_state(gs, sqvc=None, n_dfpt_maxiter: int = 250, l_kinetic_only: bool = False):
@@ -59,8 +91,8 @@ def set_ground_state(gs, sqvc=None, n_dfpt_maxiter: int = 250, l_kinetic_only: b
         sq = np.zeros(gs.npwx)
         sq[:len(sqvc)] = sqvc
         check(lib.wh_set_coulomb(ptr(sq)))
-    _state["gs"] = gs
-    _state["grid_key"] = None
+    _gs_by_sys[_cur()] = gs
+    _grid_key_by_sys[_cur()] = None
 
 
 def _out_like(a, shape):
@@ -120,7 +152,7 @@ def davidson_diago(n_pdep_eigen: int, n_pdep_times: int = 4, n_pdep_maxiter: int
                    trev_pdep_rel: float = 0.1, tr2_dfpt: float = 1e-16, want_vectors: bool = True):
     """Wstat Davidson; returns (ev, conv, dvg or None, davitr records)."""
     lib = load()
-    gs = _state["gs"]
+    gs = _gs_by_sys[_cur()]
     prm = DavidsonParams(n_pdep_eigen, n_pdep_times, n_pdep_maxiter, trev_pdep, trev_pdep_rel, tr2_dfpt)
     ev = np.zeros(n_pdep_eigen)
     conv = np.zeros(n_pdep_eigen, dtype=np.int32)
@@ -138,6 +170,25 @@ def davidson_diago(n_pdep_eigen: int, n_pdep_times: int = 4, n_pdep_maxiter: int
     return ev, conv.astype(bool), dvg, recs
 
 
+def davidson_batch(jobs, concurrent: bool = True, max_workers: int | None = None):
+    """Wstat Davidson on several registered systems.
+
+    ``jobs``: list of (system handle, keyword arguments of :func:`davidson_diago`).
+    With ``concurrent`` each system runs in its own host thread on its own
+    stream, so the GPU works on all of them at once; otherwise one after the
+    other.  Returns the list of :func:`davidson_diago` results, in order."""
+    def run(job):
+        handle, kw = job
+        system_select(handle)
+        return davidson_diago(**kw)
+
+    if not concurrent:
+        return [run(j) for j in jobs]
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=max_workers or len(jobs)) as ex:
+        return list(ex.map(run, jobs))
+
+
 def gpu_h_factory(gs):
     """``apply_h_factory`` for :func:`west_amd.groundstate.make_ground_state`:
     registers the grid (once), the potential and the projectors of the partial
@@ -145,7 +196,7 @@ def gpu_h_factory(gs):
     (the Sternheimer operator with no registered states, e = 0)."""
     lib = load()
     b = gs.basis
-    if _state.get("grid_key") != id(b):
+    if _grid_key_by_sys.get(_cur()) != id(b):
         nl = np.ascontiguousarray(b.nl, dtype=np.int32)
         nlm = np.ascontiguousarray(b.nlm, dtype=np.int32)
         g2 = np.zeros(gs.npwx)
@@ -156,7 +207,7 @@ def gpu_h_factory(gs):
         check(lib.wh_set_nonlocal(int(vkb.shape[0]), ptr(vkb) if vkb.size else None,
                                   ptr(np.ascontiguousarray(gs.dvan, dtype=np.float64)) if vkb.size else None))
         check(lib.wh_set_params(250, 0))
-        _state["grid_key"] = id(b)
+        _grid_key_by_sys[_cur()] = id(b)
     check(lib.wh_set_potential(ptr(np.ascontiguousarray(gs.vrs, dtype=np.float64))))
 
     def h_psi(psi):
