@@ -23,8 +23,9 @@
  *  - Return value: 0 = success; > 0 = number of unconverged Sternheimer
  *    systems (a warning, like ierr in the reference); < 0 = error, with the
  *    message in wh_last_error().
- *  - One process drives one device.  Calls into the library from several
- *    host threads at once are not supported.
+ *  - One process drives one device.  Several host threads may call the
+ *    library at once when each has selected its own system
+ *    (wh_system_select); two threads on one system race.
  *  - Gamma point only (iq must be 1), nspin = 1, integer occupations,
  *    norm-conserving pseudopotentials with a diagonal D per projector.
  */

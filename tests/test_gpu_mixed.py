@@ -139,3 +139,33 @@ def test_mixed_davidson(mixed, gs_si8):
     print("ev", ev, "ref", ev_ref, f"max rel err {err:.2e}", flush=True)
     assert np.all(conv == conv_ref)
     assert err < 1e-6
+
+
+@pytest.mark.parametrize("key", ["h2", "3h2", "si8"])
+def test_plane_variants(gpu, key, request, monkeypatch):
+    """Every plane-kernel variant of the FP32 operator (LDS-pass kernels sized
+    for 1-3 workgroups per CU, register-line kernels with 2-4 planes per
+    workgroup; WH_PLANE_OCC fixes the variant) against the FP64 oracle, and
+    the register-line kernels shown to have run: their FFT rounding differs
+    from the LDS-pass kernel's, so their output is not bit-identical to it."""
+    from oracle.hamiltonian import apply_sternheimerop
+    gs = request.getfixturevalue(f"gs_{key}")
+    psi = random_wfc(gs, 6, seed=13)
+    e = np.linspace(-0.6, 0.2, 6)
+    ref = apply_sternheimerop(gs, psi, e, gs.alpha_pv)
+    outs = []
+    try:
+        for var in range(1, 7):
+            monkeypatch.setenv("WH_PLANE_OCC", str(var))
+            _register(gpu, gs)
+            gpu.set_precision("mixed6")
+            out = gpu.apply_sternheimerop_to_m_wfcs(gs.nbndval, psi, e, gs.alpha_pv)
+            err = _rel(out, ref)
+            print(f"{key} variant {var}: rel err {err:.2e}", flush=True)
+            assert err < 1e-5
+            outs.append(out)
+    finally:
+        gpu.set_precision("fp64")
+        monkeypatch.delenv("WH_PLANE_OCC", raising=False)
+        _register(gpu, gs)
+    assert any(not np.array_equal(o, outs[0]) for o in outs[3:])

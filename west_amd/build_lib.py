@@ -15,7 +15,7 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 BUILD = HERE / "_build"
 LIB = HERE / "libwesthip.so"
-SOURCES = ["wh_fft.hip", "wh_fft_tpl.hip", "wh_gemm.hip", "wh_gemm_bf16.hip", "wh_vec.hip", "wh_api.hip",
+SOURCES = ["wh_fft.hip", "wh_fft_tpl.hip", "wh_fft_reg.hip", "wh_gemm.hip", "wh_gemm_bf16.hip", "wh_vec.hip", "wh_api.hip",
            "wh_davidson.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -26,9 +26,14 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          "-Wno-unused-function", "-Wno-unused-command-line-argument"]
 
 
+# per-file additions: the register-line FFT kernels rely on arithmetic with
+# known zeros folding away (signed zeros, NaN and Inf never arise there)
+EXTRA = {"wh_fft_reg.hip": ["-fno-signed-zeros", "-ffinite-math-only"]}
+
+
 def _compile(src: str) -> Path:
     obj = BUILD / (Path(src).stem + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", str(CSRC / src), "-o", str(obj)]
+    cmd = [HIPCC, *FLAGS, *EXTRA.get(src, []), "-c", str(CSRC / src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-6000:]}")

@@ -9,6 +9,7 @@
 #include "../../include/west_hip.h"
 #include "wh_context.h"
 #include "wh_fft_tpl.h"
+#include "wh_fft_reg.h"
 #include "wh_gemm.h"
 #include "wh_gemm_bf16.h"
 #include "wh_vec.h"
@@ -258,7 +259,10 @@ int linsolve_dev(int m, z64* b, z64* x, const std::vector<double>& e_col, const 
   int* d_l2i = c.w_l2i.as<int>(m);
   double* d_beta = c.w_beta.as<double>(m);
   double* d_eu = c.w_eu.as<double>(m);
-  double* d_rho = c.w_rho.as<double>(m);
+  // rho of each band's current g, by band: from prec_rho for the first
+  // residual, then from the update kernel of the previous iteration
+  double* d_rhob = c.w_rhob.as<double>(m);
+  const RhoOut rho_out{c.d_g2kin, d_eprec_col, d_rhob};
   // step 1: g = A x - b
   stern_apply(x, g, d_e_col, m, c.alpha_pv);
   axpby(static_cast<long>(vec), -1.0, b, 1.0, g, s);
@@ -270,13 +274,14 @@ int linsolve_dev(int m, z64* b, z64* x, const std::vector<double>& e_col, const 
   const int itmax = maxiter > 0 ? maxiter : c.n_dfpt_maxiter;
   for (int iter = 1; iter <= itmax; ++iter) {
     WH_HIP(hipMemcpyAsync(d_l2i, l2i.data(), sizeof(int) * nc, hipMemcpyHostToDevice, s));
-    WH_TIMED("cg_prec", prec_rho(nc, npw, c.gstart, d_l2i, g, ld, c.d_g2kin, d_eprec_col, d_rho, s));
-    WH_HIP(hipMemcpyAsync(rho_c.data(), d_rho, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
+    if (iter == 1)
+      WH_TIMED("cg_prec", prec_rho(nc, npw, c.gstart, d_l2i, g, ld, c.d_g2kin, d_eprec_col, d_rhob, s, true));
+    WH_HIP(hipMemcpyAsync(rho_c.data(), d_rhob, sizeof(double) * m, hipMemcpyDeviceToHost, s));
     WH_HIP(hipStreamSynchronize(s));
     ++c.cg_iters;
     for (int lb = 0; lb < nc; ++lb) {
       const int ib = l2i[lb];
-      rho[ib] = rho_c[lb];
+      rho[ib] = rho_c[ib];
       if (rho[ib] < tr2) is_conv[ib] = 1;
     }
     int n2 = 0;
@@ -294,7 +299,7 @@ int linsolve_dev(int m, z64* b, z64* x, const std::vector<double>& e_col, const 
     WH_HIP(hipMemcpyAsync(d_eu, eu.data(), sizeof(double) * nc, hipMemcpyHostToDevice, s));
     WH_TIMED("cg_dir", direction(nc, npw, d_l2i, d_beta, g, c.d_g2kin, d_eprec_col, h, hold, bc, ld, s));
     stern_apply(bc, tc, d_eu, nc, c.alpha_pv);
-    WH_TIMED("cg_update", cg_update(nc, npw, c.gstart, d_l2i, h, g, tc, x, ld, nullptr, s));
+    WH_TIMED("cg_update", cg_update(nc, npw, c.gstart, d_l2i, h, g, tc, x, ld, nullptr, s, &rho_out));
     std::swap(h, hold);   // hold <- h  (linsolve 319)
     for (int lb = 0; lb < nc; ++lb) rhoold[l2i[lb]] = rho[l2i[lb]];
     // keep the staging vectors alive until the copies above have been consumed
@@ -371,9 +376,18 @@ void stern_apply32(const z32* x, z32* t, const double* d_e, int m, double alpha)
     for (int q0 = 0; q0 < npair; q0 += chunk) {
       const int nq = std::min(chunk, npair - q0);
       WH_TIMED("zinv32", K->zinv(c.plan.dev, items + q0, nq, x, l32, x, l32, sa, ZINV_PAIR, s));
+      // variants 1..max_occ: LDS-pass kernel sized for that many workgroups
+      // per CU; above: register-line kernels
+      const int nvar = K->max_occ + plane_reg_variants(c.plan.dev, true);
       WH_TIMED("plane_vloc32", plane_occ_launch(
-                                   c.plan.vloc32_occ, K->max_occ, nq,
-                                   [&](int occ) { K->vloc(c.plan.dev, ident, nq, sa, sb, v32, 0, 1.0 / c.nnr, occ, s); },
+                                   c.plan.vloc32_occ, nvar, nq,
+                                   [&](int v) {
+                                     if (v <= K->max_occ)
+                                       K->vloc(c.plan.dev, ident, nq, sa, sb, v32, 0, 1.0 / c.nnr, v, s);
+                                     else
+                                       plane_reg_vloc(v - K->max_occ - 1, c.plan.dev, ident, nq, sa, sb, v32, 0,
+                                                      1.0 / c.nnr, s);
+                                   },
                                    s));
       WH_TIMED("zfwd32", K->zfwd(c.plan.dev, items + q0, nq, sb, x, l32, t, l32, za, s));
       WH_HIP(hipGetLastError());
@@ -424,6 +438,9 @@ int linsolve_mixed(int m, z64* b, z64* x, const std::vector<double>& e_col, cons
   double* d_rho = c.w_rho.as<double>(m);
   double* d_rho64 = c.m_rho64.as<double>(m);
   double* d_eu64 = c.m_eu64.as<double>(m);
+  // rho of each band's FP32 g after an update, by band (fused into the update kernel)
+  double* d_rhob = c.w_rhob.as<double>(m);
+  const RhoOut rho_out{c.d_g2kin, d_eprec_col, d_rhob};
   WH_HIP(hipMemsetAsync(dx, 0, sizeof(z32) * v32, s));
 
   std::vector<char> is_conv(m, 0), to_fp64(m, 0);
@@ -485,13 +502,24 @@ int linsolve_mixed(int m, z64* b, z64* x, const std::vector<double>& e_col, cons
     std::vector<int> small;
     for (int ib = 0; ib < m; ++ib)
       if (rho[ib] < tr2) small.push_back(ib);
-    if (!small.empty()) refine(small, true);
+    if (!small.empty()) {
+      refine(small, true);
+      std::vector<int> again;
+      for (int ib : small)
+        if (!is_conv[ib]) again.push_back(ib);
+      if (!again.empty()) precond(again, d_u2i);
+    }
   }
   std::vector<int> act;
   for (int ib = 0; ib < m; ++ib)
     if (!is_conv[ib]) act.push_back(ib);
   for (int iter = 1; iter <= c.n_dfpt_maxiter && !act.empty(); ++iter) {
-    precond(act, d_l2i);
+    if (iter > 1) {
+      // every band in act was updated in the previous iteration
+      WH_HIP(hipMemcpyAsync(tmp.data(), d_rhob, sizeof(double) * m, hipMemcpyDeviceToHost, s));
+      WH_HIP(hipStreamSynchronize(s));
+      for (int ib : act) rho[ib] = tmp[ib];
+    }
     ++c.cg_iters;
     list.clear();
     for (int ib : act)
@@ -529,7 +557,7 @@ int linsolve_mixed(int m, z64* b, z64* x, const std::vector<double>& e_col, cons
     WH_HIP(hipMemcpyAsync(d_eu, eu.data(), sizeof(double) * nc, hipMemcpyHostToDevice, s));
     WH_TIMED("cg_dir32", direction(nc, npw, d_l2i, d_beta, g, c.d_g2kin, d_eprec_col, h, hold, bc, l32, s));
     stern_apply32(bc, tc, d_eu, nc, c.alpha_pv);
-    WH_TIMED("cg_update32", cg_update(nc, npw, c.gstart, d_l2i, h, g, tc, dx, l32, nullptr, s));
+    WH_TIMED("cg_update32", cg_update(nc, npw, c.gstart, d_l2i, h, g, tc, dx, l32, nullptr, s, &rho_out));
     std::swap(h, hold);
     for (int lb = 0; lb < nc; ++lb) rhoold[l2i[lb]] = rho[l2i[lb]];
     WH_HIP(hipStreamSynchronize(s));

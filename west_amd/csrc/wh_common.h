@@ -1,6 +1,8 @@
 // Common definitions for the MI355X Wstat library (gfx950, HIP).
 #pragma once
 
+#include <cstdlib>
+
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -78,6 +80,15 @@ __host__ __device__ __forceinline__ cplx<T> mul_i(cplx<T> a, int dir) {
 
 using z64 = cplx<double>;
 using z32 = cplx<float>;
+
+// WH_TUNE_LOG=1: print the timings behind every timed choice to stderr
+inline bool tune_log() {
+  static const bool on = [] {
+    const char* e = std::getenv("WH_TUNE_LOG");
+    return e != nullptr && e[0] == '1';
+  }();
+  return on;
+}
 
 inline int ceil_div(long a, long b) { return static_cast<int>((a + b - 1) / b); }
 

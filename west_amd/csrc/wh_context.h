@@ -94,7 +94,7 @@ struct Context {
   // workspaces
   DevBuf w_items, w_items2, w_items_id, w_sticks_a, w_sticks_b, w_cmat;
   DevBuf w_x, w_t, w_g, w_h, w_hold, w_bc, w_tc, w_b, w_dvr, w_in, w_out, w_dvin, w_dng;
-  DevBuf w_l2i, w_beta, w_eu, w_rho, w_ecol, w_epcol, w_dscr;
+  DevBuf w_l2i, w_beta, w_eu, w_rho, w_rhob, w_ecol, w_epcol, w_dscr;
   // mixed precision: FP32 operator with split-BF16 projectors, FP64 refinement
   int precision = 0;              // 0: FP64, 1: mixed (3-product split BF16), 2: mixed (6-product)
   double refine_delta = 1e-4;     // refine when rho < refine_delta * rho of the last refinement

@@ -17,6 +17,8 @@
 #pragma once
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 
 #include "wh_common.h"
 
@@ -92,6 +94,7 @@ void plane_occ_launch(int& chosen, int max_occ, int nitems, Run run, hipStream_t
     WH_HIP(hipEventSynchronize(b));
     float ms = 0.f;
     WH_HIP(hipEventElapsedTime(&ms, a, b));
+    if (tune_log()) std::fprintf(stderr, "[west_hip] plane variant %d: %.3f ms for %d items\n", occ, ms, nitems);
     if (occ == 1 || ms < best) {
       best = ms;
       pick = occ;

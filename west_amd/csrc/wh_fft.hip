@@ -609,9 +609,9 @@ void StickPlan::build(int nr1, int nr2, int nr3, int npw, const int* nl, const i
   const char* gen = std::getenv("WH_FFT_GENERIC");
   tpl = (gen && gen[0] == '1') ? nullptr : tpl_kernels_f64(dev);
   tpl32 = tpl_kernels_f32(dev);
-  // plane-kernel LDS sizing: WH_PLANE_OCC (1..3) fixes it, else timed on the first large launch
+  // plane-kernel variant: WH_PLANE_OCC (1..9) fixes it, else timed on the first large launch
   const char* occ_env = std::getenv("WH_PLANE_OCC");
-  const int occ = (occ_env && occ_env[0] >= '1' && occ_env[0] <= '3') ? occ_env[0] - '0' : 0;
+  const int occ = (occ_env && occ_env[0] >= '1' && occ_env[0] <= '9') ? occ_env[0] - '0' : 0;
   vloc_occ = occ;
   vloc32_occ = occ;
 }

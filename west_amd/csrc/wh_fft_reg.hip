@@ -1,0 +1,286 @@
+// Plane kernel of the local-potential transform with register-resident lines.
+//
+// Same step and data layout as k_plane_vloc_t of wh_fft_tpl.hip (one a3-plane
+// of sticks: inverse FFT along a2 on the |i| <= imax rows, inverse FFT along
+// a1, times V(r), forward along a1 keeping |i| <= imax, forward along a2), but
+// every 1D transform is done whole by one thread with its N points in
+// registers: all radix passes, twiddles (compile-time constants) and the
+// V(r) multiply happen without LDS traffic, and LDS only carries the two
+// transposes between the a2 and the a1 lines.  A workgroup takes PZ planes
+// of one item so that its PZ * N a1-lines fill whole wavefronts.
+//
+// The a1 transforms know at compile time which of their inputs are zero
+// (|i| > IMX, the sphere bound) and which outputs are dropped; this file is
+// compiled with -fno-signed-zeros -ffinite-math-only so that the arithmetic
+// on those known zeros folds away.  Neither flag changes any finite result.
+#include <map>
+#include <mutex>
+#include <utility>
+
+#include "wh_fft.h"
+#include "wh_fft_dev.h"
+#include "wh_fft_reg.h"
+
+namespace wh {
+
+namespace {
+
+template <int... I, class F>
+__device__ __forceinline__ void sfor_seq(std::integer_sequence<int, I...>, F&& f) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+// f(integral_constant<int, i>) for i = 0 .. N-1, unrolled at compile time
+template <int N, class F>
+__device__ __forceinline__ void sfor(F&& f) {
+  sfor_seq(std::make_integer_sequence<int, N>{}, static_cast<F&&>(f));
+}
+
+// In-register transform of v[0..N): the Stockham passes of Plan<N> with every
+// index and twiddle exp(DIR 2 pi i j k / (P R)) a compile-time constant.
+template <typename T, int N, int DIR, int S = 0>
+__device__ __forceinline__ void fft_line(cplx<T>* v) {
+  using PL = Plan<N, false>;
+  if constexpr (S < PL::nst) {
+    constexpr int R = PL::radix(S);
+    constexpr int P = PL::stride(S);
+    constexpr int NBL = N / R;
+    cplx<T> w[N];
+    sfor<NBL>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      constexpr int k = i % P;
+      cplx<T> b[R];
+      sfor<R>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        if constexpr (k != 0 && j != 0) {
+          constexpr T c = T(cos2pi(j * k, P * R));
+          constexpr T s = T(DIR * sin2pi(j * k, P * R));
+          b[j] = twiddle_mul(v[i + j * NBL], c, s);
+        } else {
+          b[j] = v[i + j * NBL];
+        }
+      });
+      dft<T, R, DIR>(b);
+      sfor<R>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        w[(i - k) * R + k + j * P] = b[j];
+      });
+    });
+    sfor<N>([&](auto I) { v[decltype(I)::value] = w[decltype(I)::value]; });
+    fft_line<T, N, DIR, S + 1>(v);
+  }
+}
+
+// compile-time bound on imax the a1 pruning assumes (the smooth grid of a
+// norm-conserving calculation has N >= 4 imax + 1)
+template <int N>
+constexpr int imx() {
+  return (N + 3) / 4;
+}
+
+template <int N, int PZ>
+constexpr int nt_reg() {
+  return (PZ * N + 63) / 64 * 64;
+}
+
+// PT: storage type of the potential; PZ: planes per workgroup.
+// LDS holds PZ planes of NR = 2 IMX + 1 rows (row IMX + i for a1 frequency i,
+// |i| <= IMX; rows beyond the sphere's |i| <= imax stay zero), so that the a1
+// lines load and store their rows at compile-time offsets without branches.
+template <typename T, int N, typename PT, int PZ>
+__global__ __launch_bounds__((nt_reg<N, PZ>())) __attribute__((amdgpu_waves_per_eu(2))) void k_plane_vloc_reg(
+    StickPlanDev P, const FftItem* items, const cplx<T>* sin, cplx<T>* sout, const PT* vpot, long ldpot,
+    double scale) {
+  constexpr int RS = N + 1;   // row stride: odd, so row-parallel accesses spread over all banks
+  constexpr int IMX = imx<N>();
+  constexpr int NR = 2 * IMX + 1;
+  constexpr int PE = NR * RS;   // elements per plane
+  constexpr int NTR = nt_reg<N, PZ>();
+  extern __shared__ char smem_raw[];
+  cplx<T>* pl = reinterpret_cast<cplx<T>*>(smem_raw);   // [PZ][NR][RS]
+  const int nrows = P.nrows;
+  const int r0 = IMX - P.imax;   // first row of the sphere
+  const int nct = P.nct;
+  const int z0 = blockIdx.x * PZ;
+  const int nz = min(PZ, N - z0);
+  const FftItem item = items[blockIdx.y];
+  const long sl = static_cast<long>(N) * nct;
+  const int tid = threadIdx.x;
+
+  // sticks of planes z0 .. z0 + nz - 1 (contiguous) into zeroed rows
+  for (int t = tid; t < nz * PE; t += NTR) pl[t] = mk<T>(T(0), T(0));
+  __syncthreads();
+  {
+    const cplx<T>* src = sin + static_cast<long>(item.a) * sl + static_cast<long>(z0) * nct;
+    for (int t = tid; t < nz * nct; t += NTR) {
+      const int p = t / nct;
+      const int pos = P.col_pos[t - p * nct];
+      const int r = pos / N;
+      pl[p * PE + (r0 + r) * RS + (pos - r * N)] = src[t];
+    }
+  }
+  __syncthreads();
+
+  // a2 lines (rows of the sphere): inverse
+  for (int t = tid; t < nz * nrows; t += NTR) {
+    const int p = t / nrows;
+    cplx<T>* row = pl + p * PE + (r0 + t - p * nrows) * RS;
+    cplx<T> v[N];
+    sfor<N>([&](auto Y) { v[decltype(Y)::value] = row[decltype(Y)::value]; });
+    fft_line<T, N, +1>(v);
+    sfor<N>([&](auto Y) { row[decltype(Y)::value] = v[decltype(Y)::value]; });
+  }
+  __syncthreads();
+
+  // a1 lines: inverse from the |i| <= IMX rows, V(r), forward back to them
+  {
+    const T sc = T(scale);
+    const PT* V0 = vpot + (item.pot >= 0 ? static_cast<long>(item.pot) * ldpot : 0L);
+    for (int t = tid; t < nz * N; t += NTR) {
+      const int p = t / N;
+      const int y = t - p * N;
+      cplx<T>* col = pl + p * PE + IMX * RS + y;   // row of i = 0
+      cplx<T> v[N];
+      sfor<N>([&](auto X) {
+        constexpr int x = decltype(X)::value;
+        if constexpr (x <= IMX) {
+          v[x] = col[x * RS];
+        } else if constexpr (x >= N - IMX) {
+          v[x] = col[-(N - x) * RS];
+        } else {
+          v[x] = mk<T>(T(0), T(0));
+        }
+      });
+      fft_line<T, N, +1>(v);
+      const PT* vr = V0 + (static_cast<long>(z0 + p) * N + y) * N;
+      // keep the potential loads after the inverse transform (register pressure)
+      __builtin_amdgcn_sched_barrier(0);
+      sfor<N>([&](auto X) {
+        constexpr int x = decltype(X)::value;
+        v[x] = cscale(v[x], T(vr[x]) * sc);
+      });
+      fft_line<T, N, -1>(v);
+      sfor<N>([&](auto X) {
+        constexpr int x = decltype(X)::value;
+        if constexpr (x <= IMX) {
+          col[x * RS] = v[x];
+        } else if constexpr (x >= N - IMX) {
+          col[-(N - x) * RS] = v[x];
+        }
+      });
+    }
+  }
+  __syncthreads();
+
+  // a2 lines: forward
+  for (int t = tid; t < nz * nrows; t += NTR) {
+    const int p = t / nrows;
+    cplx<T>* row = pl + p * PE + (r0 + t - p * nrows) * RS;
+    cplx<T> v[N];
+    sfor<N>([&](auto Y) { v[decltype(Y)::value] = row[decltype(Y)::value]; });
+    fft_line<T, N, -1>(v);
+    sfor<N>([&](auto Y) { row[decltype(Y)::value] = v[decltype(Y)::value]; });
+  }
+  __syncthreads();
+
+  cplx<T>* dst = sout + static_cast<long>(blockIdx.y) * sl + static_cast<long>(z0) * nct;
+  for (int t = tid; t < nz * nct; t += NTR) {
+    const int p = t / nct;
+    const int pos = P.col_pos[t - p * nct];
+    const int r = pos / N;
+    dst[t] = pl[p * PE + (r0 + r) * RS + (pos - r * N)];
+  }
+}
+
+// planes per workgroup of variant v
+constexpr int kPz[] = {2, 3, 4};
+constexpr int kNumPz = 3;
+
+size_t reg_smem(const StickPlanDev& P, int pz, size_t elem) {
+  const int nr = 2 * ((P.n1 + 3) / 4) + 1;
+  return elem * static_cast<size_t>(pz) * nr * (P.n1 + 1);
+}
+
+template <class K>
+void allow_smem_reg(K kernel, size_t bytes) {
+  static std::map<const void*, size_t> done;
+  static std::mutex done_mutex;
+  std::lock_guard<std::mutex> lk(done_mutex);
+  auto key = reinterpret_cast<const void*>(kernel);
+  auto itr = done.find(key);
+  if (itr != done.end() && itr->second >= bytes) return;
+  WH_HIP(hipFuncSetAttribute(key, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
+  done[key] = bytes;
+}
+
+template <typename T, int N, typename PT, int PZ>
+void launch_pz(const StickPlanDev& P, const FftItem* it, int n, const cplx<T>* si, cplx<T>* so, const PT* v, long ldv,
+               double scale, hipStream_t s) {
+  const size_t sm = reg_smem(P, PZ, sizeof(cplx<T>));
+  allow_smem_reg(k_plane_vloc_reg<T, N, PT, PZ>, sm);
+  k_plane_vloc_reg<T, N, PT, PZ><<<dim3(ceil_div(N, PZ), n), nt_reg<N, PZ>(), sm, s>>>(P, it, si, so, v, ldv, scale);
+}
+
+template <typename T, int N, typename PT>
+void launch_n(int var, const StickPlanDev& P, const FftItem* it, int n, const cplx<T>* si, cplx<T>* so, const PT* v,
+              long ldv, double scale, hipStream_t s) {
+  switch (kPz[var]) {
+    case 2: return launch_pz<T, N, PT, 2>(P, it, n, si, so, v, ldv, scale, s);
+    case 3: return launch_pz<T, N, PT, 3>(P, it, n, si, so, v, ldv, scale, s);
+    default: return launch_pz<T, N, PT, 4>(P, it, n, si, so, v, ldv, scale, s);
+  }
+}
+
+// compiled line lengths (FP32 only: an FP64 line of these lengths does not fit
+// in the registers of one thread)
+#define WH_REG_SIZES(X) X(18) X(20) X(24) X(30) X(32) X(36) X(40) X(45) X(48) X(50) X(54) X(60) X(64) X(72) X(75) X(80)
+
+bool compiled(int n, bool fp32) {
+  if (!fp32) return false;
+  switch (n) {
+#define WH_REG_CASE(NN) \
+  case NN:              \
+    return true;
+    WH_REG_SIZES(WH_REG_CASE)
+#undef WH_REG_CASE
+    default: return false;
+  }
+}
+
+void dispatch32(int var, const StickPlanDev& P, const FftItem* it, int n, const z32* si, z32* so, const float* v,
+                long ldv, double scale, hipStream_t s) {
+  switch (P.n1) {
+#define WH_REG_CASE(NN) \
+  case NN:              \
+    return launch_n<float, NN, float>(var, P, it, n, si, so, v, ldv, scale, s);
+    WH_REG_SIZES(WH_REG_CASE)
+#undef WH_REG_CASE
+    default: WH_REQUIRE(false, "plane_reg_vloc: no kernel for this grid");
+  }
+}
+
+}  // namespace
+
+int plane_reg_variants(const StickPlanDev& P, bool fp32) {
+  if (P.n1 != P.n2 || P.n1 != P.n3 || !compiled(P.n1, fp32)) return 0;
+  if (P.imax > (P.n1 + 3) / 4) return 0;
+  const size_t elem = fp32 ? sizeof(z32) : sizeof(z64);
+  int nv = 0;
+  for (int v = 0; v < kNumPz; ++v)
+    if (reg_smem(P, kPz[v], elem) <= 160u * 1024u) nv = v + 1;
+  return nv;
+}
+
+void plane_reg_vloc(int v, const StickPlanDev& P, const FftItem* items, int n, const z64* sin, z64* sout,
+                    const double* vpot, long ldpot, double scale, hipStream_t s) {
+  (void)items, (void)n, (void)sin, (void)sout, (void)vpot, (void)ldpot, (void)scale, (void)s;
+  WH_REQUIRE(v >= 0 && v < plane_reg_variants(P, false), "plane_reg_vloc: no such variant");
+}
+
+void plane_reg_vloc(int v, const StickPlanDev& P, const FftItem* items, int n, const z32* sin, z32* sout,
+                    const float* vpot, long ldpot, double scale, hipStream_t s) {
+  WH_REQUIRE(v >= 0 && v < plane_reg_variants(P, true), "plane_reg_vloc: no such variant");
+  WH_REQUIRE(static_cast<const void*>(sin) != static_cast<const void*>(sout), "plane_reg_vloc: input and output alias");
+  dispatch32(v, P, items, n, sin, sout, vpot, ldpot, scale, s);
+}
+
+}  // namespace wh

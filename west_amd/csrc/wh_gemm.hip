@@ -296,9 +296,11 @@ void dgemm(bool transa, int M, int N, int K, double alpha, const double* A, long
 static void dgemm_bra_own(int M, int N, int K, double alpha, const double* A, long lda, const double* B, long ldb,
                           double g0, const double* scale, double* C, long ldc, hipStream_t s, int wg_target = 512) {
   const int tiles = ceil_div(M, BM) * ceil_div(N, BN);
-  // enough workgroups for the compute units, each K chunk at least 64 tiles long
-  int split = std::max(1, std::min(16, ceil_div(wg_target, tiles)));
-  split = std::max(1, std::min(split, K / (64 * BK)));
+  // enough workgroups for the compute units (few output tiles, as in the
+  // narrow products of a partial batch, need a deep split), each K chunk at
+  // least 16 steps long
+  int split = std::max(1, std::min(64, ceil_div(wg_target, tiles)));
+  split = std::max(1, std::min(split, K / (16 * BK)));
   const int kchunk = ceil_div(ceil_div(std::max(K, 1), split), BK) * BK;
   split = ceil_div(std::max(K, 1), kchunk);
   const long pstride = static_cast<long>(ldc) * N;
@@ -338,7 +340,7 @@ void dgemm_bra(int M, int N, int K, double alpha, const double* A, long lda, con
     else dgemm_bra_own(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s, 512 << v);
   };
   auto it = chosen.find(key);
-  if (it == chosen.end() && N >= 512) {
+  if (it == chosen.end() && N >= 32) {
     // time every variant on this call's arguments (a first round warms them up)
     hipEvent_t a, b;
     WH_HIP(hipEventCreate(&a));
@@ -359,6 +361,9 @@ void dgemm_bra(int M, int N, int K, double alpha, const double* A, long lda, con
     int best = 0;
     for (int v = 1; v < 4; ++v)
       if (t[v] < t[best]) best = v;
+    if (tune_log())
+      std::fprintf(stderr, "[west_hip] dgemm_bra M=%d N=%d K=%d: own512 %.3f own1024 %.3f own2048 %.3f rocblas %.3f ms -> %d\n",
+                   M, N, K, t[0], t[1], t[2], t[3], best);
     it = chosen.emplace(key, best).first;
   }
   const int pick = it != chosen.end() ? it->second : 0;

@@ -6,18 +6,24 @@
 
 namespace wh {
 
-// rho[lb] = <M^-1 g|g> for the columns l2i[0..nc)
+// rho[lb] (rho[l2i[lb]] with by_band) = <M^-1 g|g> for the columns l2i[0..nc)
 void prec_rho(int nc, int npw, int gstart, const int* l2i, const z64* g, long ld, const double* g2kin,
-              const double* eprec, double* rho, hipStream_t s);
+              const double* eprec, double* rho, hipStream_t s, bool by_band = false);
 void prec_rho(int nc, int npw, int gstart, const int* l2i, const z32* g, long ld, const double* g2kin,
-              const double* eprec, double* rho, hipStream_t s);
+              const double* eprec, double* rho, hipStream_t s, bool by_band = false);
+// optional output of cg_update: rho[ib] = <M^-1 g|g> of every updated band's new g
+struct RhoOut {
+  const double* g2kin = nullptr;
+  const double* eprec = nullptr;
+  double* rho = nullptr;
+};
 // h = -M^-1 g + beta hold, bc = h (compact)
 void direction(int nc, int npw, const int* l2i, const double* beta, const z64* g, const double* g2kin,
                const double* eprec, z64* h, const z64* hold, z64* bc, long ld, hipStream_t s);
 void direction(int nc, int npw, const int* l2i, const double* beta, const z32* g, const double* g2kin,
                const double* eprec, z32* h, const z32* hold, z32* bc, long ld, hipStream_t s);
 void cg_update(int nc, int npw, int gstart, const int* l2i, const z32* h, z32* g, const z32* tc, z32* x, long ld,
-               double* ac_out, hipStream_t s);
+               double* ac_out, hipStream_t s, const RhoOut* rho = nullptr);
 void kinetic_minus_e(int m, int npw, const z32* x, z32* t, long ld, const double* g2kin, const double* e,
                      hipStream_t s);
 // mixed-precision refinement (u2i: bands of the update list)
@@ -36,7 +42,7 @@ void copy_rows(int n, int npw, const int* list, const z64* in, long ld_in, z64* 
                hipStream_t s);
 void gather_real(int n, const int* list, const double* in, double* out, hipStream_t s);
 void cg_update(int nc, int npw, int gstart, const int* l2i, const z64* h, z64* g, const z64* tc, z64* x, long ld,
-               double* ac_out, hipStream_t s);
+               double* ac_out, hipStream_t s, const RhoOut* rho = nullptr);
 void gather_sticks(int m, int npw, const int* perm, const z64* in, long ldin, z64* out, long ldout, const double* w,
                    hipStream_t s);
 void scatter_caller(int m, int npw, int npwx, const int* perm, const z64* in, long ldin, z64* out, long ldout,

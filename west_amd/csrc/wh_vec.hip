@@ -38,7 +38,8 @@ __device__ __forceinline__ double gamma_dot_part(const z64* a, const z64* b, int
 // V is the vector precision; the inner products are summed in FP64.
 template <typename V>
 __global__ __launch_bounds__(NT) void k_prec_rho(int npw, int gstart, const int* l2i, const cplx<V>* g, long ld,
-                                                 const double* g2kin, const double* eprec, double* rho) {
+                                                 const double* g2kin, const double* eprec, double* rho,
+                                                 bool by_band) {
   __shared__ double red[NT / 64];
   const int lb = blockIdx.x;
   const int ib = l2i[lb];
@@ -54,7 +55,7 @@ __global__ __launch_bounds__(NT) void k_prec_rho(int npw, int gstart, const int*
   }
   s = block_sum(s, red);
   r0 = block_sum(r0, red);
-  if (threadIdx.x == 0) rho[lb] = 2.0 * s - (gstart == 2 ? r0 : 0.0);
+  if (threadIdx.x == 0) rho[by_band ? ib : lb] = 2.0 * s - (gstart == 2 ? r0 : 0.0);
 }
 
 // h[ib] = -M^-1 g[ib] + beta[lb] hold[ib];  bc[lb] = h[ib]   (linsolve 150-157, 226-240)
@@ -124,10 +125,13 @@ __global__ __launch_bounds__(NT) void k_cg_update(int npw, int gstart, const int
 // The same update with the band's h and t (and g when CG_ is set) held in
 // registers between the inner products and the update (one read of each
 // instead of two): 1024 threads, up to E elements each.
+// With RHO the band's next rho = <M^-1 g|g> (as k_prec_rho forms it) is summed
+// from the updated g while it is in registers and stored at rho[ib].
 constexpr int NTU = 1024;
-template <typename V, int E, bool CG_>
+template <typename V, int E, bool CG_, bool RHO>
 __global__ __launch_bounds__(NTU) void k_cg_update_reg(int npw, int gstart, const int* l2i, const cplx<V>* h,
-                                                      cplx<V>* g, const cplx<V>* tc, cplx<V>* x, long ld, long ldc) {
+                                                      cplx<V>* g, const cplx<V>* tc, cplx<V>* x, long ld, long ldc,
+                                                      const double* g2kin, const double* eprec, double* rho) {
   __shared__ double red[NTU / 64];
   const int lb = blockIdx.x;
   const int ib = l2i[lb];
@@ -159,6 +163,8 @@ __global__ __launch_bounds__(NTU) void k_cg_update_reg(int npw, int gstart, cons
   }
   __syncthreads();   // every thread has read gi[0] before it is updated
   const V step = V(a / c);
+  const double ep = RHO ? eprec[ib] : 1.0;
+  double sr = 0.0, r0 = 0.0;
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int q = threadIdx.x + e * NTU;
@@ -168,8 +174,21 @@ __global__ __launch_bounds__(NTU) void k_cg_update_reg(int npw, int gstart, cons
       if constexpr (CG_) gq = gv[e];
       else gq = gi[q];
       xi[q] = mk<V>(xv.x - step * hv[e].x, xv.y - step * hv[e].y);
-      gi[q] = mk<V>(gq.x - step * tv[e].x, gq.y - step * tv[e].y);
+      const cplx<V> gn = mk<V>(gq.x - step * tv[e].x, gq.y - step * tv[e].y);
+      gi[q] = gn;
+      if constexpr (RHO) {
+        const V f = V(1.0 / fmax(1.0, g2kin[q] / ep));
+        const cplx<V> pv = mk<V>(gn.x * f, gn.y * f);
+        const double t = static_cast<double>(pv.x) * gn.x + static_cast<double>(pv.y) * gn.y;
+        sr += t;
+        if (q == 0) r0 = static_cast<double>(pv.x) * gn.x;
+      }
     }
+  }
+  if constexpr (RHO) {
+    sr = block_sum<NTU>(sr, red);
+    // element 0 is thread 0's
+    if (threadIdx.x == 0) rho[ib] = 2.0 * sr - (gstart == 2 ? r0 : 0.0);
   }
 }
 
@@ -337,16 +356,16 @@ __global__ __launch_bounds__(NT) void k_eprec(int npw, const z64* evc, long ld, 
 }  // namespace
 
 void prec_rho(int nc, int npw, int gstart, const int* l2i, const z64* g, long ld, const double* g2kin,
-              const double* eprec, double* rho, hipStream_t s) {
+              const double* eprec, double* rho, hipStream_t s, bool by_band) {
   if (nc <= 0) return;
-  k_prec_rho<double><<<nc, NT, 0, s>>>(npw, gstart, l2i, g, ld, g2kin, eprec, rho);
+  k_prec_rho<double><<<nc, NT, 0, s>>>(npw, gstart, l2i, g, ld, g2kin, eprec, rho, by_band);
   WH_HIP(hipGetLastError());
 }
 
 void prec_rho(int nc, int npw, int gstart, const int* l2i, const z32* g, long ld, const double* g2kin,
-              const double* eprec, double* rho, hipStream_t s) {
+              const double* eprec, double* rho, hipStream_t s, bool by_band) {
   if (nc <= 0) return;
-  k_prec_rho<float><<<nc, NT, 0, s>>>(npw, gstart, l2i, g, ld, g2kin, eprec, rho);
+  k_prec_rho<float><<<nc, NT, 0, s>>>(npw, gstart, l2i, g, ld, g2kin, eprec, rho, by_band);
   WH_HIP(hipGetLastError());
 }
 
@@ -366,29 +385,40 @@ void direction(int nc, int npw, const int* l2i, const double* beta, const z32* g
   WH_HIP(hipGetLastError());
 }
 
+template <typename V, bool RHO>
+static void cg_update_reg(int nc, int npw, int gstart, const int* l2i, const cplx<V>* h, cplx<V>* g,
+                          const cplx<V>* tc, cplx<V>* x, long ld, const RhoOut& r, hipStream_t s) {
+  // FP64 at 9 elements per thread keeps h and t only (g too would spill)
+  if (npw <= 4 * NTU)
+    k_cg_update_reg<V, 4, true, RHO><<<nc, NTU, 0, s>>>(npw, gstart, l2i, h, g, tc, x, ld, ld, r.g2kin, r.eprec,
+                                                         r.rho);
+  else
+    k_cg_update_reg<V, 9, (sizeof(V) == 4), RHO><<<nc, NTU, 0, s>>>(npw, gstart, l2i, h, g, tc, x, ld, ld, r.g2kin,
+                                                                    r.eprec, r.rho);
+}
+
 template <typename V>
 static void cg_update_any(int nc, int npw, int gstart, const int* l2i, const cplx<V>* h, cplx<V>* g,
-                          const cplx<V>* tc, cplx<V>* x, long ld, double* ac_out, hipStream_t s) {
+                          const cplx<V>* tc, cplx<V>* x, long ld, double* ac_out, const RhoOut* r, hipStream_t s) {
   if (nc <= 0) return;
-  // FP64 at 9 elements per thread keeps h and t only (g too would spill)
-  if (!ac_out && npw <= 4 * NTU) {
-    k_cg_update_reg<V, 4, true><<<nc, NTU, 0, s>>>(npw, gstart, l2i, h, g, tc, x, ld, ld);
-  } else if (!ac_out && npw <= 9 * NTU) {
-    k_cg_update_reg<V, 9, (sizeof(V) == 4)><<<nc, NTU, 0, s>>>(npw, gstart, l2i, h, g, tc, x, ld, ld);
+  if (!ac_out && npw <= 9 * NTU) {
+    if (r) cg_update_reg<V, true>(nc, npw, gstart, l2i, h, g, tc, x, ld, *r, s);
+    else cg_update_reg<V, false>(nc, npw, gstart, l2i, h, g, tc, x, ld, RhoOut{}, s);
   } else {
     k_cg_update<V><<<nc, NT, 0, s>>>(npw, gstart, l2i, h, g, tc, x, ld, ld, ac_out);
+    if (r) k_prec_rho<V><<<nc, NT, 0, s>>>(npw, gstart, l2i, g, ld, r->g2kin, r->eprec, r->rho, true);
   }
   WH_HIP(hipGetLastError());
 }
 
 void cg_update(int nc, int npw, int gstart, const int* l2i, const z64* h, z64* g, const z64* tc, z64* x, long ld,
-               double* ac_out, hipStream_t s) {
-  cg_update_any<double>(nc, npw, gstart, l2i, h, g, tc, x, ld, ac_out, s);
+               double* ac_out, hipStream_t s, const RhoOut* rho) {
+  cg_update_any<double>(nc, npw, gstart, l2i, h, g, tc, x, ld, ac_out, rho, s);
 }
 
 void cg_update(int nc, int npw, int gstart, const int* l2i, const z32* h, z32* g, const z32* tc, z32* x, long ld,
-               double* ac_out, hipStream_t s) {
-  cg_update_any<float>(nc, npw, gstart, l2i, h, g, tc, x, ld, ac_out, s);
+               double* ac_out, hipStream_t s, const RhoOut* rho) {
+  cg_update_any<float>(nc, npw, gstart, l2i, h, g, tc, x, ld, ac_out, rho, s);
 }
 
 void gather_sticks(int m, int npw, const int* perm, const z64* in, long ldin, z64* out, long ldout, const double* w,
