@@ -205,18 +205,20 @@ struct Plan {
     for (int i = 0; i < s; ++i) p *= radix(i);
     return p;
   }
+  // twiddles of pass s >= 1 are stored [k][j] (k < stride(s), 1 <= j < radix(s))
+  // from twoff(s) (the reversed plan's tables follow the forward plan's) with
+  // row length tws(s) = radix(s) - 1 rounded up to odd: the R - 1 roots one
+  // butterfly needs are adjacent, and neighbouring butterflies read an odd
+  // number of elements apart, so their reads fall on distinct LDS banks
+  static constexpr int tws(int s) { return (radix(s) - 1) | 1; }
   static constexpr int twtotal() {
     int o = 0;
-    for (int i = 1; i < nst; ++i) o += stride(i) * (radix(i) - 1);
+    for (int i = 1; i < nst; ++i) o += stride(i) * tws(i);
     return o;
   }
-  // twiddles of pass s >= 1 are stored [k][j] (k < stride(s), 1 <= j < radix(s))
-  // from twoff(s) (the reversed plan's tables follow the forward plan's): the
-  // R - 1 roots one butterfly needs are adjacent, and neighbouring butterflies
-  // read 16 (R - 1) bytes apart (no LDS bank clashes)
   static constexpr int twoff(int s) {
     int o = REV ? Plan<N, false>::twtotal() : 0;
-    for (int i = 1; i < s; ++i) o += stride(i) * (radix(i) - 1);
+    for (int i = 1; i < s; ++i) o += stride(i) * tws(i);
     return o;
   }
   static constexpr int twsize() {

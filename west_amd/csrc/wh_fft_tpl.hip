@@ -65,7 +65,7 @@ __device__ __forceinline__ void pass(int nlines, const cplx<T>* tw, Ld ld, St st
       const int i = bf - line * NBL;
       const int k = (P == 1) ? 0 : i % P;
       if (P > 1 && k != 0) {
-        const cplx<T>* wk = tw + PL::twoff(S) + k * (R - 1);
+        const cplx<T>* wk = tw + PL::twoff(S) + k * PL::tws(S);
 #pragma unroll
         for (int j = 1; j < R; ++j) {
           cplx<T> w = wk[j - 1];
@@ -125,7 +125,7 @@ __device__ __forceinline__ void load_tw_plan(cplx<T>* tw, const double* g) {
       const int k = t / (R - 1);
       const int j = t - k * (R - 1) + 1;
       const int m = (j * k * step) % N;
-      tw[PL::twoff(s) + t] = mk<T>(T(g[2 * m]), T(g[2 * m + 1]));
+      tw[PL::twoff(s) + k * PL::tws(s) + j - 1] = mk<T>(T(g[2 * m]), T(g[2 * m + 1]));
     }
   }
 }
@@ -171,7 +171,7 @@ __device__ __forceinline__ void pass_turn(int nlines, const cplx<T>* tw, Ld ld, 
       const int line = bf / NBL;
       const int i = bf - line * NBL;   // = k of the last pass (i < P)
       if (P > 1 && i != 0) {
-        const cplx<T>* wk = tw + PF::twoff(S) + i * (R - 1);
+        const cplx<T>* wk = tw + PF::twoff(S) + i * PF::tws(S);
 #pragma unroll
         for (int j = 1; j < R; ++j) {
           cplx<T> w = wk[j - 1];
