@@ -42,6 +42,8 @@ WORKLOADS = {
     "si216": ({"nrep": 3, "a": 10.26, "ecutwfc": 25.0, "sigma": 0.05, "seed": 2024}, 128),
     # batch of 24 uneven H2 clusters (2-24 molecules, 12-24 bohr cells, 16-64 PDEP each): Davidson wall time
     "c4": (None, None),
+    # Wstat Davidson for 256 PDEP on the si64 ground state (nvecx 1024), FP64 and mixed
+    "si64dav": ({"nrep": 2, "a": 10.26, "ecutwfc": 25.0, "sigma": 0.05, "seed": 2024}, 64),
 }
 
 
@@ -137,6 +139,9 @@ def main():
         f"gap {gs.et[gs.nbndval] - gs.et[gs.nbndval - 1] if len(gs.et) > gs.nbndval else float('nan'):.4f} Ry")
     sq = sqvc_init(b, gs.npwx)
     wstat.set_ground_state(gs, sqvc=sq)
+    if args.workload == "si64dav":
+        _davidson_si64(wstat, torch, args, system, gs)
+        return
     dvg_h = start_vectors(b, gs.npwx, npert, seed=7)
     dev = torch.device("cuda:0")
     dvg = torch.from_numpy(dvg_h).to(dev)
@@ -317,6 +322,48 @@ def _batch_c4(wstat, torch, args):
            "fp64": res["fp64"],
            "mixed_precision": dict(res["mixed"], mode=args.mixed_mode,
                                    ev_max_rel_err_vs_fp64_converged_setting=err)}
+    print(json.dumps(rec), flush=True)
+
+
+def _davidson_si64(wstat, torch, args, system, gs, npdep=256):
+    """Wstat Davidson for 256 PDEP of the 64-atom Si cell (n_pdep_times 4,
+    nvecx 1024), FP64 and mixed, with the default and the converged settings;
+    eigenvalue errors against the converged FP64 run."""
+    b = system.basis
+    res, evs = {}, {}
+    for mode in ("fp64", args.mixed_mode):
+        wstat.set_precision(mode)
+        key = "fp64" if mode == "fp64" else "mixed"
+        res[key] = {}
+        for name, setting in (("default", DAV_DEFAULT), ("converged_setting", dict(DAV_CONVERGED, n_pdep_maxiter=100))):
+            log(f"[bench] si64 davidson {mode} {name} ...")
+            torch.cuda.synchronize()
+            c0 = wstat.counters()
+            t = time.perf_counter()
+            ev, conv, _, recs = wstat.davidson_diago(npdep, n_pdep_times=4, want_vectors=False, **setting)
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t
+            c1 = wstat.counters()
+            evs[key, name] = ev
+            res[key][name] = {"wall_s": round(wall, 3), "iterations": len(recs), "converged": int(conv.sum()),
+                              "sternheimer_applications": c1[0] - c0[0]}
+            log(f"[bench] si64 davidson {mode} {name}: {wall:.1f}s, {len(recs)} iterations, "
+                f"{int(conv.sum())}/{npdep} converged")
+    wstat.set_precision("fp64")
+    ref = evs["fp64", "converged_setting"]
+    for key in ("fp64", "mixed"):
+        res[key]["default"]["ev_max_rel_err_vs_converged_fp64"] = _max_rel(evs[key, "default"], ref)
+    rec = {"metric": "Wstat Davidson wall time, 256 PDEP of the 64-atom Si cell",
+           "value": res["fp64"]["converged_setting"]["wall_s"], "unit": "s", "n_gpus": 1,
+           "higher_is_better": False, "dtype": "f64", "data": "synthetic (seeded Si supercell ground state)",
+           "config": {"workload": f"si64dav: {system.name}, npw={b.npw}, fft={'x'.join(map(str, b.nr))}, "
+                                  f"nbndval={gs.nbndval}, nkb={system.nkb}, {npdep} PDEP, n_pdep_times 4",
+                      "davidson_default": DAV_DEFAULT, "davidson_converged": dict(DAV_CONVERGED, n_pdep_maxiter=100)},
+           "fp64": res["fp64"],
+           "mixed_precision": dict(res["mixed"], mode=args.mixed_mode,
+                                   ev_max_rel_err_vs_fp64_converged_setting=_max_rel(
+                                       evs["mixed", "converged_setting"], ref)),
+           "ev_fp64_lowest": [round(float(x), 8) for x in ref[:4]]}
     print(json.dumps(rec), flush=True)
 
 

@@ -254,6 +254,10 @@ extern "C" int wh_davidson_diago(const wh_davidson_params* prm, double* ev_out, 
         r.time_iter = now_s() - tit;
       }
       ++nlog;
+      // the per-step report of the reference (output_ev_and_time, davidson_diago.f90:355)
+      if (tune_log())
+        std::fprintf(stderr, "[west_hip] davidson iter %d: notcnv %d, dfpt_dim %d, diago_dim %d, %.2f s\n",
+                     dav_iter, notcnv, dfpt_dim, diago_dim, now_s() - t0);
     };
     // rows [0, ncol) of X <- X * VR(0:n, 0:ncol) (refresh); uses tmp
     auto refresh = [&](z64* X, int n, int ncol) {
