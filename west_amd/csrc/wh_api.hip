@@ -218,6 +218,12 @@ void stern_apply(const z64* x, z64* t, const double* d_e, int m, double alpha) {
     z64* sa = c.w_sticks_a.as<z64>(static_cast<size_t>(chunk) * c.plan.sticks_elems());
     z64* sb = c.w_sticks_b.as<z64>(static_cast<size_t>(chunk) * c.plan.sticks_elems());
     ZfwdArgs za{ZFWD_PAIR_STERN, 1.0, c.d_g2kin, d_e, nullptr, 0};
+    // register-line z kernels where the grid has them (WH_ZREG=0: the LDS-pass ones)
+    static const bool zreg_env = [] {
+      const char* e = std::getenv("WH_ZREG");
+      return !(e && e[0] == '0');
+    }();
+    const bool zreg = zreg_env && zreg_available(c.plan.dev);
     for (int q0 = 0; q0 < npair; q0 += chunk) {
       const int nq = std::min(chunk, npair - q0);
       WH_TIMED("zinv", launch_zinv(c.plan, items + q0, nq, x, ld, sa, ZINV_PAIR, s));
@@ -373,9 +379,18 @@ void stern_apply32(const z32* x, z32* t, const double* d_e, int m, double alpha)
     z32* sb = c.w_sticks_b.as<z32>(static_cast<size_t>(chunk) * c.plan.sticks_elems());
     const float* v32 = static_cast<const float*>(c.m_vrs32.p);
     ZfwdArgs za{ZFWD_PAIR_STERN, 1.0, c.d_g2kin, d_e, nullptr, 0};
+    // register-line z kernels where the grid has them (WH_ZREG=0: the LDS-pass ones)
+    static const bool zreg_env = [] {
+      const char* e = std::getenv("WH_ZREG");
+      return !(e && e[0] == '0');
+    }();
+    const bool zreg = zreg_env && zreg_available(c.plan.dev);
     for (int q0 = 0; q0 < npair; q0 += chunk) {
       const int nq = std::min(chunk, npair - q0);
-      WH_TIMED("zinv32", K->zinv(c.plan.dev, items + q0, nq, x, l32, x, l32, sa, ZINV_PAIR, s));
+      if (zreg)
+        WH_TIMED("zinv32", zinv_reg(c.plan.dev, items + q0, nq, x, l32, x, l32, sa, ZINV_PAIR, s));
+      else
+        WH_TIMED("zinv32", K->zinv(c.plan.dev, items + q0, nq, x, l32, x, l32, sa, ZINV_PAIR, s));
       // variants 1..max_occ: LDS-pass kernel sized for that many workgroups
       // per CU; above: register-line kernels
       const int nvar = K->max_occ + plane_reg_variants(c.plan.dev, true);
@@ -389,7 +404,10 @@ void stern_apply32(const z32* x, z32* t, const double* d_e, int m, double alpha)
                                                       1.0 / c.nnr, s);
                                    },
                                    s));
-      WH_TIMED("zfwd32", K->zfwd(c.plan.dev, items + q0, nq, sb, x, l32, t, l32, za, s));
+      if (zreg)
+        WH_TIMED("zfwd32", zfwd_reg(c.plan.dev, items + q0, nq, sb, x, l32, t, l32, za, s));
+      else
+        WH_TIMED("zfwd32", K->zfwd(c.plan.dev, items + q0, nq, sb, x, l32, t, l32, za, s));
       WH_HIP(hipGetLastError());
     }
   }

@@ -39,6 +39,8 @@ struct StickPlanDev {
   int nct;        // all columns (2*ncs - 1)
   int imax;       // max |i| over columns
   int nrows;      // 2*imax + 1
+  int kext;       // max |k| over the sphere
+  int zblk_pts;   // most points in kZregCols consecutive stored columns
   const int* col_off;   // (ncs+1)
   const int* col_kmin;  // (ncs)
   const int* col_pos;   // (nct) (i+imax)*n2 + (j mod n2)
@@ -49,6 +51,9 @@ struct StickPlanDev {
   const double* tw2;
   const double* tw3;
 };
+
+// stored columns per workgroup of the register-line z kernels
+constexpr int kZregCols = 128;
 
 struct TplKernels;
 struct TplKernels32;

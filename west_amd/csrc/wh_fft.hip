@@ -591,6 +591,11 @@ void StickPlan::build(int nr1, int nr2, int nr3, int npw, const int* nl, const i
   StickPlanDev& d = dev;
   d.n1 = nr1; d.n2 = nr2; d.n3 = nr3;
   d.npw = npw; d.ncs = ncs; d.nct = nct; d.imax = imax; d.nrows = nrows;
+  d.kext = 0;
+  for (int s = 0; s < npw; ++s) d.kext = std::max(d.kext, std::abs(pt_k[s]));
+  d.zblk_pts = 0;
+  for (int c0 = 0; c0 < ncs; c0 += kZregCols)
+    d.zblk_pts = std::max(d.zblk_pts, col_off[std::min(c0 + kZregCols, ncs)] - col_off[c0]);
   d.ax1 = make_axis(nr1);
   d.ax2 = make_axis(nr2);
   d.ax3 = make_axis(nr3);

@@ -19,4 +19,13 @@ void plane_reg_vloc(int v, const StickPlanDev& P, const FftItem* items, int n, c
 void plane_reg_vloc(int v, const StickPlanDev& P, const FftItem* items, int n, const z32* sin, z32* sout,
                     const float* vpot, long ldpot, double scale, hipStream_t s);
 
+// Register-line z transforms of the FP32 operator (same contract as the
+// size-specialised zinv/zfwd; zfwd in the pair modes only), available when
+// the grid is cubic of a compiled size and the sphere spans |k| <= N/4.
+bool zreg_available(const StickPlanDev& P);
+void zinv_reg(const StickPlanDev& P, const FftItem* items, int n, const z32* psiA, long ldA, const z32* psiB,
+              long ldB, z32* sticks, int mode, hipStream_t s);
+void zfwd_reg(const StickPlanDev& P, const FftItem* items, int n, const z32* sticks, const z32* in, long ldin,
+              z32* out, long ldout, const ZfwdArgs& A, hipStream_t s);
+
 }  // namespace wh

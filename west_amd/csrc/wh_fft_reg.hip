@@ -230,6 +230,194 @@ void launch_n(int var, const StickPlanDev& P, const FftItem* it, int n, const cp
   }
 }
 
+// ---------------------------------------------------------------------------
+// z transforms with register-resident lines (FP32 operator).  A workgroup
+// takes kZregCols stored columns of one item: thread t < kZregCols holds the
+// a3 line of stored column c0 + t, thread kZregCols + t the line of its mirror
+// (none for column 0, whose line holds both halves).  Same layout and
+// packing as k_zinv_t / k_zfwd_t of wh_fft_tpl.hip.
+// ---------------------------------------------------------------------------
+constexpr int ZNT = 2 * kZregCols;
+
+// sphere -> sticks.  The block's points are staged in LDS with coalesced
+// loads; each thread then builds its line at compile-time positions (the
+// line's nonzeros lie at |k| <= IMX) and transforms it in registers.
+template <int N>
+__global__ __launch_bounds__(ZNT) __attribute__((amdgpu_waves_per_eu(2))) void k_zinv_reg(
+    StickPlanDev P, const FftItem* items, const z32* psiA, long ldA, const z32* psiB, long ldB, z32* sticks, int mode) {
+  constexpr int KX = imx<N>();
+  extern __shared__ char smem_raw[];
+  z32* pa = reinterpret_cast<z32*>(smem_raw);
+  z32* pb = pa + P.zblk_pts;
+  const int it = blockIdx.y;
+  const int c0 = blockIdx.x * kZregCols;
+  const int c1 = min(c0 + kZregCols, P.ncs);
+  const int S0 = P.col_off[c0];
+  const int npt = P.col_off[c1] - S0;
+  const FftItem item = items[it];
+  const z32* A = psiA + static_cast<long>(item.a) * ldA + S0;
+  const z32* B = item.b >= 0 ? psiB + static_cast<long>(item.b) * ldB + S0 : nullptr;
+  {
+    // all of a thread's loads in flight at once (a column block holds at most
+    // kZregCols (2 IMX + 1) points)
+    constexpr int U = (kZregCols * (2 * KX + 1) + ZNT - 1) / ZNT;
+    z32 ra[U], rb[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int t = threadIdx.x + u * ZNT;
+      if (t < npt) {
+        ra[u] = A[t];
+        rb[u] = B ? B[t] : mk<float>(0.f, 0.f);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int t = threadIdx.x + u * ZNT;
+      if (t < npt) {
+        pa[t] = ra[u];
+        pb[t] = rb[u];
+      }
+    }
+  }
+  __syncthreads();
+  const bool own = threadIdx.x < kZregCols;
+  const int c = c0 + (own ? threadIdx.x : threadIdx.x - kZregCols);
+  if (c >= c1 || (!own && c == 0)) return;   // no barrier below
+  const int s0 = P.col_off[c] - S0;
+  const int kmin = P.col_kmin[c];
+  const int kmax = kmin + (P.col_off[c + 1] - P.col_off[c]) - 1;
+  const bool gets_mirror = !own || c == 0;   // this line holds mirror values
+  z32 v[N];
+  sfor<N>([&](auto Z) {
+    constexpr int z = decltype(Z)::value;
+    if constexpr (z > KX && z < N - KX) {
+      v[z] = mk<float>(0.f, 0.f);
+    } else {
+      constexpr int ko = z <= KX ? z : z - N;    // point whose own value lands at z
+      constexpr int km = z <= KX ? -z : N - z;   // point whose mirror value lands at z
+      const bool okm = gets_mirror && km >= kmin && km <= kmax;
+      const bool oko = own && ko >= kmin && ko <= kmax;
+      z32 val = mk<float>(0.f, 0.f);
+      if (okm || oko) {
+        const int q = s0 + (okm ? km : ko) - kmin;
+        const z32 a1 = pa[q];
+        const z32 a2 = pb[q];
+        if (!okm) val = mk<float>(a1.x - a2.y, a1.y + a2.x);        // b(nl) = a1 + i a2
+        else if (mode == ZINV_PAIR_REALG0 && c == 0 && km == 0) val = mk<float>(a1.x, a2.x);
+        else val = mk<float>(a1.x + a2.y, a2.x - a1.y);              // b(nlm) = conj(a1 - i a2), last
+      }
+      v[z] = val;
+    }
+  });
+  fft_line<float, N, +1>(v);
+  z32* out = sticks + static_cast<long>(it) * N * P.nct + (own ? c : c + P.ncs - 1);
+  sfor<N>([&](auto Z) { out[static_cast<long>(decltype(Z)::value) * P.nct] = v[decltype(Z)::value]; });
+}
+
+// sticks -> sphere (pair modes).  Each line is loaded (coalesced along the
+// stored columns), transformed in registers, and its |k| <= IMX outputs go to
+// LDS; the fp/fm unpack and epilogue then run over the block's points.
+template <int N>
+__global__ __launch_bounds__(ZNT) __attribute__((amdgpu_waves_per_eu(2))) void k_zfwd_reg(
+    StickPlanDev P, const FftItem* items, const z32* sticks, const z32* in, long ldin, z32* out, long ldout,
+    ZfwdArgs A) {
+  constexpr int KX = imx<N>();
+  constexpr int NJ = 2 * KX + 1;   // kept positions per line: j = k (k >= 0), KX - k (k < 0)
+  extern __shared__ char smem_raw[];
+  z32* X = reinterpret_cast<z32*>(smem_raw);   // [2 kZregCols lines][NJ | 1]
+  constexpr int XS = NJ | 1;
+  const int it = blockIdx.y;
+  const int c0 = blockIdx.x * kZregCols;
+  const int c1 = min(c0 + kZregCols, P.ncs);
+  const FftItem item = items[it];
+  {
+    const bool own = threadIdx.x < kZregCols;
+    const int lc = own ? threadIdx.x : threadIdx.x - kZregCols;
+    const int c = c0 + lc;
+    if (c < c1 && (own || c != 0)) {
+      const z32* src = sticks + static_cast<long>(it) * N * P.nct + (own ? c : c + P.ncs - 1);
+      z32 v[N];
+      sfor<N>([&](auto Z) { v[decltype(Z)::value] = src[static_cast<long>(decltype(Z)::value) * P.nct]; });
+      fft_line<float, N, -1>(v);
+      z32* row = X + (2 * lc + (own ? 0 : 1)) * XS;
+      sfor<N>([&](auto Z) {
+        constexpr int z = decltype(Z)::value;
+        if constexpr (z <= KX) row[z] = v[z];
+        else if constexpr (z >= N - KX) row[KX + N - z] = v[z];
+      });
+    }
+  }
+  __syncthreads();
+  const int s0 = P.col_off[c0];
+  const int s1 = P.col_off[c1];
+  z32* oa = out + static_cast<long>(item.a) * ldout;
+  z32* ob = item.b >= 0 ? out + static_cast<long>(item.b) * ldout : nullptr;
+  // the block's points, U per thread with their loads issued together
+  constexpr int U = (kZregCols * (2 * KX + 1) + ZNT - 1) / ZNT;
+  const bool stern = A.mode == ZFWD_PAIR_STERN;
+  const z32* ina = in + static_cast<long>(item.a) * ldin;
+  const z32* inb = item.b >= 0 ? in + static_cast<long>(item.b) * ldin : nullptr;
+  const double ea = stern ? A.e[item.a] : 0.0;
+  const double eb = (stern && ob) ? A.e[item.b] : 0.0;
+  int pc[U], pk[U];
+  double g2[U];
+  z32 ia[U], ib[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int s = s0 + threadIdx.x + u * ZNT;
+    if (s < s1) {
+      pc[u] = P.pt_col[s];
+      pk[u] = P.pt_k[s];
+      if (stern) {
+        g2[u] = A.g2kin[s];
+        ia[u] = ina[s];
+        if (inb) ib[u] = inb[s];
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int s = s0 + threadIdx.x + u * ZNT;
+    if (s < s1) {
+      const int lc = pc[u] - c0;
+      const int k = pk[u];
+      const int jz = k >= 0 ? k : KX - k;    // position of k in its own line
+      const int jm = k <= 0 ? -k : KX + k;   // position of -k in the mirror line
+      const z32 F = X[(2 * lc) * XS + jz];
+      const int ml = (c0 + lc == 0) ? 0 : 2 * lc + 1;
+      const z32 Fm = X[ml * XS + jm];
+      const z32 fp = cscale(F + Fm, 0.5f);
+      const z32 fm = cscale(F - Fm, 0.5f);
+      const z32 b1 = mk<float>(fp.x, fm.y);
+      const z32 b2 = mk<float>(fp.y, -fm.x);
+      if (!stern) {
+        oa[s] = cscale(b1, float(A.scale));
+        if (ob) ob[s] = cscale(b2, float(A.scale));
+      } else {
+        oa[s] = b1 + cscale(ia[u], float(g2[u] - ea));
+        if (ob) ob[s] = b2 + cscale(ib[u], float(g2[u] - eb));
+      }
+    }
+  }
+}
+
+template <int N>
+void zinv_n(const StickPlanDev& P, const FftItem* it, int n, const z32* a, long lda, const z32* b, long ldb,
+            z32* st, int mode, hipStream_t s) {
+  const size_t sm = 2 * sizeof(z32) * static_cast<size_t>(P.zblk_pts);
+  allow_smem_reg(k_zinv_reg<N>, sm);
+  k_zinv_reg<N><<<dim3(ceil_div(P.ncs, kZregCols), n), ZNT, sm, s>>>(P, it, a, lda, b, ldb, st, mode);
+}
+
+template <int N>
+void zfwd_n(const StickPlanDev& P, const FftItem* it, int n, const z32* st, const z32* in, long ldin, z32* out,
+            long ldout, const ZfwdArgs& A, hipStream_t s) {
+  constexpr int XS = (2 * imx<N>() + 1) | 1;
+  const size_t sm = sizeof(z32) * 2 * kZregCols * XS;
+  allow_smem_reg(k_zfwd_reg<N>, sm);
+  k_zfwd_reg<N><<<dim3(ceil_div(P.ncs, kZregCols), n), ZNT, sm, s>>>(P, it, st, in, ldin, out, ldout, A);
+}
+
 // compiled line lengths (FP32 only: an FP64 line of these lengths does not fit
 // in the registers of one thread)
 #define WH_REG_SIZES(X) X(18) X(20) X(24) X(30) X(32) X(36) X(40) X(45) X(48) X(50) X(54) X(60) X(64) X(72) X(75) X(80)
@@ -258,7 +446,51 @@ void dispatch32(int var, const StickPlanDev& P, const FftItem* it, int n, const 
   }
 }
 
+void dispatch_zinv(const StickPlanDev& P, const FftItem* it, int n, const z32* a, long lda, const z32* b, long ldb,
+                   z32* st, int mode, hipStream_t s) {
+  switch (P.n3) {
+#define WH_REG_CASE(NN) \
+  case NN:              \
+    return zinv_n<NN>(P, it, n, a, lda, b, ldb, st, mode, s);
+    WH_REG_SIZES(WH_REG_CASE)
+#undef WH_REG_CASE
+    default: WH_REQUIRE(false, "zinv_reg: no kernel for this grid");
+  }
+}
+
+void dispatch_zfwd(const StickPlanDev& P, const FftItem* it, int n, const z32* st, const z32* in, long ldin, z32* out,
+                   long ldout, const ZfwdArgs& A, hipStream_t s) {
+  switch (P.n3) {
+#define WH_REG_CASE(NN) \
+  case NN:              \
+    return zfwd_n<NN>(P, it, n, st, in, ldin, out, ldout, A, s);
+    WH_REG_SIZES(WH_REG_CASE)
+#undef WH_REG_CASE
+    default: WH_REQUIRE(false, "zfwd_reg: no kernel for this grid");
+  }
+}
+
 }  // namespace
+
+bool zreg_available(const StickPlanDev& P) {
+  return P.n1 == P.n2 && P.n1 == P.n3 && compiled(P.n3, true) && P.kext <= (P.n3 + 3) / 4 &&
+         2 * sizeof(z32) * static_cast<size_t>(P.zblk_pts) <= 80u * 1024u;
+}
+
+void zinv_reg(const StickPlanDev& P, const FftItem* items, int n, const z32* psiA, long ldA, const z32* psiB,
+              long ldB, z32* sticks, int mode, hipStream_t s) {
+  WH_REQUIRE(zreg_available(P), "zinv_reg: not available for this grid");
+  if (n <= 0) return;
+  dispatch_zinv(P, items, n, psiA, ldA, psiB, ldB, sticks, mode, s);
+}
+
+void zfwd_reg(const StickPlanDev& P, const FftItem* items, int n, const z32* sticks, const z32* in, long ldin,
+              z32* out, long ldout, const ZfwdArgs& A, hipStream_t s) {
+  WH_REQUIRE(zreg_available(P), "zfwd_reg: not available for this grid");
+  WH_REQUIRE(A.mode == ZFWD_PAIR_STORE || A.mode == ZFWD_PAIR_STERN, "zfwd_reg: pair modes only");
+  if (n <= 0) return;
+  dispatch_zfwd(P, items, n, sticks, in, ldin, out, ldout, A, s);
+}
 
 int plane_reg_variants(const StickPlanDev& P, bool fp32) {
   if (P.n1 != P.n2 || P.n1 != P.n3 || !compiled(P.n1, fp32)) return 0;
