@@ -35,11 +35,11 @@ __device__ __forceinline__ void sfor(F&& f) {
   sfor_seq(std::make_integer_sequence<int, N>{}, static_cast<F&&>(f));
 }
 
-// In-register transform of v[0..N): the Stockham passes of Plan<N> with every
-// index and twiddle exp(DIR 2 pi i j k / (P R)) a compile-time constant.
-template <typename T, int N, int DIR, int S = 0>
+// In-register transform of v[0..N): the Stockham passes of Plan<N, REV> with
+// every index and twiddle exp(DIR 2 pi i j k / (P R)) a compile-time constant.
+template <typename T, int N, int DIR, bool REV = false, int S = 0>
 __device__ __forceinline__ void fft_line(cplx<T>* v) {
-  using PL = Plan<N, false>;
+  using PL = Plan<N, REV>;
   if constexpr (S < PL::nst) {
     constexpr int R = PL::radix(S);
     constexpr int P = PL::stride(S);
@@ -66,7 +66,7 @@ __device__ __forceinline__ void fft_line(cplx<T>* v) {
       });
     });
     sfor<N>([&](auto I) { v[decltype(I)::value] = w[decltype(I)::value]; });
-    fft_line<T, N, DIR, S + 1>(v);
+    fft_line<T, N, DIR, REV, S + 1>(v);
   }
 }
 
@@ -158,7 +158,9 @@ __global__ __launch_bounds__((nt_reg<N, PZ>())) __attribute__((amdgpu_waves_per_
         constexpr int x = decltype(X)::value;
         v[x] = cscale(v[x], T(vr[x]) * sc);
       });
-      fft_line<T, N, -1>(v);
+      // reversed radix order: of its outputs only |i| <= IMX are kept, which
+      // prunes more of its last pass (and spills nothing at N = 72)
+      fft_line<T, N, -1, true>(v);
       sfor<N>([&](auto X) {
         constexpr int x = decltype(X)::value;
         if constexpr (x <= IMX) {
