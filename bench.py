@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 # Measured HBM bytes per transform item (band pair) of the plane kernels on the
 # si64 workload: rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a streaming kernel
 # of known size (profiles/plane_vloc_si64_hbm_traffic.md).
-TRAFFIC_PER_ITEM = {"si64": {"plane_vloc": 985465 + 974592, "plane_vloc32": 499557 + 564293}}
+TRAFFIC_PER_ITEM = {"si64": {"plane_vloc": 1024257 + 974592, "plane_vloc32": 491706 + 487296}}
 FP64_PEAK_TFLOPS = 78.6
 
 WORKLOADS = {
@@ -215,7 +215,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision != "mixed" else "f32+bf16x3, f64 refinement",
+        "dtype": "f64" if args.precision != "mixed" else
+                 f"f32+bf16x{3 if args.mixed_mode == 'mixed' else 6}, f64 refinement",
         "data": "synthetic (seeded Si supercell ground state computed on the device; seeded smooth perturbations)",
         "config": {"workload": f"{args.workload}: {system.name}, npw={b.npw}, fft={n1}x{n2}x{n3}, "
                                f"nbndval={nv}, nkb={system.nkb}, {npert} perturbations/batch",
