@@ -139,12 +139,20 @@ __device__ __forceinline__ void store_b(double (*Bs)[BN + PAD], const double (&r
   for (int q = 0; q < 8; ++q) Bs[kq + q][n] = rb[q];
 }
 
-// blockIdx.z = split index; k range [kz*kchunk, min(K, (kz+1)*kchunk)).
-// split > 1: raw partial sums go to C + kz*pstride (alpha, beta ignored).
+// Tiles (bx, by, kz): gx x gy output tiles times gz K chunks, k range
+// [kz*kchunk, min(K, (kz+1)*kchunk)); gz > 1: raw partial sums go to
+// C + kz*pstride (alpha, beta ignored).  The grid is 1D and its blocks are
+// dispatched round-robin over the 8 XCDs (block L on XCD L % 8); tile
+// W = (L % 8) * ceil(tiles / 8) + L / 8 puts consecutive tiles, which share
+// their B panel (same by, kz) or A panel (same bx, kz), on one XCD and its L2.
+constexpr int kXcds = 8;
+__host__ __device__ inline int xcd_blocks(int tiles) { return (tiles + kXcds - 1) / kXcds * kXcds; }
+
 template <bool TA>
 __global__ __launch_bounds__(NT, 2) void k_dgemm(int M, int N, int K, int kchunk, double alpha,
                                               const double* __restrict__ A, long lda, const double* __restrict__ B,
-                                              long ldb, double beta, double* __restrict__ C, long ldc, long pstride) {
+                                              long ldb, double beta, double* __restrict__ C, long ldc, long pstride,
+                                              int gx, int gy, int gz) {
   __shared__ double As[2][BK][BM + PAD];
   __shared__ double Bs[2][BK][BN + PAD];
   const int tid = threadIdx.x;
@@ -152,9 +160,12 @@ __global__ __launch_bounds__(NT, 2) void k_dgemm(int M, int N, int K, int kchunk
   const int wave = tid >> 6;
   const int wm = wave & 1;
   const int wn = wave >> 1;
-  const int m0 = blockIdx.x * BM;
-  const int n0 = blockIdx.y * BN;
-  const int kz = blockIdx.z;
+  const int tiles = gx * gy * gz;
+  const int W = static_cast<int>(blockIdx.x % kXcds) * (xcd_blocks(tiles) / kXcds) + static_cast<int>(blockIdx.x / kXcds);
+  if (W >= tiles) return;   // uniform per workgroup
+  const int m0 = (W % gx) * BM;
+  const int n0 = ((W / gx) % gy) * BN;
+  const int kz = W / (gx * gy);
   const int kbeg = kz * kchunk;
   const int kend = min(K, kbeg + kchunk);
   v4d acc[4][4];
@@ -196,7 +207,7 @@ __global__ __launch_bounds__(NT, 2) void k_dgemm(int M, int N, int K, int kchunk
     }
     __syncthreads();
   }
-  const bool partial = gridDim.z > 1;
+  const bool partial = gz > 1;
   double* Cz = C + (partial ? static_cast<long>(kz) * pstride : 0L);
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -283,12 +294,13 @@ void dgemm(bool transa, int M, int N, int K, double alpha, const double* A, long
                "rocblas_dgemm failed");
     return;
   }
-  dim3 grid(ceil_div(M, BM), ceil_div(N, BN), 1);
+  const int gx = ceil_div(M, BM), gy = ceil_div(N, BN);
+  const int nb = xcd_blocks(gx * gy);
   const int kc = std::max(K, 1);
   if (transa)
-    k_dgemm<true><<<grid, NT, 0, s>>>(M, N, std::max(K, 0), kc, alpha, A, lda, B, ldb, beta, C, ldc, 0);
+    k_dgemm<true><<<nb, NT, 0, s>>>(M, N, std::max(K, 0), kc, alpha, A, lda, B, ldb, beta, C, ldc, 0, gx, gy, 1);
   else
-    k_dgemm<false><<<grid, NT, 0, s>>>(M, N, std::max(K, 0), kc, alpha, A, lda, B, ldb, beta, C, ldc, 0);
+    k_dgemm<false><<<nb, NT, 0, s>>>(M, N, std::max(K, 0), kc, alpha, A, lda, B, ldb, beta, C, ldc, 0, gx, gy, 1);
   WH_HIP(hipGetLastError());
 }
 
@@ -305,8 +317,9 @@ static void dgemm_bra_own(int M, int N, int K, double alpha, const double* A, lo
   split = ceil_div(std::max(K, 1), kchunk);
   const long pstride = static_cast<long>(ldc) * N;
   double* P = partials().get(static_cast<size_t>(split) * pstride);
-  dim3 grid(ceil_div(M, BM), ceil_div(N, BN), split);
-  k_dgemm<true><<<grid, NT, 0, s>>>(M, N, K, kchunk, 1.0, A, lda, B, ldb, 0.0, P, ldc, pstride);
+  const int gx = ceil_div(M, BM), gy = ceil_div(N, BN);
+  k_dgemm<true><<<xcd_blocks(gx * gy * split), NT, 0, s>>>(M, N, K, kchunk, 1.0, A, lda, B, ldb, 0.0, P, ldc,
+                                                           pstride, gx, gy, split);
   WH_HIP(hipGetLastError());
   const long n = static_cast<long>(M) * N;
   k_reduce_g0<<<ceil_div(n, 256), 256, 0, s>>>(M, N, split, P, pstride, alpha, g0, A, lda, B, ldb, scale, C, ldc);
