@@ -279,9 +279,11 @@ int linsolve_dev(int m, z64* b, z64* x, const std::vector<double>& e_col, const 
   int nc = m;
   const int itmax = maxiter > 0 ? maxiter : c.n_dfpt_maxiter;
   for (int iter = 1; iter <= itmax; ++iter) {
-    WH_HIP(hipMemcpyAsync(d_l2i, l2i.data(), sizeof(int) * nc, hipMemcpyHostToDevice, s));
-    if (iter == 1)
+    // (from iteration 2 on, d_l2i already holds this list: it was sent with beta)
+    if (iter == 1) {
+      WH_HIP(hipMemcpyAsync(d_l2i, l2i.data(), sizeof(int) * nc, hipMemcpyHostToDevice, s));
       WH_TIMED("cg_prec", prec_rho(nc, npw, c.gstart, d_l2i, g, ld, c.d_g2kin, d_eprec_col, d_rhob, s, true));
+    }
     WH_HIP(hipMemcpyAsync(rho_c.data(), d_rhob, sizeof(double) * m, hipMemcpyDeviceToHost, s));
     WH_HIP(hipStreamSynchronize(s));
     ++c.cg_iters;
@@ -308,9 +310,10 @@ int linsolve_dev(int m, z64* b, z64* x, const std::vector<double>& e_col, const 
     WH_TIMED("cg_update", cg_update(nc, npw, c.gstart, d_l2i, h, g, tc, x, ld, nullptr, s, &rho_out));
     std::swap(h, hold);   // hold <- h  (linsolve 319)
     for (int lb = 0; lb < nc; ++lb) rhoold[l2i[lb]] = rho[l2i[lb]];
-    // keep the staging vectors alive until the copies above have been consumed
-    WH_HIP(hipStreamSynchronize(s));
+    // the host lists sent above are next modified after the synchronisation
+    // that follows the next iteration's rho read, which orders those copies
   }
+  WH_HIP(hipStreamSynchronize(s));
   if (ierr_col) {
     ierr_col->assign(m, 0);
     for (int lb = 0; lb < nc; ++lb) (*ierr_col)[l2i[lb]] = 1;
@@ -578,8 +581,10 @@ int linsolve_mixed(int m, z64* b, z64* x, const std::vector<double>& e_col, cons
     WH_TIMED("cg_update32", cg_update(nc, npw, c.gstart, d_l2i, h, g, tc, dx, l32, nullptr, s, &rho_out));
     std::swap(h, hold);
     for (int lb = 0; lb < nc; ++lb) rhoold[l2i[lb]] = rho[l2i[lb]];
-    WH_HIP(hipStreamSynchronize(s));
+    // the host lists sent above are next modified after the synchronisation
+    // that follows the next iteration's rho read, which orders those copies
   }
+  WH_HIP(hipStreamSynchronize(s));
   // bands stopped by n_dfpt_maxiter keep their last FP32 increment
   if (!act.empty()) {
     WH_HIP(hipMemcpyAsync(d_u2i, act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice, s));

@@ -230,6 +230,49 @@ __global__ __launch_bounds__(NT, 2) void k_dgemm(int M, int N, int K, int kchunk
     }
 }
 
+// Bra product with few rows (M <= MB, e.g. the bands of a small molecule with
+// no projectors): one workgroup per (column j, K chunk z) streams B's column
+// once and keeps the M running sums per thread (A's rows come through the
+// caches); partial sums P_z(i, j) at P + z * pstride + j * ldc + i.
+constexpr int NTS = 256;
+template <int MB>
+__global__ __launch_bounds__(NTS) void k_bra_skinny(int M, int K, int kchunk, const double* __restrict__ A, long lda,
+                                                   const double* __restrict__ B, long ldb, double* __restrict__ P,
+                                                   long ldc, long pstride) {
+  __shared__ double red[MB][NTS / 64];
+  const int j = blockIdx.x;
+  const int z = blockIdx.y;
+  const int kb = z * kchunk;
+  const int ke = min(K, kb + kchunk);
+  const double* Bj = B + static_cast<long>(j) * ldb;
+  double acc[MB];
+#pragma unroll
+  for (int i = 0; i < MB; ++i) acc[i] = 0.0;
+  for (int k = kb + threadIdx.x; k < ke; k += NTS) {
+    const double b = Bj[k];
+#pragma unroll
+    for (int i = 0; i < MB; ++i)
+      if (i < M) acc[i] += A[static_cast<long>(i) * lda + k] * b;
+  }
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < MB; ++i) {
+    if (i < M) {
+      double v = acc[i];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      if (lane == 0) red[i][w] = v;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < M) {
+    double v = 0.0;
+#pragma unroll
+    for (int q = 0; q < NTS / 64; ++q) v += red[threadIdx.x][q];
+    P[static_cast<long>(z) * pstride + static_cast<long>(j) * ldc + threadIdx.x] = v;
+  }
+}
+
 // C(i, j) = s_i * (alpha * sum_z P_z(i, j) - g0 * a(1, i) * b(1, j))
 //   P_z: split partials (ldc, stride pstride); a(1, i) = A[i*lda], b(1, j) = B[j*ldb]
 __global__ void k_reduce_g0(int M, int N, int nsplit, const double* P, long pstride, double alpha, double g0,
@@ -326,6 +369,24 @@ static void dgemm_bra_own(int M, int N, int K, double alpha, const double* A, lo
   WH_HIP(hipGetLastError());
 }
 
+static void dgemm_bra_skinny(int M, int N, int K, double alpha, const double* A, long lda, const double* B, long ldb,
+                             double g0, const double* scale, double* C, long ldc, hipStream_t s) {
+  // about 2048 workgroups, K chunks of at least 1024
+  int split = std::max(1, std::min(ceil_div(2048, N), K / 1024));
+  const int kchunk = ceil_div(std::max(K, 1), split);
+  split = ceil_div(std::max(K, 1), kchunk);
+  const long pstride = static_cast<long>(ldc) * N;
+  double* P = partials().get(static_cast<size_t>(split) * pstride);
+  dim3 grid(N, split);
+  if (M <= 8) k_bra_skinny<8><<<grid, NTS, 0, s>>>(M, K, kchunk, A, lda, B, ldb, P, ldc, pstride);
+  else if (M <= 16) k_bra_skinny<16><<<grid, NTS, 0, s>>>(M, K, kchunk, A, lda, B, ldb, P, ldc, pstride);
+  else k_bra_skinny<32><<<grid, NTS, 0, s>>>(M, K, kchunk, A, lda, B, ldb, P, ldc, pstride);
+  WH_HIP(hipGetLastError());
+  const long n = static_cast<long>(M) * N;
+  k_reduce_g0<<<ceil_div(n, 256), 256, 0, s>>>(M, N, split, P, pstride, alpha, g0, A, lda, B, ldb, scale, C, ldc);
+  WH_HIP(hipGetLastError());
+}
+
 static void dgemm_bra_lib(int M, int N, int K, double alpha, const double* A, long lda, const double* B, long ldb,
                           double g0, const double* scale, double* C, long ldc, hipStream_t s) {
   const double zero = 0.0;
@@ -340,6 +401,8 @@ void dgemm_bra(int M, int N, int K, double alpha, const double* A, long lda, con
   if (M <= 0 || N <= 0) return;
   const Backend be = backend();
   if (be == Backend::Rocblas && K > 0) return dgemm_bra_lib(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s);
+  // a few rows: the matrix-core tiles (128 rows) would be almost empty
+  if (M <= 32 && K > 0) return dgemm_bra_skinny(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s);
   if (be == Backend::Own || K <= 0) return dgemm_bra_own(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s);
   // variants: 0..2 the kernel with its K split aimed at 512, 1024, 2048 workgroups; 3 rocBLAS
   static std::map<std::tuple<int, int, int>, int> chosen;   // (M, K, log2 N) -> variant
