@@ -273,6 +273,27 @@ __global__ __launch_bounds__(NTS) void k_bra_skinny(int M, int K, int kchunk, co
   }
 }
 
+// C = alpha A B + beta C (column major, A: M x K, B: K x N) for small K
+// (the ket product of a molecule with a few bands and no projectors): one
+// thread per element of C, B's column in LDS.
+template <int KB>
+__global__ __launch_bounds__(NTS) void k_ket_skinny(int M, int K, double alpha, const double* __restrict__ A, long lda,
+                                                   const double* __restrict__ B, long ldb, double beta,
+                                                   double* __restrict__ C, long ldc) {
+  __shared__ double bj[KB];
+  const int j = blockIdx.y;
+  if (threadIdx.x < K) bj[threadIdx.x] = B[static_cast<long>(j) * ldb + threadIdx.x];
+  __syncthreads();
+  const int r = blockIdx.x * NTS + threadIdx.x;
+  if (r >= M) return;
+  double acc = 0.0;
+#pragma unroll
+  for (int i = 0; i < KB; ++i)
+    if (i < K) acc += A[static_cast<long>(i) * lda + r] * bj[i];
+  double* c = C + static_cast<long>(j) * ldc + r;
+  *c = (beta == 0.0) ? alpha * acc : alpha * acc + beta * (*c);
+}
+
 // C(i, j) = s_i * (alpha * sum_z P_z(i, j) - g0 * a(1, i) * b(1, j))
 //   P_z: split partials (ldc, stride pstride); a(1, i) = A[i*lda], b(1, j) = B[j*ldb]
 __global__ void k_reduce_g0(int M, int N, int nsplit, const double* P, long pstride, double alpha, double g0,
@@ -330,6 +351,15 @@ Scratch& partials() {
 void dgemm(bool transa, int M, int N, int K, double alpha, const double* A, long lda, const double* B, long ldb,
            double beta, double* C, long ldc, hipStream_t s) {
   if (M <= 0 || N <= 0) return;
+  if (!transa && K > 0 && K <= 32 && backend() != Backend::Rocblas) {
+    // inner dimension of a few bands: a streaming kernel (the matrix-core tiles would be nearly empty)
+    dim3 grid(ceil_div(M, NTS), N);
+    if (K <= 8) k_ket_skinny<8><<<grid, NTS, 0, s>>>(M, K, alpha, A, lda, B, ldb, beta, C, ldc);
+    else if (K <= 16) k_ket_skinny<16><<<grid, NTS, 0, s>>>(M, K, alpha, A, lda, B, ldb, beta, C, ldc);
+    else k_ket_skinny<32><<<grid, NTS, 0, s>>>(M, K, alpha, A, lda, B, ldb, beta, C, ldc);
+    WH_HIP(hipGetLastError());
+    return;
+  }
   if (backend() != Backend::Own && K > 0) {
     WH_REQUIRE(rocblas_dgemm(blas(s), transa ? rocblas_operation_transpose : rocblas_operation_none,
                              rocblas_operation_none, M, N, K, &alpha, A, lda, B, ldb, &beta, C, ldc) ==
