@@ -68,8 +68,10 @@ static void free_context(Context* cp) {
                     &c.w_in, &c.w_out, &c.w_dvin, &c.w_dng, &c.w_l2i, &c.w_beta, &c.w_eu, &c.w_rho, &c.w_ecol,
                     &c.w_epcol, &c.w_dscr, &c.m_vrs32, &c.m_q32, &c.m_qp[0], &c.m_qp[1], &c.m_qp[2], &c.m_qtp[0],
                     &c.m_qtp[1], &c.m_qtp[2], &c.m_g, &c.m_h, &c.m_hold, &c.m_dx, &c.m_bc, &c.m_tc, &c.m_cm,
-                    &c.m_work, &c.m_u2i, &c.m_rho64, &c.m_eu64, &c.m_fb, &c.m_fx, &c.m_fe, &c.m_fep})
+                    &c.m_work, &c.m_u2i, &c.m_rho64, &c.m_eu64, &c.m_fb, &c.m_fx, &c.m_fe, &c.m_fep,
+                    &c.w_rhob})
     b->release();
+  for (HostBuf* b : {&c.h_l2i, &c.h_beta, &c.h_eu, &c.h_rho}) b->release();
   for (DevBuf& b : c.dav) b.release();
   (void)hipStreamDestroy(c.stream);
   delete cp;
@@ -273,18 +275,22 @@ int linsolve_dev(int m, z64* b, z64* x, const std::vector<double>& e_col, const 
   stern_apply(x, g, d_e_col, m, c.alpha_pv);
   axpby(static_cast<long>(vec), -1.0, b, 1.0, g, s);
   std::vector<char> is_conv(m, 0);
-  std::vector<int> l2i(m);
-  std::iota(l2i.begin(), l2i.end(), 0);
-  std::vector<double> rho(m, 0.0), rhoold(m, 0.0), rho_c(m), beta(m), eu(m);
+  // pinned: the lists and coefficients go to the device with asynchronous copies
+  int* l2i = c.h_l2i.as<int>(m);
+  double* rho_c = c.h_rho.as<double>(m);
+  double* beta = c.h_beta.as<double>(m);
+  double* eu = c.h_eu.as<double>(m);
+  std::iota(l2i, l2i + m, 0);
+  std::vector<double> rho(m, 0.0), rhoold(m, 0.0);
   int nc = m;
   const int itmax = maxiter > 0 ? maxiter : c.n_dfpt_maxiter;
   for (int iter = 1; iter <= itmax; ++iter) {
     // (from iteration 2 on, d_l2i already holds this list: it was sent with beta)
     if (iter == 1) {
-      WH_HIP(hipMemcpyAsync(d_l2i, l2i.data(), sizeof(int) * nc, hipMemcpyHostToDevice, s));
+      WH_HIP(hipMemcpyAsync(d_l2i, l2i, sizeof(int) * nc, hipMemcpyHostToDevice, s));
       WH_TIMED("cg_prec", prec_rho(nc, npw, c.gstart, d_l2i, g, ld, c.d_g2kin, d_eprec_col, d_rhob, s, true));
     }
-    WH_HIP(hipMemcpyAsync(rho_c.data(), d_rhob, sizeof(double) * m, hipMemcpyDeviceToHost, s));
+    WH_HIP(hipMemcpyAsync(rho_c, d_rhob, sizeof(double) * m, hipMemcpyDeviceToHost, s));
     WH_HIP(hipStreamSynchronize(s));
     ++c.cg_iters;
     for (int lb = 0; lb < nc; ++lb) {
@@ -302,9 +308,9 @@ int linsolve_dev(int m, z64* b, z64* x, const std::vector<double>& e_col, const 
       beta[lb] = (iter != 1) ? rho[ib] / rhoold[ib] : 0.0;
       eu[lb] = e_col[ib];
     }
-    WH_HIP(hipMemcpyAsync(d_l2i, l2i.data(), sizeof(int) * nc, hipMemcpyHostToDevice, s));
-    WH_HIP(hipMemcpyAsync(d_beta, beta.data(), sizeof(double) * nc, hipMemcpyHostToDevice, s));
-    WH_HIP(hipMemcpyAsync(d_eu, eu.data(), sizeof(double) * nc, hipMemcpyHostToDevice, s));
+    WH_HIP(hipMemcpyAsync(d_l2i, l2i, sizeof(int) * nc, hipMemcpyHostToDevice, s));
+    WH_HIP(hipMemcpyAsync(d_beta, beta, sizeof(double) * nc, hipMemcpyHostToDevice, s));
+    WH_HIP(hipMemcpyAsync(d_eu, eu, sizeof(double) * nc, hipMemcpyHostToDevice, s));
     WH_TIMED("cg_dir", direction(nc, npw, d_l2i, d_beta, g, c.d_g2kin, d_eprec_col, h, hold, bc, ld, s));
     stern_apply(bc, tc, d_eu, nc, c.alpha_pv);
     WH_TIMED("cg_update", cg_update(nc, npw, c.gstart, d_l2i, h, g, tc, x, ld, nullptr, s, &rho_out));

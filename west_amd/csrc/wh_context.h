@@ -40,6 +40,36 @@ struct DevBuf {
   }
 };
 
+// Pinned host staging (hipHostMalloc) for the small per-iteration copies of
+// the CG (band lists, coefficients, rho): asynchronous DMA instead of staged
+// pageable copies.  Same growth rule as DevBuf.
+struct HostBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void* get(size_t n) {
+    if (n > bytes) {
+      if (p) {
+        WH_HIP(hipDeviceSynchronize());
+        WH_HIP(hipHostFree(p));
+      }
+      p = nullptr;
+      bytes = 0;
+      WH_HIP(hipHostMalloc(&p, n, hipHostMallocDefault));
+      bytes = n;
+    }
+    return p;
+  }
+  template <typename T>
+  T* as(size_t count) {
+    return static_cast<T*>(get(sizeof(T) * std::max<size_t>(count, 1)));
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
 struct Profiler {
   struct Pending {
     std::string name;
@@ -95,6 +125,7 @@ struct Context {
   DevBuf w_items, w_items2, w_items_id, w_sticks_a, w_sticks_b, w_cmat;
   DevBuf w_x, w_t, w_g, w_h, w_hold, w_bc, w_tc, w_b, w_dvr, w_in, w_out, w_dvin, w_dng;
   DevBuf w_l2i, w_beta, w_eu, w_rho, w_rhob, w_ecol, w_epcol, w_dscr;
+  HostBuf h_l2i, h_beta, h_eu, h_rho;   // pinned staging of the FP64 CG loop
   // mixed precision: FP32 operator with split-BF16 projectors, FP64 refinement
   int precision = 0;              // 0: FP64, 1: mixed (3-product split BF16), 2: mixed (6-product)
   double refine_delta = 1e-4;     // refine when rho < refine_delta * rho of the last refinement
