@@ -71,7 +71,9 @@ static void free_context(Context* cp) {
                     &c.m_work, &c.m_u2i, &c.m_rho64, &c.m_eu64, &c.m_fb, &c.m_fx, &c.m_fe, &c.m_fep,
                     &c.w_rhob})
     b->release();
-  for (HostBuf* b : {&c.h_l2i, &c.h_beta, &c.h_eu, &c.h_rho}) b->release();
+  for (HostBuf* b : {&c.h_l2i, &c.h_beta, &c.h_eu, &c.h_rho, &c.hm_l2i, &c.hm_beta, &c.hm_eu, &c.hm_rho, &c.hm_eu64,
+                     &c.hm_u})
+    b->release();
   for (DevBuf& b : c.dav) b.release();
   (void)hipStreamDestroy(c.stream);
   delete cp;
@@ -471,8 +473,15 @@ int linsolve_mixed(int m, z64* b, z64* x, const std::vector<double>& e_col, cons
   WH_HIP(hipMemsetAsync(dx, 0, sizeof(z32) * v32, s));
 
   std::vector<char> is_conv(m, 0), to_fp64(m, 0);
-  std::vector<int> l2i(m), list, stall(m, 0), since(m, 0);
-  std::vector<double> rho(m, 0.0), rhoold(m, 0.0), rho_ref(m, 0.0), tmp(m), beta(m), eu(m), eu64(m);
+  std::vector<int> list, stall(m, 0), since(m, 0);
+  std::vector<double> rho(m, 0.0), rhoold(m, 0.0), rho_ref(m, 0.0);
+  // pinned staging of the per-iteration copies
+  int* l2i = c.hm_l2i.as<int>(m);
+  int* uh = c.hm_u.as<int>(m);
+  double* tmp = c.hm_rho.as<double>(m);
+  double* beta = c.hm_beta.as<double>(m);
+  double* eu = c.hm_eu.as<double>(m);
+  double* eu64 = c.hm_eu64.as<double>(m);
   list.reserve(m);
   // FP32 iterations after which a refinement is forced
   const int kforce = 40;
@@ -483,14 +492,17 @@ int linsolve_mixed(int m, z64* b, z64* x, const std::vector<double>& e_col, cons
   // conditioning) is finished by the FP64 CG instead.
   auto refine = [&](const std::vector<int>& u, bool first) {
     const int nu = static_cast<int>(u.size());
-    for (int k = 0; k < nu; ++k) eu64[k] = e_col[u[k]];
-    WH_HIP(hipMemcpyAsync(d_u2i, u.data(), sizeof(int) * nu, hipMemcpyHostToDevice, s));
-    WH_HIP(hipMemcpyAsync(d_eu64, eu64.data(), sizeof(double) * nu, hipMemcpyHostToDevice, s));
+    for (int k = 0; k < nu; ++k) {
+      eu64[k] = e_col[u[k]];
+      uh[k] = u[k];
+    }
+    WH_HIP(hipMemcpyAsync(d_u2i, uh, sizeof(int) * nu, hipMemcpyHostToDevice, s));
+    WH_HIP(hipMemcpyAsync(d_eu64, eu64, sizeof(double) * nu, hipMemcpyHostToDevice, s));
     WH_TIMED("refine_fold", fold_increment(nu, npw, d_u2i, x, ld, dx, l32, xc, s));
     stern_apply(xc, tc64, d_eu64, nu, c.alpha_pv);
     WH_TIMED("refine_residual", true_residual(nu, npw, c.gstart, d_u2i, tc64, b, ld, c.d_g2kin, d_eprec_col, g, l32,
                                               d_rho64, s));
-    WH_HIP(hipMemcpyAsync(tmp.data(), d_rho64, sizeof(double) * nu, hipMemcpyDeviceToHost, s));
+    WH_HIP(hipMemcpyAsync(tmp, d_rho64, sizeof(double) * nu, hipMemcpyDeviceToHost, s));
     WH_HIP(hipStreamSynchronize(s));
     for (int k = 0; k < nu; ++k) {
       const int ib = u[k];
@@ -508,9 +520,10 @@ int linsolve_mixed(int m, z64* b, z64* x, const std::vector<double>& e_col, cons
   // h = M^-1 g and rho for the bands in idx (device list d_idx)
   auto precond = [&](const std::vector<int>& idx, int* d_idx) {
     const int n = static_cast<int>(idx.size());
-    WH_HIP(hipMemcpyAsync(d_idx, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    std::copy(idx.begin(), idx.end(), uh);
+    WH_HIP(hipMemcpyAsync(d_idx, uh, sizeof(int) * n, hipMemcpyHostToDevice, s));
     WH_TIMED("cg_prec32", prec_rho(n, npw, c.gstart, d_idx, g, l32, c.d_g2kin, d_eprec_col, d_rho, s));
-    WH_HIP(hipMemcpyAsync(tmp.data(), d_rho, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    WH_HIP(hipMemcpyAsync(tmp, d_rho, sizeof(double) * n, hipMemcpyDeviceToHost, s));
     WH_HIP(hipStreamSynchronize(s));
     for (int k = 0; k < n; ++k) rho[idx[k]] = tmp[k];
   };
@@ -543,7 +556,7 @@ int linsolve_mixed(int m, z64* b, z64* x, const std::vector<double>& e_col, cons
   for (int iter = 1; iter <= c.n_dfpt_maxiter && !act.empty(); ++iter) {
     if (iter > 1) {
       // every band in act was updated in the previous iteration
-      WH_HIP(hipMemcpyAsync(tmp.data(), d_rhob, sizeof(double) * m, hipMemcpyDeviceToHost, s));
+      WH_HIP(hipMemcpyAsync(tmp, d_rhob, sizeof(double) * m, hipMemcpyDeviceToHost, s));
       WH_HIP(hipStreamSynchronize(s));
       for (int ib : act) rho[ib] = tmp[ib];
     }
@@ -561,7 +574,7 @@ int linsolve_mixed(int m, z64* b, z64* x, const std::vector<double>& e_col, cons
     int nc = 0;
     for (int ib : act)
       if (!is_conv[ib] && !to_fp64[ib]) l2i[nc++] = ib;
-    act.assign(l2i.begin(), l2i.begin() + nc);
+    act.assign(l2i, l2i + nc);
     if (nc == 0) break;
     // the last few slow bands (their iterations are latency-bound, and the
     // FP64 CG restarted from the current FP64 iterate needs fewer of them),
@@ -579,9 +592,9 @@ int linsolve_mixed(int m, z64* b, z64* x, const std::vector<double>& e_col, cons
       beta[lb] = (iter != 1) ? rho[ib] / rhoold[ib] : 0.0;
       eu[lb] = e_col[ib];
     }
-    WH_HIP(hipMemcpyAsync(d_l2i, l2i.data(), sizeof(int) * nc, hipMemcpyHostToDevice, s));
-    WH_HIP(hipMemcpyAsync(d_beta, beta.data(), sizeof(double) * nc, hipMemcpyHostToDevice, s));
-    WH_HIP(hipMemcpyAsync(d_eu, eu.data(), sizeof(double) * nc, hipMemcpyHostToDevice, s));
+    WH_HIP(hipMemcpyAsync(d_l2i, l2i, sizeof(int) * nc, hipMemcpyHostToDevice, s));
+    WH_HIP(hipMemcpyAsync(d_beta, beta, sizeof(double) * nc, hipMemcpyHostToDevice, s));
+    WH_HIP(hipMemcpyAsync(d_eu, eu, sizeof(double) * nc, hipMemcpyHostToDevice, s));
     WH_TIMED("cg_dir32", direction(nc, npw, d_l2i, d_beta, g, c.d_g2kin, d_eprec_col, h, hold, bc, l32, s));
     stern_apply32(bc, tc, d_eu, nc, c.alpha_pv);
     WH_TIMED("cg_update32", cg_update(nc, npw, c.gstart, d_l2i, h, g, tc, dx, l32, nullptr, s, &rho_out));

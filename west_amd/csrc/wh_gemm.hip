@@ -351,7 +351,7 @@ Scratch& partials() {
 void dgemm(bool transa, int M, int N, int K, double alpha, const double* A, long lda, const double* B, long ldb,
            double beta, double* C, long ldc, hipStream_t s) {
   if (M <= 0 || N <= 0) return;
-  if (!transa && K > 0 && K <= 32 && backend() != Backend::Rocblas) {
+  if (!transa && K > 0 && K <= 32 && N <= 256 && backend() != Backend::Rocblas) {
     // inner dimension of a few bands: a streaming kernel (the matrix-core tiles would be nearly empty)
     dim3 grid(ceil_div(M, NTS), N);
     if (K <= 8) k_ket_skinny<8><<<grid, NTS, 0, s>>>(M, K, alpha, A, lda, B, ldb, beta, C, ldc);
@@ -431,8 +431,9 @@ void dgemm_bra(int M, int N, int K, double alpha, const double* A, long lda, con
   if (M <= 0 || N <= 0) return;
   const Backend be = backend();
   if (be == Backend::Rocblas && K > 0) return dgemm_bra_lib(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s);
-  // a few rows: the matrix-core tiles (128 rows) would be almost empty
-  if (M <= 32 && K > 0) return dgemm_bra_skinny(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s);
+  // a few rows and columns: the matrix-core tiles (128 x 128) would be almost
+  // empty (with many columns the tiles' reuse of A wins again)
+  if (M <= 32 && N <= 256 && K > 0) return dgemm_bra_skinny(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s);
   if (be == Backend::Own || K <= 0) return dgemm_bra_own(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s);
   // variants: 0..2 the kernel with its K split aimed at 512, 1024, 2048 workgroups; 3 rocBLAS
   static std::map<std::tuple<int, int, int>, int> chosen;   // (M, K, log2 N) -> variant
