@@ -58,9 +58,17 @@ __device__ __forceinline__ void split4(f4 v, bf4 (&p)[Pieces<TERMS>::n]) {
   }
 }
 
+// The grid is 1D over gx x gy x gz tiles (gz: K chunks) in an XCD-aware
+// order: blocks are dispatched round-robin over the 8 XCDs (block L on XCD
+// L % 8), and block L runs tile W = (L % 8) ceil(T / 8) + L / 8, so
+// consecutive tiles, which share an A or B panel, share an XCD and its L2.
+constexpr int kXcds = 8;
+inline int xcd_blocks(int tiles) { return (tiles + kXcds - 1) / kXcds * kXcds; }
+
 template <int TERMS, bool PARTIAL>
 __global__ __launch_bounds__(NT) void k_gemm_tn(int M, int N, int K, int kchunk, PiecePtrs ap, long lda, const float* __restrict__ b, long ldb, float alpha,
-                                                float beta, float* out, long ci, long cj, long pstride, int vec_out) {
+                                                float beta, float* out, long ci, long cj, long pstride, int vec_out,
+                                                int gx, int gy, int gz) {
   constexpr int NP = Pieces<TERMS>::n;
   __shared__ __attribute__((aligned(16))) __bf16 sa[NP][BM * LDP];
   __shared__ __attribute__((aligned(16))) __bf16 sb[NP][BN * LDP];
@@ -69,9 +77,13 @@ __global__ __launch_bounds__(NT) void k_gemm_tn(int M, int N, int K, int kchunk,
   const int w = t >> 6;
   const int wm = w & 1;
   const int wn = w >> 1;
-  const int i0 = blockIdx.x * BM;
-  const int j0 = blockIdx.y * BN;
-  const int kb = blockIdx.z * kchunk;
+  const int tiles = gx * gy * gz;
+  const int W = static_cast<int>(blockIdx.x % kXcds) * ((tiles + kXcds - 1) / kXcds) + static_cast<int>(blockIdx.x / kXcds);
+  if (W >= tiles) return;   // uniform per workgroup
+  const int bz = W / (gx * gy);
+  const int i0 = (W % gx) * BM;
+  const int j0 = ((W / gx) % gy) * BN;
+  const int kb = bz * kchunk;
   const int ke = min(K, kb + kchunk);
   const __bf16* apc[NP];
 #pragma unroll
@@ -178,7 +190,7 @@ __global__ __launch_bounds__(NT) void k_gemm_tn(int M, int N, int K, int kchunk,
     }
   }
 
-  float* o0 = out + (PARTIAL ? static_cast<long>(blockIdx.z) * pstride : 0L);
+  float* o0 = out + (PARTIAL ? static_cast<long>(bz) * pstride : 0L);
 #pragma unroll
   for (int x = 0; x < 4; ++x)
 #pragma unroll
@@ -303,8 +315,9 @@ void launch_tn(dim3 grid, int M, int N, int K, int kchunk, const void* const ap[
                float alpha, float beta, float* out, long ci, long cj, long pstride, int vec, hipStream_t s) {
   PiecePtrs pp{{static_cast<const __bf16*>(ap[0]), static_cast<const __bf16*>(ap[1]),
                 static_cast<const __bf16*>(ap[2])}};
-  k_gemm_tn<TERMS, PARTIAL><<<grid, NT, 0, s>>>(M, N, K, kchunk, pp, lda, b, ldb, alpha, beta, out, ci, cj, pstride,
-                                                vec);
+  const int gx = static_cast<int>(grid.x), gy = static_cast<int>(grid.y), gz = static_cast<int>(grid.z);
+  k_gemm_tn<TERMS, PARTIAL><<<xcd_blocks(gx * gy * gz), NT, 0, s>>>(M, N, K, kchunk, pp, lda, b, ldb, alpha, beta, out,
+                                                                   ci, cj, pstride, vec, gx, gy, gz);
 }
 
 }  // namespace
