@@ -7,15 +7,14 @@
 namespace wh {
 
 // Number of register-line plane variants for this plan (0: the grid is not
-// one of the compiled sizes, or its sphere is too wide for them).
+// one of the compiled sizes, its sphere is too wide for them, or fp32 is
+// false: an FP64 line of these lengths does not fit one thread's registers).
 int plane_reg_variants(const StickPlanDev& P, bool fp32);
 
-// Variant v (0 <= v < plane_reg_variants) of the plane step
+// Variant v (0 <= v < plane_reg_variants(P, true)) of the FP32 plane step
 // sticks_out = P_sphere F_12 V F_12^-1 sticks_in (same contract as the
 // LDS-pass plane kernels: items[i].a selects the input sticks, output item i;
 // items[i].pot >= 0 selects a per-item potential vpot + pot * ldpot).
-void plane_reg_vloc(int v, const StickPlanDev& P, const FftItem* items, int n, const z64* sin, z64* sout,
-                    const double* vpot, long ldpot, double scale, hipStream_t s);
 void plane_reg_vloc(int v, const StickPlanDev& P, const FftItem* items, int n, const z32* sin, z32* sout,
                     const float* vpot, long ldpot, double scale, hipStream_t s);
 

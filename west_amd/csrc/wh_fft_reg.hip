@@ -504,12 +504,6 @@ int plane_reg_variants(const StickPlanDev& P, bool fp32) {
   return nv;
 }
 
-void plane_reg_vloc(int v, const StickPlanDev& P, const FftItem* items, int n, const z64* sin, z64* sout,
-                    const double* vpot, long ldpot, double scale, hipStream_t s) {
-  (void)items, (void)n, (void)sin, (void)sout, (void)vpot, (void)ldpot, (void)scale, (void)s;
-  WH_REQUIRE(v >= 0 && v < plane_reg_variants(P, false), "plane_reg_vloc: no such variant");
-}
-
 void plane_reg_vloc(int v, const StickPlanDev& P, const FftItem* items, int n, const z32* sin, z32* sout,
                     const float* vpot, long ldpot, double scale, hipStream_t s) {
   WH_REQUIRE(v >= 0 && v < plane_reg_variants(P, true), "plane_reg_vloc: no such variant");
