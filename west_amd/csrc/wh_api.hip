@@ -222,12 +222,6 @@ void stern_apply(const z64* x, z64* t, const double* d_e, int m, double alpha) {
     z64* sa = c.w_sticks_a.as<z64>(static_cast<size_t>(chunk) * c.plan.sticks_elems());
     z64* sb = c.w_sticks_b.as<z64>(static_cast<size_t>(chunk) * c.plan.sticks_elems());
     ZfwdArgs za{ZFWD_PAIR_STERN, 1.0, c.d_g2kin, d_e, nullptr, 0};
-    // register-line z kernels where the grid has them (WH_ZREG=0: the LDS-pass ones)
-    static const bool zreg_env = [] {
-      const char* e = std::getenv("WH_ZREG");
-      return !(e && e[0] == '0');
-    }();
-    const bool zreg = zreg_env && zreg_available(c.plan.dev);
     for (int q0 = 0; q0 < npair; q0 += chunk) {
       const int nq = std::min(chunk, npair - q0);
       WH_TIMED("zinv", launch_zinv(c.plan, items + q0, nq, x, ld, sa, ZINV_PAIR, s));
