@@ -251,3 +251,26 @@ def test_systems_concurrent(gpu, gs_h2, gs_3h2, gs_si8):
         gpu.system_select(0)
         for h in handles:
             gpu.system_destroy(h)
+
+
+@pytest.mark.parametrize("key", ["h2", "3h2"])
+def test_cg_graphs_match_eager(gpu, key, request, monkeypatch):
+    """Small batches replay their CG iterations from HIP graphs (captured the
+    second time an active-column count comes up): the response is bit-identical
+    to the all-eager run (WH_GRAPH=0), over many replays with the loop's host
+    synchronisations between them, and matches the FP64 oracle."""
+    from oracle.dfpt import dfpt
+    gs = request.getfixturevalue(f"gs_{key}")
+    _register(gpu, gs)
+    dvg = random_wfc(gs, 4, seed=37)
+    ref = dfpt(gs, dvg, 1e-22)
+    monkeypatch.setenv("WH_GRAPH", "0")
+    eager = gpu.dfpt(4, dvg, 1e-22)
+    monkeypatch.delenv("WH_GRAPH")
+    c0 = gpu.counters()
+    replay = gpu.dfpt(4, dvg, 1e-22)
+    c1 = gpu.counters()
+    print(f"{key}: {c1[1] - c0[1]} CG iterations, rel err {_rel(replay, ref):.2e}", flush=True)
+    assert c1[1] - c0[1] > 4
+    assert np.array_equal(replay, eager)
+    assert _rel(replay, ref) < 1e-9

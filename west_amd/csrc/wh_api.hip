@@ -45,6 +45,15 @@ Context& ctx() {
   return *c;
 }
 
+// true when this process holds one system: a stream capture must then not
+// meet another thread's device-wide synchronisation
+static bool single_system() {
+  std::lock_guard<std::mutex> lk(g_sys_mutex);
+  int n = 0;
+  for (Context* c : g_sys) n += c != nullptr;
+  return n == 1;
+}
+
 static Context* new_context(int device) {
   Context* c = new Context();
   c->device = device;
@@ -280,6 +289,25 @@ int linsolve_dev(int m, z64* b, z64* x, const std::vector<double>& e_col, const 
   std::vector<double> rho(m, 0.0), rhoold(m, 0.0);
   int nc = m;
   const int itmax = maxiter > 0 ? maxiter : c.n_dfpt_maxiter;
+  // Small batches (a molecule's few bands: every product below is a
+  // streaming kernel) are launch-bound: the device work of an iteration
+  // (direction, operator, update) is captured in a HIP graph the second time
+  // the same active-column count and h/hold assignment come up, and replayed
+  // while they stay.  All buffers were sized by the first, eager iteration
+  // (nc only shrinks), so nothing inside allocates.  WH_GRAPH=0 keeps every
+  // iteration eager; so do the profiler's timed mode and a process holding
+  // several systems (their threads' device-wide synchronisations may not
+  // meet a capture).
+  const int nqp = c.l_kinetic_only ? c.nbndval : c.nbndval + c.nkb;
+  const char* genv = std::getenv("WH_GRAPH");
+  const bool graphs = !(genv && genv[0] == '0') && !c.prof.on && nqp <= 32 && m <= 256 && single_system();
+  struct CgGraph {
+    int nc;
+    const void* h;
+    hipGraphExec_t exec;
+  };
+  std::vector<CgGraph> cg_graphs;
+  int eager_nc = -1;   // active-column count of the last eager iteration
   for (int iter = 1; iter <= itmax; ++iter) {
     // (from iteration 2 on, d_l2i already holds this list: it was sent with beta)
     if (iter == 1) {
@@ -307,15 +335,37 @@ int linsolve_dev(int m, z64* b, z64* x, const std::vector<double>& e_col, const 
     WH_HIP(hipMemcpyAsync(d_l2i, l2i, sizeof(int) * nc, hipMemcpyHostToDevice, s));
     WH_HIP(hipMemcpyAsync(d_beta, beta, sizeof(double) * nc, hipMemcpyHostToDevice, s));
     WH_HIP(hipMemcpyAsync(d_eu, eu, sizeof(double) * nc, hipMemcpyHostToDevice, s));
-    WH_TIMED("cg_dir", direction(nc, npw, d_l2i, d_beta, g, c.d_g2kin, d_eprec_col, h, hold, bc, ld, s));
-    stern_apply(bc, tc, d_eu, nc, c.alpha_pv);
-    WH_TIMED("cg_update", cg_update(nc, npw, c.gstart, d_l2i, h, g, tc, x, ld, nullptr, s, &rho_out));
+    auto body = [&]() {
+      WH_TIMED("cg_dir", direction(nc, npw, d_l2i, d_beta, g, c.d_g2kin, d_eprec_col, h, hold, bc, ld, s));
+      stern_apply(bc, tc, d_eu, nc, c.alpha_pv);
+      WH_TIMED("cg_update", cg_update(nc, npw, c.gstart, d_l2i, h, g, tc, x, ld, nullptr, s, &rho_out));
+    };
+    const CgGraph* cached = nullptr;
+    for (const CgGraph& gr : cg_graphs)
+      if (gr.nc == nc && gr.h == h) cached = &gr;
+    if (cached) {
+      WH_HIP(hipGraphLaunch(cached->exec, s));
+    } else if (graphs && eager_nc == nc) {
+      hipGraph_t graph = nullptr;
+      WH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+      body();
+      WH_HIP(hipStreamEndCapture(s, &graph));
+      hipGraphExec_t exec = nullptr;
+      WH_HIP(hipGraphInstantiateWithFlags(&exec, graph, 0));
+      WH_HIP(hipGraphDestroy(graph));
+      cg_graphs.push_back({nc, h, exec});
+      WH_HIP(hipGraphLaunch(exec, s));
+    } else {
+      body();
+      eager_nc = nc;
+    }
     std::swap(h, hold);   // hold <- h  (linsolve 319)
     for (int lb = 0; lb < nc; ++lb) rhoold[l2i[lb]] = rho[l2i[lb]];
     // the host lists sent above are next modified after the synchronisation
     // that follows the next iteration's rho read, which orders those copies
   }
   WH_HIP(hipStreamSynchronize(s));
+  for (const CgGraph& gr : cg_graphs) WH_HIP(hipGraphExecDestroy(gr.exec));
   if (ierr_col) {
     ierr_col->assign(m, 0);
     for (int lb = 0; lb < nc; ++lb) (*ierr_col)[l2i[lb]] = 1;
