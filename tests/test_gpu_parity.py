@@ -274,3 +274,27 @@ def test_cg_graphs_match_eager(gpu, key, request, monkeypatch):
     assert c1[1] - c0[1] > 4
     assert np.array_equal(replay, eager)
     assert _rel(replay, ref) < 1e-9
+
+
+def test_pdep_db_from_davidson(gpu, gs_3h2, tmp_path):
+    """Davidson eigenpairs saved in the PDEP database format and read back:
+    the vectors read from disk, put through apply_operator again, give the
+    saved eigenvalues as Gamma Rayleigh quotients <v|A v>/<v|v>."""
+    from west_amd import pdep_db
+    gs = gs_3h2
+    _register(gpu, gs)
+    b = gs.basis
+    ev, conv, dvg, _ = gpu.davidson_diago(4, tr2_dfpt=1e-20, trev_pdep=1e-6, trev_pdep_rel=1e-4)
+    save = str(tmp_path / "wstat.save")
+    pdep_db.pdep_db_write(save, ev, dvg, b.npw, b.at, b.bg)
+    ev2, v = pdep_db.pdep_db_read(save, b.npw, gs.npwx)
+    np.testing.assert_array_equal(ev2, ev)
+    np.testing.assert_array_equal(v[:, :b.npw], dvg[:, :b.npw])
+    av = gpu.apply_operator(v.shape[0], v, tr2=1e-20)
+
+    def dot(x, y):   # <x|y> at Gamma: 2 Re sum x* y - x(0) y(0)
+        return 2 * np.real(np.sum(np.conj(x) * y, axis=1)) - np.real(x[:, 0] * y[:, 0])
+
+    rq = dot(v, av) / dot(v, v)
+    print("ev", ev, "rayleigh", rq, flush=True)
+    assert np.max(np.abs(rq - ev) / np.abs(ev)) < 1e-6
