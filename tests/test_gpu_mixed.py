@@ -143,9 +143,10 @@ def test_mixed_davidson(mixed, gs_si8):
 
 @pytest.mark.parametrize("key", ["h2", "3h2", "si8"])
 def test_plane_variants(gpu, key, request, monkeypatch):
-    """Every plane-kernel variant of the FP32 operator (LDS-pass kernels sized
-    for 1-3 workgroups per CU, register-line kernels with 2-4 planes per
-    workgroup; WH_PLANE_OCC fixes the variant) against the FP64 oracle, and
+    """Every plane-kernel variant of the FP32 and the FP64 operator (LDS-pass
+    kernels sized for 1-3 workgroups per CU, register-line kernels with 2-4
+    planes per workgroup; WH_PLANE_OCC fixes the variant) against the FP64
+    oracle (FP64 to 1e-12), and
     the register-line kernels shown to have run: their FFT rounding differs
     from the LDS-pass kernel's, so their output is not bit-identical to it."""
     from oracle.hamiltonian import apply_sternheimerop
@@ -164,6 +165,12 @@ def test_plane_variants(gpu, key, request, monkeypatch):
             print(f"{key} variant {var}: rel err {err:.2e}", flush=True)
             assert err < 1e-5
             outs.append(out)
+            # the FP64 operator's plane variants (register-line kernels as well)
+            gpu.set_precision("fp64")
+            out64 = gpu.apply_sternheimerop_to_m_wfcs(gs.nbndval, psi, e, gs.alpha_pv)
+            err64 = _rel(out64, ref)
+            print(f"{key} variant {var} fp64: rel err {err64:.2e}", flush=True)
+            assert err64 < 1e-12
     finally:
         gpu.set_precision("fp64")
         monkeypatch.delenv("WH_PLANE_OCC", raising=False)

@@ -40,11 +40,18 @@ def test_sternheimer_operator(gpu, key, m, request):
     assert _rel(out, ref) < 1e-12
 
 
-@pytest.mark.parametrize("nrep,ecut", [(1, 25.0), (2, 25.0)])
-def test_operator_workload_grids(gpu, nrep, ecut):
-    """The operator on the workload grids (36^3 and 72^3; the plane kernel of
-    the latter needs more than 64 KB of LDS), with the Harris potential and
-    random orthonormal occupied states."""
+@pytest.mark.parametrize("plane_variant", [None, 1, 2, 4, 5, 6])
+@pytest.mark.parametrize("nrep,ecut", [(1, 25.0), (1, 70.0), (2, 25.0)])
+def test_operator_workload_grids(gpu, nrep, ecut, plane_variant, monkeypatch):
+    """The operator on the workload grids (36^3, 60^3 and 72^3; the plane kernel
+    of the last needs more than 64 KB of LDS), with the Harris potential and
+    random orthonormal occupied states.  plane_variant fixes the plane kernel
+    (WH_PLANE_OCC: 1-3 the LDS-pass kernel sized for that many workgroups per
+    CU, where compiled; above, the register-line kernels with 2, 3, 4 planes
+    per workgroup where their LDS fits; a number beyond the last runs the
+    last); None leaves the choice to the library."""
+    if plane_variant is not None:
+        monkeypatch.setenv("WH_PLANE_OCC", str(plane_variant))
     from oracle.hamiltonian import apply_sternheimerop
     from west_amd.groundstate import GroundState, _orthonormalize, from_real, to_real
     from west_amd.systems import atomic_density_g, kb_projectors, ks_potential, make_si_system
@@ -56,11 +63,12 @@ def test_operator_workload_grids(gpu, nrep, ecut):
     gs = GroundState(basis=b, npwx=b.npw, vrs=ks_potential(s, atomic_density_g(s)), vkb=np.ascontiguousarray(vkb.T),
                      dvan=dvan, evc=ev, et=np.linspace(-0.5, 0.2, nv), nbndval=nv, occ=np.ones(nv), alpha_pv=1.4)
     _register(gpu, gs)
-    print(f"case nrep={nrep} nr={b.nr} npw={b.npw}", flush=True)
+    print(f"case nrep={nrep} nr={b.nr} npw={b.npw} plane variant {plane_variant}", flush=True)
     psi = random_wfc(gs, 3, seed=7)
     e = np.array([-0.4, -0.1, 0.05])
     ref = apply_sternheimerop(gs, psi, e, gs.alpha_pv)
     out = gpu.apply_sternheimerop_to_m_wfcs(nv, psi, e, gs.alpha_pv)
+    monkeypatch.delenv("WH_PLANE_OCC", raising=False)
     assert _rel(out, ref) < 1e-12
 
 

@@ -18,6 +18,7 @@
 #include <mutex>
 
 #include "wh_fft.h"
+#include "wh_fft_reg.h"
 #include "wh_fft_tpl.h"
 
 namespace wh {
@@ -660,9 +661,18 @@ void launch_plane_vloc(const StickPlan& p, const FftItem* d_items, int nitems, c
   const StickPlanDev& P = p.dev;
   if (p.tpl) {
     WH_REQUIRE(static_cast<const void*>(sin) != static_cast<const void*>(sout), "plane_vloc: input and output alias");
+    // variants 1..max_occ: LDS-pass kernel sized for that many workgroups per
+    // CU; above: register-line kernels
+    const int max_occ = p.tpl->max_occ;
     plane_occ_launch(
-        p.vloc_occ, p.tpl->max_occ, nitems,
-        [&](int occ) { p.tpl->vloc(P, d_items, nitems, sin, sout, vpot, ldpot, scale, occ, s); }, s);
+        p.vloc_occ, max_occ + plane_reg_variants(P, false), nitems,
+        [&](int v) {
+          if (v <= max_occ)
+            p.tpl->vloc(P, d_items, nitems, sin, sout, vpot, ldpot, scale, v, s);
+          else
+            plane_reg_vloc(v - max_occ - 1, P, d_items, nitems, sin, sout, vpot, ldpot, scale, s);
+        },
+        s);
     WH_HIP(hipGetLastError());
     return;
   }

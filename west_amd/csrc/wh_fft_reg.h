@@ -6,9 +6,9 @@
 
 namespace wh {
 
-// Number of register-line plane variants for this plan (0: the grid is not
-// one of the compiled sizes, its sphere is too wide for them, or fp32 is
-// false: an FP64 line of these lengths does not fit one thread's registers).
+// Number of register-line plane variants for this plan in FP32 (fp32) or
+// FP64 (0: the grid is not cubic of a compiled size -- FP64 lines stop at 72
+// points -- or its sphere is too wide for them).
 int plane_reg_variants(const StickPlanDev& P, bool fp32);
 
 // Variant v (0 <= v < plane_reg_variants(P, true)) of the FP32 plane step
@@ -17,6 +17,9 @@ int plane_reg_variants(const StickPlanDev& P, bool fp32);
 // items[i].pot >= 0 selects a per-item potential vpot + pot * ldpot).
 void plane_reg_vloc(int v, const StickPlanDev& P, const FftItem* items, int n, const z32* sin, z32* sout,
                     const float* vpot, long ldpot, double scale, hipStream_t s);
+// The same in FP64 (0 <= v < plane_reg_variants(P, false)).
+void plane_reg_vloc(int v, const StickPlanDev& P, const FftItem* items, int n, const z64* sin, z64* sout,
+                    const double* vpot, long ldpot, double scale, hipStream_t s);
 
 // Register-line z transforms of the FP32 operator (same contract as the
 // size-specialised zinv/zfwd; zfwd in the pair modes only), available when

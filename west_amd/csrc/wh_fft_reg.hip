@@ -13,6 +13,10 @@
 // (|i| > IMX, the sphere bound) and which outputs are dropped; this file is
 // compiled with -fno-signed-zeros -ffinite-math-only so that the arithmetic
 // on those known zeros folds away.  Neither flag changes any finite result.
+//
+// The FP64 plane kernels are built from this file in a translation unit of
+// their own (wh_fft_reg64.hip defines WH_REG_FP64), so that the two sets of
+// instantiations compile in parallel.
 #include <map>
 #include <mutex>
 #include <utility>
@@ -87,7 +91,8 @@ constexpr int nt_reg() {
 // |i| <= IMX; rows beyond the sphere's |i| <= imax stay zero), so that the a1
 // lines load and store their rows at compile-time offsets without branches.
 template <typename T, int N, typename PT, int PZ>
-__global__ __launch_bounds__((nt_reg<N, PZ>())) __attribute__((amdgpu_waves_per_eu(2))) void k_plane_vloc_reg(
+// FP64 lines need the accumulation registers as well (one wave per SIMD)
+__global__ __launch_bounds__((nt_reg<N, PZ>())) __attribute__((amdgpu_waves_per_eu((sizeof(T) == 8 ? 1 : 2)))) void k_plane_vloc_reg(
     StickPlanDev P, const FftItem* items, const cplx<T>* sin, cplx<T>* sout, const PT* vpot, long ldpot,
     double scale) {
   constexpr int RS = N + 1;   // row stride: odd, so row-parallel accesses spread over all banks
@@ -420,21 +425,53 @@ void zfwd_n(const StickPlanDev& P, const FftItem* it, int n, const z32* st, cons
   k_zfwd_reg<N><<<dim3(ceil_div(P.ncs, kZregCols), n), ZNT, sm, s>>>(P, it, st, in, ldin, out, ldout, A);
 }
 
-// compiled line lengths (FP32 only: an FP64 line of these lengths does not fit
-// in the registers of one thread)
+// compiled line lengths: FP32, and FP64 up to 72 points (a 72-point FP64
+// line takes 288 of a thread's 512 vector and accumulation registers; longer
+// ones would spill)
 #define WH_REG_SIZES(X) X(18) X(20) X(24) X(30) X(32) X(36) X(40) X(45) X(48) X(50) X(54) X(60) X(64) X(72) X(75) X(80)
+#define WH_REG64_SIZES(X) X(18) X(20) X(24) X(30) X(32) X(36) X(40) X(45) X(48) X(50) X(54) X(60) X(64) X(72)
 
 bool compiled(int n, bool fp32) {
-  if (!fp32) return false;
-  switch (n) {
 #define WH_REG_CASE(NN) \
   case NN:              \
     return true;
+  if (!fp32) {
+    switch (n) {
+      WH_REG64_SIZES(WH_REG_CASE)
+      default: return false;
+    }
+  }
+  switch (n) {
     WH_REG_SIZES(WH_REG_CASE)
-#undef WH_REG_CASE
     default: return false;
   }
+#undef WH_REG_CASE
 }
+
+#ifdef WH_REG_FP64
+
+void dispatch64(int var, const StickPlanDev& P, const FftItem* it, int n, const z64* si, z64* so, const double* v,
+                long ldv, double scale, hipStream_t s) {
+  switch (P.n1) {
+#define WH_REG_CASE(NN) \
+  case NN:              \
+    return launch_n<double, NN, double>(var, P, it, n, si, so, v, ldv, scale, s);
+    WH_REG64_SIZES(WH_REG_CASE)
+#undef WH_REG_CASE
+    default: WH_REQUIRE(false, "plane_reg_vloc: no FP64 kernel for this grid");
+  }
+}
+
+}  // namespace
+
+void plane_reg_vloc(int v, const StickPlanDev& P, const FftItem* items, int n, const z64* sin, z64* sout,
+                    const double* vpot, long ldpot, double scale, hipStream_t s) {
+  WH_REQUIRE(v >= 0 && v < plane_reg_variants(P, false), "plane_reg_vloc: no such variant");
+  WH_REQUIRE(static_cast<const void*>(sin) != static_cast<const void*>(sout), "plane_reg_vloc: input and output alias");
+  dispatch64(v, P, items, n, sin, sout, vpot, ldpot, scale, s);
+}
+
+#else
 
 void dispatch32(int var, const StickPlanDev& P, const FftItem* it, int n, const z32* si, z32* so, const float* v,
                 long ldv, double scale, hipStream_t s) {
@@ -510,5 +547,7 @@ void plane_reg_vloc(int v, const StickPlanDev& P, const FftItem* items, int n, c
   WH_REQUIRE(static_cast<const void*>(sin) != static_cast<const void*>(sout), "plane_reg_vloc: input and output alias");
   dispatch32(v, P, items, n, sin, sout, vpot, ldpot, scale, s);
 }
+
+#endif  // WH_REG_FP64
 
 }  // namespace wh
