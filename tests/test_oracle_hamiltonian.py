@@ -101,3 +101,54 @@ def test_precondition_eprec_alpha(gs_si8):
     # alpha_pv = max(2 (max occupied - min all), 0.01)
     assert np.isclose(get_alpha_pv(gs.et, gs.nbndval), max(2 * (gs.et[:gs.nbndval].max() - gs.et.min()), 0.01))
     assert get_alpha_pv(np.array([1.0, 1.0]), 1) == 0.01
+
+
+def test_set_eprec_is_kinetic_energy_from_real_space_gradients(gs_si8):
+    """set_eprec (Tools/set_eprec.f90:46-51) sums |psi(G)|^2 g2kin over the
+    half sphere without the Gamma doubling, so eprec = 1.35 T / 2 with T the
+    kinetic energy <psi|-lap|psi> (Ry).  T here comes from another route: the
+    gradient of psi(r), built with numpy's own FFT from the coefficients
+    placed at +G and -G by Miller index, T = (1/N) sum_r |grad psi(r)|^2."""
+    gs = gs_si8
+    b = gs.basis
+    nr, nnr = b.nr, b.nnr
+    G = b.mill @ b.bg                                   # Cartesian G, bohr^-1
+    pos, neg = np.mod(b.mill, nr), np.mod(-b.mill[1:], nr)
+    ep = set_eprec(gs, gs.evc)
+    for v in range(4):
+        c = gs.evc[v, :b.npw]
+        T = 0.0
+        for a in range(3):
+            grid = np.zeros(nr, dtype=np.complex128)
+            grid[tuple(pos.T)] = 1j * G[:, a] * c
+            grid[tuple(neg.T)] = np.conj(1j * G[1:, a] * c[1:])
+            d = np.fft.ifftn(grid) * nnr                # unnormalised inverse, real function
+            assert np.abs(d.imag).max() < 1e-12 * max(np.abs(d.real).max(), 1.0)
+            T += np.sum(d.real ** 2) / nnr
+        assert np.isclose(ep[v], 1.35 * T / 2, rtol=1e-12)
+
+
+def test_precondition_limits(gs_si8):
+    """precondition_m_wfcts.f90:41-46, pf = f / max(1, g2kin / eprec), in its
+    two limits: eprec above every g2kin leaves f unchanged; a tiny eprec
+    gives f * eprec / g2kin for G != 0 and leaves G = 0 (g2kin = 0); entries
+    beyond npw are zero."""
+    gs = gs_si8
+    n = gs.basis.npw
+    g2 = gs.basis.g2kin
+    f = random_wfc(gs, 2, 11)
+    f_pad = np.concatenate([f, np.ones((2, 3), dtype=f.dtype)], axis=1)
+    pf = precondition_m_wfcts(gs, f_pad, [10.0 * g2.max(), 1e-8])
+    np.testing.assert_array_equal(pf[0, :n], f[0])
+    assert not pf[:, n:].any()
+    assert pf[1, 0] == f[1, 0] and g2[0] == 0.0
+    np.testing.assert_allclose(pf[1, 1:n], f[1, 1:] * 1e-8 / g2[1:], rtol=1e-14)
+
+
+def test_alpha_pv_by_hand():
+    """get_alpha_pv.f90:35-57: 2 (highest occupied - lowest of all bands),
+    at least 0.01; the lowest is taken over every band, occupied or not."""
+    assert np.isclose(get_alpha_pv(np.array([-0.5, -0.2, 0.1, 0.4]), 2), 0.6)
+    # an empty band below the occupied ones sets the minimum
+    assert np.isclose(get_alpha_pv(np.array([-0.2, -0.5, 0.1, -0.7]), 2), 1.0)
+    assert get_alpha_pv(np.array([0.3, 0.3, 0.9]), 2) == 0.01
