@@ -447,7 +447,9 @@ void stern_apply32(const z32* x, z32* t, const double* d_e, int m, double alpha)
       else
         WH_TIMED("zinv32", K->zinv(c.plan.dev, items + q0, nq, x, l32, x, l32, sa, ZINV_PAIR, s));
       // variants 1..max_occ: LDS-pass kernel sized for that many workgroups
-      // per CU; above: register-line kernels
+      // per CU; above: register-line kernels.  Untimed small launches keep
+      // the LDS-pass kernel (the 6-H2 Davidson, mixed: 277 ms against 284 ms
+      // with the first register-line variant)
       const int nvar = K->max_occ + plane_reg_variants(c.plan.dev, true);
       WH_TIMED("plane_vloc32", plane_occ_launch(
                                    c.plan.vloc32_occ, nvar, nq,

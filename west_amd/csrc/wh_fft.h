@@ -75,21 +75,23 @@ struct StickPlan {
   long sticks_elems() const { return static_cast<long>(dev.n3) * dev.nct; }
 };
 
-// Occupancy variant of the plane kernel for a launch of nitems items: a fixed
-// choice (WH_PLANE_OCC or an earlier timing) or, for the first launch of at
-// least 256 items, the fastest of 1..max_occ timed on that launch itself
-// (the variants differ only in LDS chunking and give identical results; the
-// launch's output is the last variant's). run(occ) enqueues one launch.
+// Variant of the plane kernel for a launch of nitems items: a fixed choice
+// (WH_PLANE_OCC or an earlier timing) or, for the first launch of at least
+// 256 items, the fastest of 1..max_occ timed on that launch itself (the
+// variants compute the same transform; the launch's output is the last
+// variant's).  Smaller launches before any timing run variant `untimed`.
+// run(v) enqueues one launch.
 template <class Run>
-void plane_occ_launch(int& chosen, int max_occ, int nitems, Run run, hipStream_t s) {
+void plane_occ_launch(int& chosen, int max_occ, int nitems, Run run, hipStream_t s, int untimed = 1) {
   if (chosen > 0 || max_occ <= 1 || nitems < 256) {
-    run(chosen > 0 ? std::min(chosen, max_occ) : 1);
+    run(chosen > 0 ? std::min(chosen, max_occ) : std::min(std::max(untimed, 1), max_occ));
     return;
   }
   hipEvent_t a, b;
   WH_HIP(hipEventCreate(&a));
   WH_HIP(hipEventCreate(&b));
-  run(1);   // warm-up (first-launch costs)
+  // warm-up: each variant's first launch also loads its code
+  for (int occ = 1; occ <= max_occ; ++occ) run(occ);
   float best = 0.f;
   int pick = 1;
   for (int occ = 1; occ <= max_occ; ++occ) {

@@ -662,17 +662,21 @@ void launch_plane_vloc(const StickPlan& p, const FftItem* d_items, int nitems, c
   if (p.tpl) {
     WH_REQUIRE(static_cast<const void*>(sin) != static_cast<const void*>(sout), "plane_vloc: input and output alias");
     // variants 1..max_occ: LDS-pass kernel sized for that many workgroups per
-    // CU; above: register-line kernels
+    // CU; above: register-line kernels (2, 3, 4 planes per workgroup), the
+    // first of which also serves the untimed small launches (6-H2 Davidson,
+    // 60^3: 235 ms against 240 ms with the LDS-pass kernel; 24 clusters,
+    // concurrent: 1.35 s against 1.42 s)
     const int max_occ = p.tpl->max_occ;
+    const int nreg = plane_reg_variants(P, false);
     plane_occ_launch(
-        p.vloc_occ, max_occ + plane_reg_variants(P, false), nitems,
+        p.vloc_occ, max_occ + nreg, nitems,
         [&](int v) {
           if (v <= max_occ)
             p.tpl->vloc(P, d_items, nitems, sin, sout, vpot, ldpot, scale, v, s);
           else
             plane_reg_vloc(v - max_occ - 1, P, d_items, nitems, sin, sout, vpot, ldpot, scale, s);
         },
-        s);
+        s, nreg > 0 ? max_occ + 1 : 1);
     WH_HIP(hipGetLastError());
     return;
   }

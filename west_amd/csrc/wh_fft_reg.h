@@ -7,7 +7,7 @@
 namespace wh {
 
 // Number of register-line plane variants for this plan in FP32 (fp32) or
-// FP64 (0: the grid is not cubic of a compiled size -- FP64 lines stop at 72
+// FP64 (0: the grid is not cubic of a compiled size -- FP64 lines stop at 64
 // points -- or its sphere is too wide for them).
 int plane_reg_variants(const StickPlanDev& P, bool fp32);
 

@@ -91,7 +91,7 @@ constexpr int nt_reg() {
 // |i| <= IMX; rows beyond the sphere's |i| <= imax stay zero), so that the a1
 // lines load and store their rows at compile-time offsets without branches.
 template <typename T, int N, typename PT, int PZ>
-// FP64 lines need the accumulation registers as well (one wave per SIMD)
+// (FP64: the registers of one wave per SIMD are allowed for)
 __global__ __launch_bounds__((nt_reg<N, PZ>())) __attribute__((amdgpu_waves_per_eu((sizeof(T) == 8 ? 1 : 2)))) void k_plane_vloc_reg(
     StickPlanDev P, const FftItem* items, const cplx<T>* sin, cplx<T>* sout, const PT* vpot, long ldpot,
     double scale) {
@@ -425,11 +425,13 @@ void zfwd_n(const StickPlanDev& P, const FftItem* it, int n, const z32* st, cons
   k_zfwd_reg<N><<<dim3(ceil_div(P.ncs, kZregCols), n), ZNT, sm, s>>>(P, it, st, in, ldin, out, ldout, A);
 }
 
-// compiled line lengths: FP32, and FP64 up to 72 points (a 72-point FP64
-// line takes 288 of a thread's 512 vector and accumulation registers; longer
-// ones would spill)
+// compiled line lengths: FP32, and FP64 up to 64 points.  FP64 lines of 36
+// points or fewer fit the vector registers at 2-4 waves per SIMD; longer
+// ones also take accumulation registers and run one wave per SIMD, which
+// still beats the LDS-pass kernel up to 60 points (by 31 % at 60^3) but not
+// at 72 (5.4 against 5.0 ms per 2,206 planes), so 72 and up are not built.
 #define WH_REG_SIZES(X) X(18) X(20) X(24) X(30) X(32) X(36) X(40) X(45) X(48) X(50) X(54) X(60) X(64) X(72) X(75) X(80)
-#define WH_REG64_SIZES(X) X(18) X(20) X(24) X(30) X(32) X(36) X(40) X(45) X(48) X(50) X(54) X(60) X(64) X(72)
+#define WH_REG64_SIZES(X) X(18) X(20) X(24) X(30) X(32) X(36) X(40) X(45) X(48) X(50) X(54) X(60) X(64)
 
 bool compiled(int n, bool fp32) {
 #define WH_REG_CASE(NN) \
