@@ -86,6 +86,14 @@ MODULE west_hip
        INTEGER(C_INT), VALUE :: n_dfpt_maxiter, l_kinetic_only
      END FUNCTION
      !
+     ! dir: TRIM(wstat_restart_dir)//C_NULL_CHAR
+     INTEGER(C_INT) FUNCTION wh_set_davidson_restart(dir, n_steps_write_restart, restart_from) &
+         & BIND(C, NAME="wh_set_davidson_restart")
+       IMPORT :: C_INT, C_CHAR
+       CHARACTER(KIND=C_CHAR), DIMENSION(*), INTENT(IN) :: dir
+       INTEGER(C_INT), VALUE :: n_steps_write_restart, restart_from
+     END FUNCTION
+     !
      INTEGER(C_INT) FUNCTION wh_system_create(handle) BIND(C, NAME="wh_system_create")
        IMPORT :: C_INT
        INTEGER(C_INT), INTENT(OUT) :: handle

@@ -166,6 +166,20 @@ typedef struct {            /* exec.davitr[] of output_ev_and_time (davidson_dia
 int wh_davidson_diago(const wh_davidson_params* params, double* ev, int* conv, double* dvg_out,
                       wh_davidson_step* log, int log_capacity, int* log_len);
 
+/* Restart files of wh_davidson_diago, in the reference's format
+ * (Wstat/davidson_restart.f90:33-222, 360-410, 413-462, 515-760; the inputs
+ * wstat_restart_dir, n_steps_write_restart and wstat_calculation 'R' of
+ * Modules/westcom.f90).  With n_steps_write_restart > 0 the Davidson writes
+ * <dir>/summary.json (dav_iter, notcnv, nbase, conv, ev, ew), <dir>/hr_vr.dat
+ * (hr then vr, nvecx x nvecx, Fortran unformatted sequential records) and the
+ * basis <dir>/V%06d.dat, N%06d.dat (PDEP vector files: 32-integer header,
+ * npw complex coefficients in the caller's G order) after the lite pass and
+ * the second pass when it is 1 (davidson_diago.f90:198,249), then every
+ * n_steps_write_restart steps (421-422), and removes them on convergence
+ * (376).  restart_from = 1 starts the next wh_davidson_diago from those files
+ * instead of random vectors (158).  dir may be NULL when both are 0. */
+int wh_set_davidson_restart(const char* wstat_restart_dir, int n_steps_write_restart, int restart_from);
+
 /* ---- counters -------------------------------------------------------- */
 
 /* Operator applications so far (the 'stern' clock count of the reference,

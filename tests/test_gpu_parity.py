@@ -307,3 +307,65 @@ def test_pdep_db_from_davidson(gpu, gs_3h2, tmp_path):
     rq = dot(v, av) / dot(v, v)
     print("ev", ev, "rayleigh", rq, flush=True)
     assert np.max(np.abs(rq - ev) / np.abs(ev)) < 1e-6
+
+
+def test_davidson_restart(gpu, gs_3h2, tmp_path):
+    """Davidson restart files (Wstat/davidson_restart.f90): a run stopped
+    after 3 steps leaves summary.json, hr_vr.dat (two Fortran records) and the
+    basis V/N files of the step-2 state; the files agree with each other (the
+    V rows orthonormal, hr(i,j) = <V_i|N_j> for i <= j < nbase), and a run
+    restarted from them continues the stopped one: the same dims trajectory
+    and eigenvalues as an uninterrupted run, with the files removed once it
+    converges."""
+    import json
+    import os
+    import warnings
+    from west_amd import pdep_db
+    gs = gs_3h2
+    _register(gpu, gs)
+    b = gs.basis
+    nvec, ntimes = 4, 4
+    nvecx = nvec * ntimes
+    kw = dict(tr2_dfpt=1e-20, trev_pdep=1e-6, trev_pdep_rel=1e-4, n_pdep_times=ntimes, want_vectors=False)
+    ev_full, conv_full, _, log_full = gpu.davidson_diago(nvec, **kw)
+    assert all(conv_full) and len(log_full) > 5
+    rdir = str(tmp_path / "wstat.restart")
+    try:
+        gpu.set_davidson_restart(rdir, n_steps_write_restart=1)
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            _, _, _, log_a = gpu.davidson_diago(nvec, n_pdep_maxiter=3, **kw)
+        doc = json.load(open(os.path.join(rdir, "summary.json")))
+        nbase = doc["nbase"]
+        print("summary", {k: doc[k] for k in ("dav_iter", "notcnv", "nbase")}, flush=True)
+        assert doc["dav_iter"] == 2 and nbase == 3 * nvec
+        assert len(doc["conv"]) == nvec and len(doc["ev"]) == nvec and len(doc["ew"]) == nvecx
+        raw = open(os.path.join(rdir, "hr_vr.dat"), "rb").read()
+        rec = nvecx * nvecx * 8
+        assert len(raw) == 2 * (rec + 8)
+        marks = np.frombuffer(raw[:4] + raw[4 + rec:8 + rec] + raw[8 + rec:12 + rec] + raw[-4:], dtype=np.int32)
+        assert np.all(marks == rec)
+        hr = np.frombuffer(raw[4:4 + rec], dtype=np.float64).reshape(nvecx, nvecx).T   # hr(i, j), column major
+        V = np.array([pdep_db.read_vector(os.path.join(rdir, f"V{j:06d}.dat"), b.npw) for j in range(1, nbase + 1)])
+        N = np.array([pdep_db.read_vector(os.path.join(rdir, f"N{j:06d}.dat"), b.npw) for j in range(1, nbase + 1)])
+        assert not os.path.exists(os.path.join(rdir, f"V{nbase + 1:06d}.dat"))
+
+        def gram(x, y):   # <x_i|y_j> at Gamma
+            return 2 * np.real(np.conj(x) @ y.T) - np.real(np.outer(np.conj(x[:, 0]), y[:, 0]))
+
+        assert np.abs(gram(V, V) - np.eye(nbase)).max() < 1e-12
+        hv = gram(V, N)
+        iu = np.triu_indices(nbase)
+        assert np.abs(hr[:nbase, :nbase][iu] - hv[iu]).max() < 1e-12 * np.abs(hv).max()
+
+        gpu.set_davidson_restart(rdir, n_steps_write_restart=0, restart_from=True)
+        ev_b, conv_b, _, log_b = gpu.davidson_diago(nvec, **kw)
+        print("ev", ev_b, "uninterrupted", ev_full, flush=True)
+        assert log_b[0]["dav_iter"] == 3
+        dims = [(r["dav_iter"], r["dfpt_dim"], r["diago_dim"]) for r in log_full]
+        assert [(r["dav_iter"], r["dfpt_dim"], r["diago_dim"]) for r in log_a] == dims[:4]
+        assert [(r["dav_iter"], r["dfpt_dim"], r["diago_dim"]) for r in log_b] == dims[3:]
+        assert all(conv_b) and np.max(np.abs(ev_b - ev_full) / np.abs(ev_full)) < 1e-9
+        assert not os.path.exists(rdir)
+    finally:
+        gpu.set_davidson_restart(None, 0, False)

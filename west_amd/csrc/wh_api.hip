@@ -1068,6 +1068,19 @@ int wh_set_params(int n_dfpt_maxiter, int l_kinetic_only) {
   WH_API_END
 }
 
+int wh_set_davidson_restart(const char* wstat_restart_dir, int n_steps_write_restart, int restart_from) {
+  WH_API_BEGIN
+  Context& c = ctx();
+  WH_REQUIRE(n_steps_write_restart >= 0, "n_steps_write_restart must be >= 0");
+  WH_REQUIRE(!(n_steps_write_restart > 0 || restart_from) || (wstat_restart_dir && wstat_restart_dir[0]),
+             "a restart directory is required");
+  c.restart_dir = wstat_restart_dir ? wstat_restart_dir : "";
+  c.n_steps_write_restart = n_steps_write_restart;
+  c.restart_from = restart_from ? 1 : 0;
+  return 0;
+  WH_API_END
+}
+
 int wh_apply_sternheimerop_to_m_wfcs(int nbndval, const double* psi, double* hpsi, const double* e,
                                      const double* alpha, int m) {
   WH_API_BEGIN

@@ -118,6 +118,11 @@ struct Context {
   bool evc_sticks_valid = false;
   // params
   int n_dfpt_maxiter = 250;
+  // Davidson restart files (wstat_restart_dir, n_steps_write_restart and a
+  // restart run, Modules/westcom.f90; Wstat/davidson_restart.f90)
+  std::string restart_dir;
+  int n_steps_write_restart = 0;
+  int restart_from = 0;
   int l_kinetic_only = 0;
   size_t sticks_budget = size_t(4) << 30;   // bytes for transform intermediates
   size_t vectors_budget = size_t(64) << 30;  // bytes for the CG vectors of one dfpt chunk

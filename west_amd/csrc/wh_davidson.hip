@@ -16,9 +16,18 @@
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <chrono>
 #include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
 
 #include "../../include/west_hip.h"
 #include "wh_context.h"
@@ -153,6 +162,160 @@ void do_mgs(z64* amat, int ms, int me) {
   }
 }
 
+// ---- restart files (Wstat/davidson_restart.f90) ---------------------------
+
+constexpr int kHdLength = 32;          // IO_kernel/mod_west_io.f90:31-35
+constexpr int kHdVersion = 210405;
+
+bool little_endian() {
+  const uint32_t one = 1;
+  unsigned char b;
+  std::memcpy(&b, &one, 1);
+  return b == 1;
+}
+
+std::string vec_name(const std::string& dir, char kind, int j) {   // V/N + i6.6 (davidson_restart.f90:151-203)
+  char buf[16];
+  std::snprintf(buf, sizeof buf, "%c%06d.dat", kind, j);
+  return dir + "/" + buf;
+}
+
+// one basis vector: header, then n complex coefficients (pdep_merge_and_write_G,
+// IO_kernel/pdep_io.f90:115-141)
+void write_vec(const std::string& fname, const z64* v, int n) {
+  int32_t hd[kHdLength] = {};
+  hd[0] = kHdVersion;
+  hd[1] = little_endian() ? 1 : 0;
+  hd[2] = n;
+  FILE* f = std::fopen(fname.c_str(), "wb");
+  WH_REQUIRE(f != nullptr, "davidson_restart_write: cannot open file: " + fname);
+  const bool ok = std::fwrite(hd, sizeof hd, 1, f) == 1 && std::fwrite(v, sizeof(z64), n, f) == static_cast<size_t>(n);
+  std::fclose(f);
+  WH_REQUIRE(ok, "davidson_restart_write: cannot write file: " + fname);
+}
+
+// pdep_read_G_and_distribute (pdep_io.f90:254-285), with its checks
+void read_vec(const std::string& fname, z64* v, int n) {
+  FILE* f = std::fopen(fname.c_str(), "rb");
+  WH_REQUIRE(f != nullptr, "pdep_read: Cannot read file: " + fname);
+  int32_t hd[kHdLength];
+  const bool got = std::fread(hd, sizeof hd, 1, f) == 1;
+  const bool ok_version = got && hd[0] == kHdVersion;
+  const bool ok_dim = ok_version && hd[2] == n;
+  const bool ok_endian = ok_dim && (hd[1] == 1) == little_endian();
+  const bool ok_data = ok_endian && std::fread(v, sizeof(z64), n, f) == static_cast<size_t>(n);
+  std::fclose(f);
+  WH_REQUIRE(ok_version, "pdep_read: Unknown file format: " + fname);
+  WH_REQUIRE(ok_dim, "pdep_read: Dimension mismatch: " + fname);
+  WH_REQUIRE(ok_endian, "pdep_read: Endianness mismatch: " + fname);
+  WH_REQUIRE(ok_data, "pdep_read: Cannot read file: " + fname);
+}
+
+// hr_vr.dat: two Fortran unformatted sequential records, hr then vr
+// (davidson_restart.f90:117-130; 4-byte length markers around each record)
+void write_hr_vr(const std::string& dir, const std::vector<double>& hr, const std::vector<double>& vr) {
+  const std::string fname = dir + "/hr_vr.dat";
+  const size_t bytes = hr.size() * sizeof(double);
+  WH_REQUIRE(bytes < (size_t(1) << 31), "davidson_restart_write: hr too large for one record");
+  const int32_t mark = static_cast<int32_t>(bytes);
+  FILE* f = std::fopen(fname.c_str(), "wb");
+  WH_REQUIRE(f != nullptr, "davidson_restart_write: cannot open file: " + fname);
+  bool ok = true;
+  for (const std::vector<double>* a : {&hr, &vr})
+    ok = ok && std::fwrite(&mark, 4, 1, f) == 1 && std::fwrite(a->data(), 1, bytes, f) == bytes &&
+         std::fwrite(&mark, 4, 1, f) == 1;
+  std::fclose(f);
+  WH_REQUIRE(ok, "davidson_restart_write: cannot write file: " + fname);
+}
+
+void read_hr_vr(const std::string& dir, std::vector<double>& hr, std::vector<double>& vr) {
+  const std::string fname = dir + "/hr_vr.dat";
+  FILE* f = std::fopen(fname.c_str(), "rb");
+  WH_REQUIRE(f != nullptr, "davidson_restart_read: cannot read file: " + fname);
+  const size_t bytes = hr.size() * sizeof(double);
+  bool ok = true;
+  for (std::vector<double>* a : {&hr, &vr}) {
+    int32_t m0 = 0, m1 = 0;
+    ok = ok && std::fread(&m0, 4, 1, f) == 1 && static_cast<size_t>(m0) == bytes &&
+         std::fread(a->data(), 1, bytes, f) == bytes && std::fread(&m1, 4, 1, f) == 1 && m1 == m0;
+  }
+  std::fclose(f);
+  WH_REQUIRE(ok, "davidson_restart_read: hr_vr.dat does not hold two records of n_pdep_basis^2 reals: " + fname);
+}
+
+// summary.json (davidson_restart.f90:104-115): dav_iter, notcnv, nbase, conv, ev, ew
+void write_summary(const std::string& dir, int dav_iter, int notcnv, int nbase, const std::vector<char>& conv,
+                   const std::vector<double>& ev, const std::vector<double>& ew) {
+  const std::string fname = dir + "/summary.json";
+  FILE* f = std::fopen(fname.c_str(), "w");
+  WH_REQUIRE(f != nullptr, "davidson_restart_write: cannot open file: " + fname);
+  std::fprintf(f, "{\n  \"dav_iter\": %d,\n  \"notcnv\": %d,\n  \"nbase\": %d,\n  \"conv\": [", dav_iter, notcnv, nbase);
+  for (size_t i = 0; i < conv.size(); ++i) std::fprintf(f, "%s%s", i ? ", " : "", conv[i] ? "true" : "false");
+  std::fprintf(f, "],\n  \"ev\": [");
+  for (size_t i = 0; i < ev.size(); ++i) std::fprintf(f, "%s%.17g", i ? ", " : "", ev[i]);
+  std::fprintf(f, "],\n  \"ew\": [");
+  for (size_t i = 0; i < ew.size(); ++i) std::fprintf(f, "%s%.17g", i ? ", " : "", ew[i]);
+  const bool ok = std::fprintf(f, "]\n}\n") > 0;
+  std::fclose(f);
+  WH_REQUIRE(ok, "davidson_restart_write: cannot write file: " + fname);
+}
+
+// the value text after "key": in a flat JSON object (the summary's layout)
+const char* json_value(const std::string& doc, const char* key) {
+  const std::string k = std::string("\"") + key + "\"";
+  const size_t at = doc.find(k);
+  if (at == std::string::npos) return nullptr;
+  const size_t colon = doc.find(':', at + k.size());
+  return colon == std::string::npos ? nullptr : doc.c_str() + colon + 1;
+}
+
+bool json_int(const std::string& doc, const char* key, int& out) {
+  const char* p = json_value(doc, key);
+  if (!p) return false;
+  char* end = nullptr;
+  const long v = std::strtol(p, &end, 10);
+  if (end == p) return false;
+  out = static_cast<int>(v);
+  return true;
+}
+
+// array of numbers or of true/false
+bool json_array(const std::string& doc, const char* key, std::vector<double>& out) {
+  const char* p = json_value(doc, key);
+  if (!p) return false;
+  while (*p == ' ' || *p == '\n' || *p == '\r' || *p == '\t') ++p;
+  if (*p != '[') return false;
+  ++p;
+  out.clear();
+  for (;;) {
+    while (*p == ' ' || *p == '\n' || *p == '\r' || *p == '\t' || *p == ',') ++p;
+    if (*p == ']') return true;
+    if (std::strncmp(p, "true", 4) == 0) {
+      out.push_back(1.0);
+      p += 4;
+    } else if (std::strncmp(p, "false", 5) == 0) {
+      out.push_back(0.0);
+      p += 5;
+    } else {
+      char* end = nullptr;
+      const double v = std::strtod(p, &end);
+      if (end == p) return false;
+      out.push_back(v);
+      p = end;
+    }
+  }
+}
+
+void remove_restart(const std::string& dir, int n_pdep_basis) {   // davidson_restart_clear (360-410)
+  std::remove((dir + "/summary.json").c_str());
+  std::remove((dir + "/hr_vr.dat").c_str());
+  for (int j = 1; j <= n_pdep_basis; ++j) {
+    std::remove(vec_name(dir, 'V', j).c_str());
+    std::remove(vec_name(dir, 'N', j).c_str());
+  }
+  ::rmdir(dir.c_str());
+}
+
 double now_s() {
   using namespace std::chrono;
   return duration<double>(steady_clock::now().time_since_epoch()).count();
@@ -271,23 +434,86 @@ extern "C" int wh_davidson_diago(const wh_davidson_params* prm, double* ev_out, 
       WH_HIP(hipStreamSynchronize(s));
     };
 
-    // ---- from scratch (davidson_diago.f90:167-196)
-    k_randomize<<<dim3(ceil_div(npw, NT), nvec), NT, 0, s>>>(npw, 1, nvec, c.d_perm, c.d_g2kin, 1.0 / c.tpiba2, dvg, ld);
-    WH_HIP(hipGetLastError());
-    do_mgs(dvg, 0, nvec);
-    apply_op(0, nvec, -1.0);
-    // ---- extra step (205-249)
-    tit = now_s();
-    ns0 = c.nstern;
-    WH_HIP(hipMemcpyAsync(dvg, dng, sizeof(z64) * nvec * npw, hipMemcpyDeviceToDevice, s));
-    do_mgs(dvg, 0, nvec);
+    // ---- restart files (Wstat/davidson_restart.f90): the state after a step,
+    // basis rows in the caller's G order
+    const std::string rdir = c.restart_dir;
+    const int nsw = c.n_steps_write_restart;
+    auto restart_write = [&](int dav_iter_w) {   // davidson_restart_write_real (33-222)
+      ::mkdir(rdir.c_str(), 0755);
+      write_summary(rdir, dav_iter_w, notcnv, nbase, conv, ev, ew);
+      write_hr_vr(rdir, hr, vr);
+      const size_t cin = static_cast<size_t>(nbase) * c.npwx;
+      std::vector<z64> h(cin);
+      z64* o = c.w_out.as<z64>(cin);
+      for (int k = 0; k < 2; ++k) {
+        scatter_caller(nbase, npw, c.npwx, c.d_perm, k ? dng : dvg, ld, o, c.npwx, s);
+        WH_HIP(hipMemcpyAsync(h.data(), o, sizeof(z64) * cin, hipMemcpyDeviceToHost, s));
+        WH_HIP(hipStreamSynchronize(s));
+        for (int j = 0; j < nbase; ++j)
+          write_vec(vec_name(rdir, k ? 'N' : 'V', j + 1), h.data() + static_cast<size_t>(j) * c.npwx, npw);
+      }
+    };
+    auto restart_read = [&]() {   // davidson_restart_read_real (413-462)
+      std::ifstream in(rdir + "/summary.json");
+      WH_REQUIRE(in.good(), "davidson_restart_read: cannot read file: " + rdir + "/summary.json");
+      std::stringstream ss;
+      ss << in.rdbuf();
+      const std::string doc = ss.str();
+      int di = 0;
+      std::vector<double> a_conv, a_ev, a_ew;
+      WH_REQUIRE(json_int(doc, "dav_iter", di) && json_int(doc, "notcnv", notcnv) && json_int(doc, "nbase", nbase) &&
+                     json_array(doc, "conv", a_conv) && json_array(doc, "ev", a_ev) && json_array(doc, "ew", a_ew),
+                 "davidson_restart_read: summary.json lacks dav_iter, notcnv, nbase, conv, ev or ew");
+      WH_REQUIRE(static_cast<int>(a_conv.size()) >= nvec && static_cast<int>(a_ev.size()) == nvec &&
+                     static_cast<int>(a_ew.size()) == nvecx && nbase >= nvec && nbase <= nvecx && notcnv >= 0 &&
+                     notcnv <= nvec && di >= -1,
+                 "davidson_restart_read: the restart does not match n_pdep_eigen and n_pdep_times");
+      for (int n = 0; n < nvec; ++n) {
+        conv[n] = a_conv[n] != 0.0;
+        ev[n] = a_ev[n];
+      }
+      for (int n = 0; n < nvecx; ++n) ew[n] = a_ew[n];
+      read_hr_vr(rdir, hr, vr);
+      const size_t cin = static_cast<size_t>(nbase) * c.npwx;
+      std::vector<z64> h(cin, z64{0.0, 0.0});
+      z64* o = c.w_out.as<z64>(cin);
+      for (int k = 0; k < 2; ++k) {
+        for (int j = 0; j < nbase; ++j)
+          read_vec(vec_name(rdir, k ? 'N' : 'V', j + 1), h.data() + static_cast<size_t>(j) * c.npwx, npw);
+        WH_HIP(hipMemcpyAsync(o, h.data(), sizeof(z64) * cin, hipMemcpyHostToDevice, s));
+        gather_sticks(nbase, npw, c.d_perm, o, c.npwx, k ? dng : dvg, ld, nullptr, s);
+        WH_HIP(hipStreamSynchronize(s));
+      }
+      return di;
+    };
+
+    int dav_iter = -1;
+    if (c.restart_from) {
+      dav_iter = restart_read();   // (154-159)
+    } else {
+      // ---- from scratch (davidson_diago.f90:167-196)
+      k_randomize<<<dim3(ceil_div(npw, NT), nvec), NT, 0, s>>>(npw, 1, nvec, c.d_perm, c.d_g2kin, 1.0 / c.tpiba2,
+                                                                dvg, ld);
+      WH_HIP(hipGetLastError());
+      do_mgs(dvg, 0, nvec);
+      apply_op(0, nvec, -1.0);
+      if (nsw == 1) restart_write(dav_iter);
+    }
     double tr2 = std::min(1.0e-4, 1.0e8 * prm->tr2_dfpt);
-    apply_op(0, nvec, tr2);
-    build_hr(0, nvec, nvec);
-    diagox(nbase);
-    for (int i = 0; i < nvec; ++i) ev[i] = ew[i];
-    record(0, tr2, nbase, nbase);
-    int dav_iter = 0;
+    if (dav_iter == -1) {
+      // ---- extra step (205-249)
+      tit = now_s();
+      ns0 = c.nstern;
+      WH_HIP(hipMemcpyAsync(dvg, dng, sizeof(z64) * nvec * npw, hipMemcpyDeviceToDevice, s));
+      do_mgs(dvg, 0, nvec);
+      apply_op(0, nvec, tr2);
+      build_hr(0, nvec, nvec);
+      diagox(nbase);
+      for (int i = 0; i < nvec; ++i) ev[i] = ew[i];
+      record(0, tr2, nbase, nbase);
+      dav_iter = 0;
+      if (nsw == 1) restart_write(dav_iter);
+    }
     bool converged = false;
     for (int kter = 1; kter <= prm->n_pdep_maxiter; ++kter) {   // (259-424)
       tit = now_s();
@@ -349,8 +575,9 @@ extern "C" int wh_davidson_diago(const wh_davidson_params* prm, double* ev_out, 
           vr[static_cast<size_t>(i) * nvecx + i] = 1.0;
         }
       }
+      if (nsw > 0 && dav_iter % nsw == 0) restart_write(dav_iter);   // (421-422)
     }
-    (void)converged;
+    if (converged && !rdir.empty()) remove_restart(rdir, nvecx);   // davidson_restart_clear (376)
     std::memcpy(ev_out, ev.data(), sizeof(double) * nvec);
     if (conv_out)
       for (int n = 0; n < nvec; ++n) conv_out[n] = conv[n] ? 1 : 0;

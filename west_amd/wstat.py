@@ -9,6 +9,7 @@ Names, argument meaning and error behaviour follow the reference:
 * :func:`dfpt`                          -- ``DFPT_kernel/dfpt_module.f90:22``
 * :func:`apply_operator`                -- ``Wstat/apply_operator.f90:14``
 * :func:`davidson_diago`                -- ``Wstat/davidson_diago.f90:31``
+* :func:`set_davidson_restart`          -- ``Wstat/davidson_restart.f90`` (inputs of ``Modules/westcom.f90``)
 
 Arrays are numpy ``(m, npwx)`` complex128 (the memory layout of the
 reference's Fortran ``(npwx, m)``) or torch tensors of the same shape on the
@@ -168,6 +169,18 @@ def davidson_diago(n_pdep_eigen: int, n_pdep_times: int = 4, n_pdep_maxiter: int
     if rc > 0:
         warnings.warn(f"WARNING : {rc} eigenvalues not converged in chidiago")
     return ev, conv.astype(bool), dvg, recs
+
+
+def set_davidson_restart(wstat_restart_dir: str | None, n_steps_write_restart: int = 0,
+                         restart_from: bool = False):
+    """Restart files of :func:`davidson_diago` in the reference's format
+    (``Wstat/davidson_restart.f90``): written to ``wstat_restart_dir`` every
+    ``n_steps_write_restart`` steps (and after both starting passes when 1),
+    removed on convergence; ``restart_from`` starts the next Davidson of the
+    selected system from them (``wstat_calculation = 'R'``)."""
+    lib = load()
+    d = None if wstat_restart_dir is None else str(wstat_restart_dir).encode()
+    check(lib.wh_set_davidson_restart(d, int(n_steps_write_restart), 1 if restart_from else 0))
 
 
 def davidson_batch(jobs, concurrent: bool = True, max_workers: int | None = None):
