@@ -369,3 +369,27 @@ def test_davidson_restart(gpu, gs_3h2, tmp_path):
         assert not os.path.exists(rdir)
     finally:
         gpu.set_davidson_restart(None, 0, False)
+
+
+def test_qe_run_system(gpu):
+    """The SiH4 run of the reference's test-suite (tests/golden/qe/qe_test001.xml:
+    its geometry, cell, cutoffs and 64^3 grid) through the path: the ground
+    state computed here on that basis, then the operator and the lite dfpt
+    against the oracle."""
+    from pathlib import Path
+
+    from oracle.dfpt import dfpt
+    from oracle.hamiltonian import apply_sternheimerop
+    from west_amd.groundstate import make_ground_state
+    from west_amd.qe_xml import read_pw_xml, system_from_pw_xml
+    s = system_from_pw_xml(read_pw_xml(str(Path(__file__).resolve().parent / "golden" / "qe" / "qe_test001.xml")))
+    gs = make_ground_state(s, gpu.gpu_h_factory, nbnd=s.nbnd_occ + 2, scf_steps=0, tol=1e-6)
+    _register(gpu, gs)
+    print(f"case {s.name} nr={s.basis.nr} npw={s.basis.npw} et={gs.et}", flush=True)
+    psi = random_wfc(gs, 3, seed=5)
+    e = np.array([-0.8, -0.3, 0.1])
+    ref = apply_sternheimerop(gs, psi, e, gs.alpha_pv)
+    out = gpu.apply_sternheimerop_to_m_wfcs(gs.nbndval, psi, e, gs.alpha_pv)
+    assert _rel(out, ref) < 1e-12
+    dvg = random_wfc(gs, 2, seed=6)
+    assert _rel(gpu.dfpt(2, dvg, -1.0), dfpt(gs, dvg, -1.0)) < 1e-12
