@@ -176,8 +176,10 @@ int wh_davidson_diago(const wh_davidson_params* params, double* ev, int* conv, d
  * npw complex coefficients in the caller's G order) after the lite pass and
  * the second pass when it is 1 (davidson_diago.f90:198,249), then every
  * n_steps_write_restart steps (421-422), and removes them on convergence
- * (376).  restart_from = 1 starts the next wh_davidson_diago from those files
- * instead of random vectors (158).  dir may be NULL when both are 0. */
+ * (376).  restart_from = 1 starts the next wh_davidson_diago (that call only;
+ * the ones after it start from scratch, as davidson_diago.f90:613 does) from
+ * those files instead of random vectors (158).  dir may be NULL when both
+ * are 0. */
 int wh_set_davidson_restart(const char* wstat_restart_dir, int n_steps_write_restart, int restart_from);
 
 /* ---- counters -------------------------------------------------------- */

@@ -367,6 +367,9 @@ def test_davidson_restart(gpu, gs_3h2, tmp_path):
         assert [(r["dav_iter"], r["dfpt_dim"], r["diago_dim"]) for r in log_b] == dims[3:]
         assert all(conv_b) and np.max(np.abs(ev_b - ev_full) / np.abs(ev_full)) < 1e-9
         assert not os.path.exists(rdir)
+        # the restart applied to that run only: the next one starts from scratch
+        ev_c, _, _, log_c = gpu.davidson_diago(nvec, **kw)
+        assert log_c[0]["dav_iter"] == 0 and np.max(np.abs(ev_c - ev_full) / np.abs(ev_full)) < 1e-9
     finally:
         gpu.set_davidson_restart(None, 0, False)
 

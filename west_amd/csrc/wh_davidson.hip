@@ -490,6 +490,7 @@ extern "C" int wh_davidson_diago(const wh_davidson_params* prm, double* ev_out, 
     int dav_iter = -1;
     if (c.restart_from) {
       dav_iter = restart_read();   // (154-159)
+      c.restart_from = 0;          // later runs start from scratch (as after a restart, 613)
     } else {
       // ---- from scratch (davidson_diago.f90:167-196)
       k_randomize<<<dim3(ceil_div(npw, NT), nvec), NT, 0, s>>>(npw, 1, nvec, c.d_perm, c.d_g2kin, 1.0 / c.tpiba2,

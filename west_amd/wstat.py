@@ -177,7 +177,7 @@ def set_davidson_restart(wstat_restart_dir: str | None, n_steps_write_restart: i
     (``Wstat/davidson_restart.f90``): written to ``wstat_restart_dir`` every
     ``n_steps_write_restart`` steps (and after both starting passes when 1),
     removed on convergence; ``restart_from`` starts the next Davidson of the
-    selected system from them (``wstat_calculation = 'R'``)."""
+    selected system from them (``wstat_calculation = 'R'``; that run only)."""
     lib = load()
     d = None if wstat_restart_dir is None else str(wstat_restart_dir).encode()
     check(lib.wh_set_davidson_restart(d, int(n_steps_write_restart), 1 if restart_from else 0))
