@@ -396,3 +396,39 @@ def test_qe_run_system(gpu):
     assert _rel(out, ref) < 1e-12
     dvg = random_wfc(gs, 2, seed=6)
     assert _rel(gpu.dfpt(2, dvg, -1.0), dfpt(gs, dvg, -1.0)) < 1e-12
+
+
+def test_edge_cases_and_errors(gpu, gs_si8):
+    """Empty batches, a batch smaller than its arrays, and the errors the
+    reference raises through errore (non-Gamma iq, inconsistent band counts,
+    a Davidson basis too small for nvec: davidson_diago.f90:108) surface as
+    exceptions carrying the library's message, not as crashes."""
+    from oracle.hamiltonian import apply_sternheimerop
+    from west_amd.lib import WestHipError, check, load
+    gs = gs_si8
+    _register(gpu, gs)
+    empty = np.zeros((0, gs.npwx), dtype=np.complex128)
+    assert gpu.apply_sternheimerop_to_m_wfcs(gs.nbndval, empty, np.zeros(0), 0.5).shape == (0, gs.npwx)
+    assert gpu.dfpt(0, empty, 1e-12).shape == (0, gs.npwx)
+    assert gpu.apply_operator(0, empty, 1e-12).shape == (0, gs.npwx)
+    assert gpu.linsolve_sternheimer_m_wfcts(gs.nbndval, 0, empty.copy(), empty.copy(), np.zeros(0), np.zeros(0),
+                                            1e-12) == 0
+    # m below the rows of the arrays: only the first m columns are formed
+    psi = random_wfc(gs, 5, seed=21)
+    e = np.linspace(-0.5, 0.1, 5)
+    out = gpu.apply_sternheimerop_to_m_wfcs(gs.nbndval, psi, e, gs.alpha_pv, m=3)
+    assert _rel(out[:3], apply_sternheimerop(gs, psi[:3], e[:3], gs.alpha_pv)) < 1e-12
+    assert not out[3:].any()
+    with pytest.raises(WestHipError, match="iq = 1"):
+        gpu.dfpt(1, psi[:1], 1e-12, iq=2)
+    with pytest.raises(WestHipError, match="nbndval"):
+        gpu.apply_sternheimerop_to_m_wfcs(gs.nbndval + 1, psi, e, gs.alpha_pv)
+    with pytest.raises(WestHipError, match="nvecx"):
+        gpu.davidson_diago(4, n_pdep_times=1)
+    with pytest.raises(WestHipError, match="directory"):
+        gpu.set_davidson_restart(None, 1)
+    with pytest.raises(WestHipError, match="precision mode"):
+        check(load().wh_set_precision(3, 0.0))
+    # the library still works after the errors
+    out = gpu.apply_sternheimerop_to_m_wfcs(gs.nbndval, psi, e, gs.alpha_pv)
+    assert _rel(out, apply_sternheimerop(gs, psi, e, gs.alpha_pv)) < 1e-12
