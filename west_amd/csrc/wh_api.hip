@@ -78,10 +78,10 @@ static void free_context(Context* cp) {
                     &c.w_epcol, &c.w_dscr, &c.m_vrs32, &c.m_q32, &c.m_qp[0], &c.m_qp[1], &c.m_qp[2], &c.m_qtp[0],
                     &c.m_qtp[1], &c.m_qtp[2], &c.m_g, &c.m_h, &c.m_hold, &c.m_dx, &c.m_bc, &c.m_tc, &c.m_cm,
                     &c.m_work, &c.m_u2i, &c.m_rho64, &c.m_eu64, &c.m_fb, &c.m_fx, &c.m_fe, &c.m_fep,
-                    &c.w_rhob})
+                    &c.w_rhob, &c.w_cgpk})
     b->release();
   for (HostBuf* b : {&c.h_l2i, &c.h_beta, &c.h_eu, &c.h_rho, &c.hm_l2i, &c.hm_beta, &c.hm_eu, &c.hm_rho, &c.hm_eu64,
-                     &c.hm_u})
+                     &c.hm_u, &c.h_cgpk})
     b->release();
   for (DevBuf& b : c.dav) b.release();
   (void)hipStreamDestroy(c.stream);
@@ -269,9 +269,13 @@ int linsolve_dev(int m, z64* b, z64* x, const std::vector<double>& e_col, const 
   z64* hold = c.w_hold.as<z64>(vec);
   z64* bc = c.w_bc.as<z64>(vec);
   z64* tc = c.w_tc.as<z64>(vec);
-  int* d_l2i = c.w_l2i.as<int>(m);
-  double* d_beta = c.w_beta.as<double>(m);
-  double* d_eu = c.w_eu.as<double>(m);
+  // each iteration's active list, beta and e go to the device in one copy,
+  // laid out for that iteration's nc: [beta (nc) | e (nc) | list (nc ints)]
+  double* d_pk = c.w_cgpk.as<double>(static_cast<size_t>(5) * m / 2 + 1);
+  double* pk = c.h_cgpk.as<double>(static_cast<size_t>(5) * m / 2 + 1);
+  int* d_l2i = c.w_l2i.as<int>(m);   // the first residual's list; then inside d_pk
+  double* d_beta = nullptr;
+  double* d_eu = nullptr;
   // rho of each band's current g, by band: from prec_rho for the first
   // residual, then from the update kernel of the previous iteration
   double* d_rhob = c.w_rhob.as<double>(m);
@@ -283,8 +287,6 @@ int linsolve_dev(int m, z64* b, z64* x, const std::vector<double>& e_col, const 
   // pinned: the lists and coefficients go to the device with asynchronous copies
   int* l2i = c.h_l2i.as<int>(m);
   double* rho_c = c.h_rho.as<double>(m);
-  double* beta = c.h_beta.as<double>(m);
-  double* eu = c.h_eu.as<double>(m);
   std::iota(l2i, l2i + m, 0);
   std::vector<double> rho(m, 0.0), rhoold(m, 0.0);
   int nc = m;
@@ -327,14 +329,17 @@ int linsolve_dev(int m, z64* b, z64* x, const std::vector<double>& e_col, const 
       if (!is_conv[ib]) l2i[n2++] = ib;
     nc = n2;
     if (nc == 0) break;
+    int* pl = reinterpret_cast<int*>(pk + 2 * nc);
     for (int lb = 0; lb < nc; ++lb) {
       const int ib = l2i[lb];
-      beta[lb] = (iter != 1) ? rho[ib] / rhoold[ib] : 0.0;
-      eu[lb] = e_col[ib];
+      pk[lb] = (iter != 1) ? rho[ib] / rhoold[ib] : 0.0;   // beta
+      pk[nc + lb] = e_col[ib];
+      pl[lb] = ib;
     }
-    WH_HIP(hipMemcpyAsync(d_l2i, l2i, sizeof(int) * nc, hipMemcpyHostToDevice, s));
-    WH_HIP(hipMemcpyAsync(d_beta, beta, sizeof(double) * nc, hipMemcpyHostToDevice, s));
-    WH_HIP(hipMemcpyAsync(d_eu, eu, sizeof(double) * nc, hipMemcpyHostToDevice, s));
+    WH_HIP(hipMemcpyAsync(d_pk, pk, (2 * sizeof(double) + sizeof(int)) * nc, hipMemcpyHostToDevice, s));
+    d_beta = d_pk;
+    d_eu = d_pk + nc;
+    d_l2i = reinterpret_cast<int*>(d_pk + 2 * nc);
     auto body = [&]() {
       WH_TIMED("cg_dir", direction(nc, npw, d_l2i, d_beta, g, c.d_g2kin, d_eprec_col, h, hold, bc, ld, s));
       stern_apply(bc, tc, d_eu, nc, c.alpha_pv);

@@ -131,6 +131,9 @@ struct Context {
   DevBuf w_x, w_t, w_g, w_h, w_hold, w_bc, w_tc, w_b, w_dvr, w_in, w_out, w_dvin, w_dng;
   DevBuf w_l2i, w_beta, w_eu, w_rho, w_rhob, w_ecol, w_epcol, w_dscr;
   HostBuf h_l2i, h_beta, h_eu, h_rho;   // pinned staging of the FP64 CG loop
+  // the FP64 CG's per-iteration upload, one copy: [beta (nc) | e (nc) | list (nc ints)]
+  DevBuf w_cgpk;
+  HostBuf h_cgpk;
   HostBuf hm_l2i, hm_beta, hm_eu, hm_rho, hm_eu64, hm_u;   // and of the mixed-precision loop
   // mixed precision: FP32 operator with split-BF16 projectors, FP64 refinement
   int precision = 0;              // 0: FP64, 1: mixed (3-product split BF16), 2: mixed (6-product)
