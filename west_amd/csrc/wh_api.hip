@@ -511,10 +511,12 @@ int linsolve_mixed(int m, z64* b, z64* x, const std::vector<double>& e_col, cons
   z32* tc = c.m_tc.as<z32>(v32);
   z64* xc = c.w_bc.as<z64>(static_cast<size_t>(m) * npw);
   z64* tc64 = c.w_tc.as<z64>(static_cast<size_t>(m) * npw);
-  int* d_l2i = c.w_l2i.as<int>(m);
   int* d_u2i = c.m_u2i.as<int>(m);
-  double* d_beta = c.w_beta.as<double>(m);
-  double* d_eu = c.w_eu.as<double>(m);
+  int* d_l2i0 = c.w_l2i.as<int>(m);   // the first residual's list
+  // each iteration's active list, beta and e go to the device in one copy,
+  // laid out for that iteration's nc: [beta (nc) | e (nc) | list (nc ints)]
+  double* d_pk = c.w_cgpk.as<double>(static_cast<size_t>(5) * m / 2 + 1);
+  double* pk = c.h_cgpk.as<double>(static_cast<size_t>(5) * m / 2 + 1);
   double* d_rho = c.w_rho.as<double>(m);
   double* d_rho64 = c.m_rho64.as<double>(m);
   double* d_eu64 = c.m_eu64.as<double>(m);
@@ -530,8 +532,6 @@ int linsolve_mixed(int m, z64* b, z64* x, const std::vector<double>& e_col, cons
   int* l2i = c.hm_l2i.as<int>(m);
   int* uh = c.hm_u.as<int>(m);
   double* tmp = c.hm_rho.as<double>(m);
-  double* beta = c.hm_beta.as<double>(m);
-  double* eu = c.hm_eu.as<double>(m);
   double* eu64 = c.hm_eu64.as<double>(m);
   list.reserve(m);
   // FP32 iterations after which a refinement is forced
@@ -586,7 +586,7 @@ int linsolve_mixed(int m, z64* b, z64* x, const std::vector<double>& e_col, cons
   to_f32(m, 2 * npw, reinterpret_cast<const double*>(x), 2L * npw, reinterpret_cast<float*>(bc), 2 * l32, s);
   stern_apply32(bc, tc, d_e_col, m, c.alpha_pv);
   WH_TIMED("refine_residual", residual32(m, npw, tc, l32, b, ld, g, s));
-  precond(list, d_l2i);
+  precond(list, d_l2i0);
   for (int ib = 0; ib < m; ++ib) rho_ref[ib] = rho[ib];
   {
     // an FP32 residual already below tr2 is confirmed in FP64 before the band is let go
@@ -638,14 +638,17 @@ int linsolve_mixed(int m, z64* b, z64* x, const std::vector<double>& e_col, cons
       act.clear();
       break;
     }
+    int* pl = reinterpret_cast<int*>(pk + 2 * nc);
     for (int lb = 0; lb < nc; ++lb) {
       const int ib = l2i[lb];
-      beta[lb] = (iter != 1) ? rho[ib] / rhoold[ib] : 0.0;
-      eu[lb] = e_col[ib];
+      pk[lb] = (iter != 1) ? rho[ib] / rhoold[ib] : 0.0;   // beta
+      pk[nc + lb] = e_col[ib];
+      pl[lb] = ib;
     }
-    WH_HIP(hipMemcpyAsync(d_l2i, l2i, sizeof(int) * nc, hipMemcpyHostToDevice, s));
-    WH_HIP(hipMemcpyAsync(d_beta, beta, sizeof(double) * nc, hipMemcpyHostToDevice, s));
-    WH_HIP(hipMemcpyAsync(d_eu, eu, sizeof(double) * nc, hipMemcpyHostToDevice, s));
+    WH_HIP(hipMemcpyAsync(d_pk, pk, (2 * sizeof(double) + sizeof(int)) * nc, hipMemcpyHostToDevice, s));
+    const double* d_beta = d_pk;
+    const double* d_eu = d_pk + nc;
+    const int* d_l2i = reinterpret_cast<const int*>(d_pk + 2 * nc);
     WH_TIMED("cg_dir32", direction(nc, npw, d_l2i, d_beta, g, c.d_g2kin, d_eprec_col, h, hold, bc, l32, s));
     stern_apply32(bc, tc, d_eu, nc, c.alpha_pv);
     WH_TIMED("cg_update32", cg_update(nc, npw, c.gstart, d_l2i, h, g, tc, dx, l32, nullptr, s, &rho_out));
