@@ -132,7 +132,7 @@ MODULE west_hip
        COMPLEX(C_DOUBLE_COMPLEX), INTENT(INOUT) :: b(*), x(*)
        REAL(C_DOUBLE), INTENT(IN) :: e(*), eprec(*)
        REAL(C_DOUBLE), VALUE :: tr2
-       INTEGER(C_INT), INTENT(OUT) :: ierr
+       INTEGER(C_INT), INTENT(OUT), OPTIONAL :: ierr
      END FUNCTION
      !
      INTEGER(C_INT) FUNCTION wh_dfpt(m, dvg, dng, tr2, iq, ierr_per_pert) BIND(C, NAME="wh_dfpt")
@@ -141,7 +141,7 @@ MODULE west_hip
        COMPLEX(C_DOUBLE_COMPLEX), INTENT(IN) :: dvg(*)
        COMPLEX(C_DOUBLE_COMPLEX), INTENT(OUT) :: dng(*)
        REAL(C_DOUBLE), VALUE :: tr2
-       INTEGER(C_INT), INTENT(OUT) :: ierr_per_pert(*)
+       INTEGER(C_INT), INTENT(OUT), OPTIONAL :: ierr_per_pert(*)
      END FUNCTION
      !
      INTEGER(C_INT) FUNCTION wh_apply_operator(m, dvg, dng, tr2, iq) BIND(C, NAME="wh_apply_operator")
@@ -152,21 +152,30 @@ MODULE west_hip
        REAL(C_DOUBLE), VALUE :: tr2
      END FUNCTION
      !
-     INTEGER(C_INT) FUNCTION wh_davidson_diago(params, ev, conv, dvg_out, log, log_capacity, log_len) &
-          & BIND(C, NAME="wh_davidson_diago")
+     ! ev_log(n_pdep_eigen, log_capacity), conv_log(n_pdep_eigen, log_capacity):
+     ! the ev and conv of each step, as output_ev_and_time records them
+     INTEGER(C_INT) FUNCTION wh_davidson_diago(params, ev, conv, dvg_out, log, log_capacity, log_len, &
+          & ev_log, conv_log) BIND(C, NAME="wh_davidson_diago")
        IMPORT :: C_INT, C_DOUBLE, C_DOUBLE_COMPLEX, wh_davidson_params, wh_davidson_step
        TYPE(wh_davidson_params), INTENT(IN) :: params
        REAL(C_DOUBLE), INTENT(OUT) :: ev(*)
-       INTEGER(C_INT), INTENT(OUT) :: conv(*)
-       COMPLEX(C_DOUBLE_COMPLEX), INTENT(OUT) :: dvg_out(*)
-       TYPE(wh_davidson_step), INTENT(OUT) :: log(*)
+       INTEGER(C_INT), INTENT(OUT), OPTIONAL :: conv(*)
+       COMPLEX(C_DOUBLE_COMPLEX), INTENT(OUT), OPTIONAL :: dvg_out(*)
+       TYPE(wh_davidson_step), INTENT(OUT), OPTIONAL :: log(*)
        INTEGER(C_INT), VALUE :: log_capacity
-       INTEGER(C_INT), INTENT(OUT) :: log_len
+       INTEGER(C_INT), INTENT(OUT), OPTIONAL :: log_len
+       REAL(C_DOUBLE), INTENT(OUT), OPTIONAL :: ev_log(*)
+       INTEGER(C_INT), INTENT(OUT), OPTIONAL :: conv_log(*)
      END FUNCTION
      !
      INTEGER(C_INT) FUNCTION wh_get_counters(stern_ncalls, cg_iters) BIND(C, NAME="wh_get_counters")
        IMPORT :: C_INT, C_LONG
-       INTEGER(C_LONG), INTENT(OUT) :: stern_ncalls, cg_iters
+       INTEGER(C_LONG), INTENT(OUT), OPTIONAL :: stern_ncalls, cg_iters
+     END FUNCTION
+     !
+     INTEGER(C_INT) FUNCTION wh_get_operator_columns(band_applications) BIND(C, NAME="wh_get_operator_columns")
+       IMPORT :: C_INT, C_LONG
+       INTEGER(C_LONG), INTENT(OUT) :: band_applications
      END FUNCTION
      !
      INTEGER(C_INT) FUNCTION wh_get_precision_counters(fp32_applies, refinements, fp64_fallbacks) &

@@ -141,7 +141,11 @@ int wh_apply_operator(int m, const double* dvg, double* dng, double tr2, int iq)
  * the start vectors are the library's seeded random vectors (do_randomize,
  * Wstat/davidson_diago.f90:1121-1182).  Outputs: ev(n_pdep_eigen),
  * conv(n_pdep_eigen) (1 = converged), dvg_out(npwqx, n_pdep_eigen) complex
- * (may be NULL), and one log record per Davidson step (may be NULL).
+ * (may be NULL), and one log record per Davidson step (may be NULL; at most
+ * log_capacity are kept, *log_len receives how many), with the step's
+ * eigenvalues and convergence flags in ev_log(n_pdep_eigen, log_capacity)
+ * and conv_log(n_pdep_eigen, log_capacity) (each may be NULL): the fields of
+ * exec.davitr written by output_ev_and_time (davidson_diago.f90:1310-1329).
  * Replaces davidson_diago_gamma (Wstat/davidson_diago.f90:31). */
 typedef struct {
   int n_pdep_eigen;
@@ -164,7 +168,7 @@ typedef struct {            /* exec.davitr[] of output_ev_and_time (davidson_dia
 } wh_davidson_step;
 
 int wh_davidson_diago(const wh_davidson_params* params, double* ev, int* conv, double* dvg_out,
-                      wh_davidson_step* log, int log_capacity, int* log_len);
+                      wh_davidson_step* log, int log_capacity, int* log_len, double* ev_log, int* conv_log);
 
 /* Restart files of wh_davidson_diago, in the reference's format
  * (Wstat/davidson_restart.f90:33-222, 360-410, 413-462, 515-760; the inputs
@@ -184,9 +188,16 @@ int wh_set_davidson_restart(const char* wstat_restart_dir, int n_steps_write_res
 
 /* ---- counters -------------------------------------------------------- */
 
-/* Operator applications so far (the 'stern' clock count of the reference,
- * test-suite/test001/ref/wstat.json:820-826) and CG iterations. */
+/* Operator calls so far as the reference's 'stern' clock counts them (one
+ * call of apply_sternheimerop_to_m_wfcs per perturbation's Sternheimer solve
+ * per CG step, DFPT_kernel/linsolve_sternheimer_m_wfcts.f90:127,251;
+ * test-suite/test001/ref/wstat.json:820-826: 3611 calls for 171
+ * perturbations), whatever the batching here; and CG iterations. */
 int wh_get_counters(long* stern_ncalls, long* cg_iters);
+
+/* Operator applications so far counted in band columns (an application to m
+ * bands counts m), all precisions: the work measure of the throughput. */
+int wh_get_operator_columns(long* band_applications);
 
 /* Of the operator applications: those made in FP32; the band residuals
  * recomputed in FP64 by the mixed-precision solve; the bands it handed to the

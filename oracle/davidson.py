@@ -134,7 +134,7 @@ def davidson_diago_gamma(gs, sqvc, n_pdep_eigen, n_pdep_times=4, n_pdep_maxiter=
     conv = np.zeros(nvec, dtype=bool)
     nbase = nvec
     notcnv = nvec
-    stats = {"nstern": 0}
+    stats = {"nstern": 0, "ncalls": 0}
     t0 = time.perf_counter()
 
     def record(dav_iter, tr2, dfpt_dim, diago_dim, ns0, tit):
@@ -142,7 +142,7 @@ def davidson_diago_gamma(gs, sqvc, n_pdep_eigen, n_pdep_times=4, n_pdep_maxiter=
             log.append({"dav_iter": dav_iter, "ev": ev.copy().tolist(), "conv": conv.copy().tolist(),
                         "notcnv": int(notcnv), "time_elap:sec": time.perf_counter() - t0,
                         "time_iter:sec": time.perf_counter() - tit,
-                        "sternop_ncalls": stats["nstern"] - ns0, "dfpt_tr2": tr2,
+                        "sternop_ncalls": stats["ncalls"] - ns0, "dfpt_tr2": tr2,
                         "dfpt_dim": int(dfpt_dim), "diago_dim": int(diago_dim)})
 
     # from scratch (167-196)
@@ -151,7 +151,7 @@ def davidson_diago_gamma(gs, sqvc, n_pdep_eigen, n_pdep_times=4, n_pdep_maxiter=
     dng[:nvec] = apply_operator(gs, sqvc, dvg[:nvec], -1.0, n_dfpt_maxiter, stats)
     # extra step (205-249)
     tit = time.perf_counter()
-    ns0 = stats["nstern"]
+    ns0 = stats["ncalls"]
     dvg[:nvec] = dng[:nvec]
     do_mgs(basis, dvg, 1, nvec)
     tr2 = min(1.0e-4, 1.0e8 * tr2_dfpt)
@@ -163,7 +163,7 @@ def davidson_diago_gamma(gs, sqvc, n_pdep_eigen, n_pdep_times=4, n_pdep_maxiter=
     dav_iter = 0
     for kter in range(1, n_pdep_maxiter + 1):                    # (259-424)
         tit = time.perf_counter()
-        ns0 = stats["nstern"]
+        ns0 = stats["ncalls"]
         dav_iter += 1
         dfpt_dim = notcnv
         diago_dim = nbase + notcnv

@@ -19,7 +19,9 @@ def linsolve_sternheimer_m_wfcts(gs, nbndval, b, x, e, eprec, tr2, n_dfpt_maxite
     """Solve (H - e_i + alpha_pv P_v) x_i = b_i.  ``b`` is destroyed (it holds
     ``hold``), ``x`` is refined in place.  Returns ierr = number of
     unconverged columns.  ``stats['nstern']`` counts operator applications
-    (columns), ``stats['iters']`` iterations."""
+    (columns), ``stats['ncalls']`` calls of ``apply_sternheimerop`` (the
+    reference's 'stern' clock, lines 127 and 251), ``stats['iters']``
+    iterations."""
     bas = gs.basis
     n = bas.npw
     gstart = bas.gstart
@@ -34,6 +36,7 @@ def linsolve_sternheimer_m_wfcts(gs, nbndval, b, x, e, eprec, tr2, n_dfpt_maxite
     rhoold = np.zeros(m)
     is_conv = np.zeros(m, dtype=bool)
     nstern = 0
+    ncalls = 1
     # Step 1 (127-138): g = A x - b
     g[:] = apply_sternheimerop(gs, x, e, alpha)
     nstern += m
@@ -63,6 +66,7 @@ def linsolve_sternheimer_m_wfcts(gs, nbndval, b, x, e, eprec, tr2, n_dfpt_maxite
         # t = A h on the compacted columns (251)
         t[:nc] = apply_sternheimerop(gs, b[:nc], eu, alpha)
         nstern += nc
+        ncalls += 1
         a_c = _gamma_dot(h[l2i], g[l2i], n, gstart)     # (253-276)
         c_c = _gamma_dot(h[l2i], t[:nc], n, gstart)
         step = (a_c / c_c)[:, None]
@@ -73,5 +77,6 @@ def linsolve_sternheimer_m_wfcts(gs, nbndval, b, x, e, eprec, tr2, n_dfpt_maxite
     ierr = int(l2i.size)
     if stats is not None:
         stats["nstern"] = stats.get("nstern", 0) + nstern
+        stats["ncalls"] = stats.get("ncalls", 0) + ncalls
         stats["iters"] = stats.get("iters", 0) + it_done
     return ierr

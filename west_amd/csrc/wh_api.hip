@@ -254,11 +254,26 @@ void stern_apply(const z64* x, z64* t, const double* d_e, int m, double alpha) {
   c.nstern += m;
 }
 
+void count_calls(int n, const int* cols, int group, const int* col_id) {
+  Context& c = ctx();
+  long calls = 0;
+  int last = -1;
+  for (int k = 0; k < n; ++k) {
+    int col = cols ? cols[k] : k;
+    if (col_id) col = col_id[col];
+    const int p = group > 0 ? col / group : 0;
+    if (p != last) ++calls;
+    last = p;
+  }
+  c.nstern_calls += calls;
+}
+
 // ---------------------------------------------------------------------------
 // Preconditioned CG  (DFPT_kernel/linsolve_sternheimer_m_wfcts.f90:107-332)
 // ---------------------------------------------------------------------------
 int linsolve_dev(int m, z64* b, z64* x, const std::vector<double>& e_col, const double* d_e_col,
-                 const double* d_eprec_col, double tr2, std::vector<int>* ierr_col, int maxiter) {
+                 const double* d_eprec_col, double tr2, std::vector<int>* ierr_col, int maxiter, int group,
+                 const int* col_id) {
   Context& c = ctx();
   hipStream_t s = c.stream;
   const int npw = c.npw;
@@ -282,6 +297,7 @@ int linsolve_dev(int m, z64* b, z64* x, const std::vector<double>& e_col, const 
   const RhoOut rho_out{c.d_g2kin, d_eprec_col, d_rhob};
   // step 1: g = A x - b
   stern_apply(x, g, d_e_col, m, c.alpha_pv);
+  count_calls(m, nullptr, group, col_id);
   axpby(static_cast<long>(vec), -1.0, b, 1.0, g, s);
   std::vector<char> is_conv(m, 0);
   // pinned: the lists and coefficients go to the device with asynchronous copies
@@ -337,6 +353,7 @@ int linsolve_dev(int m, z64* b, z64* x, const std::vector<double>& e_col, const 
       pl[lb] = ib;
     }
     WH_HIP(hipMemcpyAsync(d_pk, pk, (2 * sizeof(double) + sizeof(int)) * nc, hipMemcpyHostToDevice, s));
+    count_calls(nc, l2i, group, col_id);
     d_beta = d_pk;
     d_eu = d_pk + nc;
     d_l2i = reinterpret_cast<int*>(d_pk + 2 * nc);
@@ -495,7 +512,7 @@ void stern_apply32(const z32* x, z32* t, const double* d_e, int m, double alpha)
 }
 
 int linsolve_mixed(int m, z64* b, z64* x, const std::vector<double>& e_col, const double* d_e_col,
-                   const double* d_eprec_col, double tr2, std::vector<int>* ierr_col) {
+                   const double* d_eprec_col, double tr2, std::vector<int>* ierr_col, int group) {
   Context& c = ctx();
   prepare_mixed();
   hipStream_t s = c.stream;
@@ -551,6 +568,7 @@ int linsolve_mixed(int m, z64* b, z64* x, const std::vector<double>& e_col, cons
     WH_HIP(hipMemcpyAsync(d_eu64, eu64, sizeof(double) * nu, hipMemcpyHostToDevice, s));
     WH_TIMED("refine_fold", fold_increment(nu, npw, d_u2i, x, ld, dx, l32, xc, s));
     stern_apply(xc, tc64, d_eu64, nu, c.alpha_pv);
+    count_calls(nu, u.data(), group);
     WH_TIMED("refine_residual", true_residual(nu, npw, c.gstart, d_u2i, tc64, b, ld, c.d_g2kin, d_eprec_col, g, l32,
                                               d_rho64, s));
     WH_HIP(hipMemcpyAsync(tmp, d_rho64, sizeof(double) * nu, hipMemcpyDeviceToHost, s));
@@ -585,6 +603,7 @@ int linsolve_mixed(int m, z64* b, z64* x, const std::vector<double>& e_col, cons
   std::iota(list.begin(), list.end(), 0);
   to_f32(m, 2 * npw, reinterpret_cast<const double*>(x), 2L * npw, reinterpret_cast<float*>(bc), 2 * l32, s);
   stern_apply32(bc, tc, d_e_col, m, c.alpha_pv);
+  count_calls(m, nullptr, group);
   WH_TIMED("refine_residual", residual32(m, npw, tc, l32, b, ld, g, s));
   precond(list, d_l2i0);
   for (int ib = 0; ib < m; ++ib) rho_ref[ib] = rho[ib];
@@ -651,6 +670,7 @@ int linsolve_mixed(int m, z64* b, z64* x, const std::vector<double>& e_col, cons
     const int* d_l2i = reinterpret_cast<const int*>(d_pk + 2 * nc);
     WH_TIMED("cg_dir32", direction(nc, npw, d_l2i, d_beta, g, c.d_g2kin, d_eprec_col, h, hold, bc, l32, s));
     stern_apply32(bc, tc, d_eu, nc, c.alpha_pv);
+    count_calls(nc, l2i, group);
     WH_TIMED("cg_update32", cg_update(nc, npw, c.gstart, d_l2i, h, g, tc, dx, l32, nullptr, s, &rho_out));
     std::swap(h, hold);
     for (int lb = 0; lb < nc; ++lb) rhoold[l2i[lb]] = rho[l2i[lb]];
@@ -689,7 +709,7 @@ int linsolve_mixed(int m, z64* b, z64* x, const std::vector<double>& e_col, cons
     c.nfallback += nf;
     std::vector<int> ierr_f;
     // a full n_dfpt_maxiter budget: these are the hardest bands, restarted
-    nonconv += linsolve_dev(nf, fb, fx, e_f, fe, fep, tr2, &ierr_f, c.n_dfpt_maxiter);
+    nonconv += linsolve_dev(nf, fb, fx, e_f, fe, fep, tr2, &ierr_f, c.n_dfpt_maxiter, group, fl.data());
     WH_HIP(hipMemcpyAsync(d_u2i, fl.data(), sizeof(int) * nf, hipMemcpyHostToDevice, s));
     copy_rows(nf, npw, d_u2i, fx, ld, x, ld, true, s);
     WH_HIP(hipStreamSynchronize(s));
@@ -816,8 +836,9 @@ static int dfpt_chunk(int m, const z64* dvg, z64* dng, double tr2, const double*
     WH_HIP(hipMemcpyAsync(de, e_col.data(), sizeof(double) * M, hipMemcpyHostToDevice, s));
     WH_HIP(hipMemcpyAsync(dep, ep_col.data(), sizeof(double) * M, hipMemcpyHostToDevice, s));
     std::vector<int> ierr_col;
-    nonconv = c.precision != 0 ? linsolve_mixed(static_cast<int>(M), dvpsi, dpsi, e_col, de, dep, tr2, &ierr_col)
-                               : linsolve_dev(static_cast<int>(M), dvpsi, dpsi, e_col, de, dep, tr2, &ierr_col);
+    nonconv = c.precision != 0
+                  ? linsolve_mixed(static_cast<int>(M), dvpsi, dpsi, e_col, de, dep, tr2, &ierr_col, nv)
+                  : linsolve_dev(static_cast<int>(M), dvpsi, dpsi, e_col, de, dep, tr2, &ierr_col, 0, nv);
     if (ierr_pert)
       for (long i = 0; i < M; ++i) (*ierr_pert)[i / nv] += ierr_col[i];
   }
@@ -1119,6 +1140,7 @@ int wh_apply_sternheimerop_to_m_wfcs(int nbndval, const double* psi, double* hps
   } else {
     stern_apply(x, t, pe, m, a[0]);
   }
+  ++c.nstern_calls;
   const bool dout = on_device(hpsi);
   z64* o = dout ? reinterpret_cast<z64*>(hpsi) : c.w_out.as<z64>(cin);
   scatter_caller(m, c.npw, c.npwx, c.d_perm, t, c.npw, o, c.npwx, s);
@@ -1229,8 +1251,16 @@ int wh_get_precision_counters(long* fp32_applies, long* refinements, long* fp64_
 int wh_get_counters(long* stern_ncalls, long* cg_iters) {
   WH_API_BEGIN
   Context& c = ctx();
-  if (stern_ncalls) *stern_ncalls = c.nstern;
+  if (stern_ncalls) *stern_ncalls = c.nstern_calls;
   if (cg_iters) *cg_iters = c.cg_iters;
+  return 0;
+  WH_API_END
+}
+
+int wh_get_operator_columns(long* band_applications) {
+  WH_API_BEGIN
+  Context& c = ctx();
+  if (band_applications) *band_applications = c.nstern;
   return 0;
   WH_API_END
 }

@@ -146,7 +146,8 @@ struct Context {
   DevBuf m_fb, m_fx, m_fe, m_fep;   // bands finished by the FP64 CG (right sides, solutions, e, eprec)
   DevBuf dav[9];                    // Davidson basis, products and subspace arrays
   // counters
-  long nstern = 0;     // operator applications (all precisions)
+  long nstern = 0;     // operator applications, in band columns (all precisions)
+  long nstern_calls = 0;   // operator calls as the reference counts them: one per perturbation's linsolve per application
   long nstern32 = 0;   // of which FP32
   long nrefine = 0;    // FP64 refinement residuals (band count)
   long nfallback = 0;  // bands the mixed solve handed to the FP64 CG
@@ -179,11 +180,20 @@ void require_ready(bool need_states);
 void stern_apply(const z64* x, z64* t, const double* d_e, int m, double alpha);
 // maxiter <= 0: n_dfpt_maxiter
 int linsolve_dev(int m, z64* b, z64* x, const std::vector<double>& e_col, const double* d_e_col,
-                 const double* d_eprec_col, double tr2, std::vector<int>* ierr_col, int maxiter = 0);
+                 const double* d_eprec_col, double tr2, std::vector<int>* ierr_col, int maxiter = 0, int group = 0,
+                 const int* col_id = nullptr);
 int dfpt_dev(int m, const z64* dvg, z64* dng, double tr2, const double* out_weight, std::vector<int>* ierr_pert);
 void glbrak_q(int row0, int nq, const z64* f, int m, double* cmat);
 void stern_apply32(const z32* x, z32* t, const double* d_e, int m, double alpha);
 int linsolve_mixed(int m, z64* b, z64* x, const std::vector<double>& e_col, const double* d_e_col,
-                   const double* d_eprec_col, double tr2, std::vector<int>* ierr_col);   // cmat (nq x m) = <Q_row|f_col>
+                   const double* d_eprec_col, double tr2, std::vector<int>* ierr_col, int group = 0);
+// The reference's 'stern' clock counts calls of apply_sternheimerop, one per
+// perturbation's linsolve per CG iteration (DFPT_kernel/dfpt_module.f90:356,
+// linsolve_sternheimer_m_wfcts.f90:127,251).  Columns here are
+// perturbation-major, `group` bands per perturbation (group <= 0: one
+// perturbation); this adds the perturbations among the n columns listed in
+// ascending order in cols (null: columns 0..n-1), each column mapped through
+// col_id when given.
+void count_calls(int n, const int* cols, int group, const int* col_id = nullptr);   // cmat (nq x m) = <Q_row|f_col>
 
 }  // namespace wh

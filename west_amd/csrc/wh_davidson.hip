@@ -328,7 +328,8 @@ double now_s() {
 using namespace wh;
 
 extern "C" int wh_davidson_diago(const wh_davidson_params* prm, double* ev_out, int* conv_out, double* dvg_out,
-                                 wh_davidson_step* log, int log_capacity, int* log_len) {
+                                 wh_davidson_step* log, int log_capacity, int* log_len, double* ev_log,
+                                 int* conv_log) {
   try {
     Context& c = ctx();
     require_ready(true);
@@ -357,7 +358,7 @@ extern "C" int wh_davidson_diago(const wh_davidson_params* prm, double* ev_out, 
     int nbase = nvec, notcnv = nvec;
     const double t0 = now_s();
     double tit = t0;
-    long ns0 = c.nstern;
+    long ns0 = c.nstern_calls;
     int nlog = 0;
 
     // chi~ applied to rows [r0, r0+n): dng = sqvc chi0 (sqvc dvg)
@@ -411,10 +412,16 @@ extern "C" int wh_davidson_diago(const wh_davidson_params* prm, double* ev_out, 
         r.notcnv = notcnv;
         r.dfpt_dim = dfpt_dim;
         r.diago_dim = diago_dim;
-        r.sternop_ncalls = c.nstern - ns0;
+        r.sternop_ncalls = c.nstern_calls - ns0;
         r.dfpt_tr2 = tr2;
         r.time_elap = now_s() - t0;
         r.time_iter = now_s() - tit;
+      }
+      // ev and conv of the step (output_ev_and_time, davidson_diago.f90:1318-1319)
+      if (nlog < log_capacity) {
+        if (ev_log) std::memcpy(ev_log + static_cast<size_t>(nlog) * nvec, ev.data(), sizeof(double) * nvec);
+        if (conv_log)
+          for (int n = 0; n < nvec; ++n) conv_log[static_cast<size_t>(nlog) * nvec + n] = conv[n] ? 1 : 0;
       }
       ++nlog;
       // the per-step report of the reference (output_ev_and_time, davidson_diago.f90:355)
@@ -504,7 +511,7 @@ extern "C" int wh_davidson_diago(const wh_davidson_params* prm, double* ev_out, 
     if (dav_iter == -1) {
       // ---- extra step (205-249)
       tit = now_s();
-      ns0 = c.nstern;
+      ns0 = c.nstern_calls;
       WH_HIP(hipMemcpyAsync(dvg, dng, sizeof(z64) * nvec * npw, hipMemcpyDeviceToDevice, s));
       do_mgs(dvg, 0, nvec);
       apply_op(0, nvec, tr2);
@@ -518,7 +525,7 @@ extern "C" int wh_davidson_diago(const wh_davidson_params* prm, double* ev_out, 
     bool converged = false;
     for (int kter = 1; kter <= prm->n_pdep_maxiter; ++kter) {   // (259-424)
       tit = now_s();
-      ns0 = c.nstern;
+      ns0 = c.nstern_calls;
       ++dav_iter;
       const int dfpt_dim = notcnv;
       const int diago_dim = nbase + notcnv;

@@ -19,10 +19,13 @@
 //
 // Operand layout of the 16x16x4 f64 instruction (checked on the device):
 // lane l holds A[l%16][l/16] and B[l/16][l%16]; result register r of lane l
-// holds D[l/16 + 4r][l%16].
+// holds D[l/16 + 4r][l%16].  The kernel passes the B block as the
+// instruction's A operand and the A block as its B operand, so it forms
+// D = (A B)^T: the 16 lanes of a quarter wave then hold 16 consecutive rows
+// of C, i.e. consecutive addresses of the column-major result, and the
+// epilogue's loads and stores of C are whole 128-byte runs (the ket products
+// read and write a C of 2 npw rows and up to thousands of columns).
 #include "wh_gemm.h"
-
-#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cstdlib>
@@ -30,113 +33,62 @@
 #include <mutex>
 #include <tuple>
 #include <cstring>
+#include <functional>
 
 namespace wh {
-
-// Backend of the FP64 products. Plain products (dgemm) use rocBLAS, measured
-// about twice as fast as the kernel below for the ket shape (M = 2 npw,
-// K = nq) on MI355X. The bra products (small M = nq, long K = 2 npw, fused
-// G = 0 correction and row scale) use whichever of the split-K kernel of this
-// file (three K-split depths) and rocBLAS + k_g0_scale was fastest when first
-// timed for that (M, K) and the power-of-two bucket of N, for N >= 512 columns
-// (at 8192 columns the kernel won at nq = 384, rocBLAS at nq = 1296); smaller
-// calls use the kernel.
-// WH_GEMM=own or WH_GEMM=rocblas puts every product on one backend.
-enum class Backend { Default, Own, Rocblas };
-static Backend backend() {
-  static const Backend v = [] {
-    const char* e = std::getenv("WH_GEMM");
-    if (e && std::strcmp(e, "rocblas") == 0) return Backend::Rocblas;
-    if (e && std::strcmp(e, "own") == 0) return Backend::Own;
-    return Backend::Default;
-  }();
-  return v;
-}
-
-// one rocBLAS handle per host thread (handles are not shared between threads)
-struct BlasHandle {
-  rocblas_handle h = nullptr;
-  ~BlasHandle() {
-    if (h) rocblas_destroy_handle(h);
-  }
-};
-static rocblas_handle blas(hipStream_t s) {
-  static thread_local BlasHandle bh;
-  if (!bh.h) WH_REQUIRE(rocblas_create_handle(&bh.h) == rocblas_status_success, "rocblas_create_handle failed");
-  WH_REQUIRE(rocblas_set_stream(bh.h, s) == rocblas_status_success, "rocblas_set_stream failed");
-  return bh.h;
-}
 
 namespace {
 
 constexpr int BM = 128;
-constexpr int BN = 128;
 constexpr int BK = 16;
 constexpr int NT = 256;
 constexpr int PAD = 2;
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-template <bool TA>
-__device__ __forceinline__ void load_a(double (&ra)[8], const double* __restrict__ A, long lda, int M, int m0, int k0,
-                                       int kend) {
+// Tile loads, BK * XT / 256 elements per thread, mapped so that each load
+// instruction of a wavefront reads contiguous memory:
+//  * x-contiguous operand (element (k, x) at P[k*ld + x]: A of the plain
+//    product, XT = 128): x = tid % 128, k = tid / 128 + 2q  (512 bytes per
+//    instruction);
+//  * k-contiguous operand (element (k, x) at P[x*ld + k]: A of the bra
+//    product, B of both): k = tid % 16, x = tid / 16 + 16q  (four 128-byte runs).
+// CHK: zero-fill outside [0, X) x [k0, kend) (edge tiles only).
+template <bool KCONT, bool CHK, int XT>
+__device__ __forceinline__ void load_tile(double (&r)[BK * XT / NT], const double* __restrict__ P, long ld, int X,
+                                          int x0, int k0, int kend) {
   const int tid = threadIdx.x;
-  if (TA) {   // element (k, m) at A[m*lda + k]: 128 m x 16 k, 8 consecutive k per thread
-    const int m = tid >> 1;
-    const int kq = (tid & 1) * 8;
-    const int gm = m0 + m;
+  if (KCONT) {
+    const int gk = k0 + (tid & 15);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int gk = k0 + kq + q;
-      ra[q] = (gm < M && gk < kend) ? A[static_cast<long>(gm) * lda + gk] : 0.0;
+    for (int q = 0; q < BK * XT / NT; ++q) {
+      const int gx = x0 + (tid >> 4) + 16 * q;
+      if (CHK) r[q] = (gx < X && gk < kend) ? P[static_cast<long>(gx) * ld + gk] : 0.0;
+      else r[q] = P[static_cast<long>(gx) * ld + gk];
     }
-  } else {    // element (m, k) at A[k*lda + m]: 16 k x 128 m, 8 consecutive m per thread
-    const int k = tid >> 4;
-    const int mq = (tid & 15) * 8;
-    const int gk = k0 + k;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int gm = m0 + mq + q;
-      ra[q] = (gm < M && gk < kend) ? A[static_cast<long>(gk) * lda + gm] : 0.0;
-    }
-  }
-}
-
-template <bool TA>
-__device__ __forceinline__ void store_a(double (*As)[BM + PAD], const double (&ra)[8]) {
-  const int tid = threadIdx.x;
-  if (TA) {
-    const int m = tid >> 1;
-    const int kq = (tid & 1) * 8;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) As[kq + q][m] = ra[q];
   } else {
-    const int k = tid >> 4;
-    const int mq = (tid & 15) * 8;
+    static_assert(KCONT || XT == 128, "x-contiguous tiles are 128 wide");
+    const int gx = x0 + (tid & 127);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) As[k][mq + q] = ra[q];
+    for (int q = 0; q < BK * XT / NT; ++q) {
+      const int gk = k0 + (tid >> 7) + 2 * q;
+      if (CHK) r[q] = (gx < X && gk < kend) ? P[static_cast<long>(gk) * ld + gx] : 0.0;
+      else r[q] = P[static_cast<long>(gk) * ld + gx];
+    }
   }
 }
 
-__device__ __forceinline__ void load_b(double (&rb)[8], const double* __restrict__ B, long ldb, int N, int n0, int k0,
-                                       int kend) {
+// the LDS tile is [k][x] for both operands
+template <bool KCONT, int XT>
+__device__ __forceinline__ void store_tile(double (*S)[XT + PAD], const double (&r)[BK * XT / NT]) {
   const int tid = threadIdx.x;
-  const int n = tid >> 1;
-  const int kq = (tid & 1) * 8;
-  const int gn = n0 + n;
+  if (KCONT) {
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int gk = k0 + kq + q;
-    rb[q] = (gn < N && gk < kend) ? B[static_cast<long>(gn) * ldb + gk] : 0.0;
+    for (int q = 0; q < BK * XT / NT; ++q) S[tid & 15][(tid >> 4) + 16 * q] = r[q];
+  } else {
+#pragma unroll
+    for (int q = 0; q < BK * XT / NT; ++q) S[(tid >> 7) + 2 * q][tid & 127] = r[q];
   }
-}
-
-__device__ __forceinline__ void store_b(double (*Bs)[BN + PAD], const double (&rb)[8]) {
-  const int tid = threadIdx.x;
-  const int n = tid >> 1;
-  const int kq = (tid & 1) * 8;
-#pragma unroll
-  for (int q = 0; q < 8; ++q) Bs[kq + q][n] = rb[q];
 }
 
 // Tiles (bx, by, kz): gx x gy output tiles times gz K chunks, k range
@@ -148,13 +100,17 @@ __device__ __forceinline__ void store_b(double (*Bs)[BN + PAD], const double (&r
 constexpr int kXcds = 8;
 __host__ __device__ inline int xcd_blocks(int tiles) { return (tiles + kXcds - 1) / kXcds * kXcds; }
 
-template <bool TA>
+// WN: 16-column MFMA blocks per wavefront along N, i.e. tiles of
+// BM x (32 WN) (WN = 4: 128 x 128; narrower tiles give more workgroups to
+// products with few columns).
+template <bool TA, int WN>
 __global__ __launch_bounds__(NT, 2) void k_dgemm(int M, int N, int K, int kchunk, double alpha,
                                               const double* __restrict__ A, long lda, const double* __restrict__ B,
                                               long ldb, double beta, double* __restrict__ C, long ldc, long pstride,
                                               int gx, int gy, int gz) {
+  constexpr int TN = 32 * WN;
   __shared__ double As[2][BK][BM + PAD];
-  __shared__ double Bs[2][BK][BN + PAD];
+  __shared__ double Bs[2][BK][TN + PAD];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -164,70 +120,121 @@ __global__ __launch_bounds__(NT, 2) void k_dgemm(int M, int N, int K, int kchunk
   const int W = static_cast<int>(blockIdx.x % kXcds) * (xcd_blocks(tiles) / kXcds) + static_cast<int>(blockIdx.x / kXcds);
   if (W >= tiles) return;   // uniform per workgroup
   const int m0 = (W % gx) * BM;
-  const int n0 = ((W / gx) % gy) * BN;
+  const int n0 = ((W / gx) % gy) * TN;
   const int kz = W / (gx * gy);
   const int kbeg = kz * kchunk;
   const int kend = min(K, kbeg + kchunk);
-  v4d acc[4][4];
+  // interior tiles load without bounds checks (all but a short last K step)
+  const bool interior = m0 + BM <= M && n0 + TN <= N;
+  const bool partial = gz > 1;
+  // C = alpha AB + beta C on an interior tile: the C loads of a row block are
+  // issued while the previous block is combined and stored
+  const bool rmw = !partial && beta != 0.0 && interior;
+  double* Cz = C + (partial ? static_cast<long>(kz) * pstride : 0L);
+  const int row0 = m0 + wm * 64 + (lane & 15);
+  const int col0 = n0 + wn * (16 * WN) + (lane >> 4);
+  double cv[4 * WN];
+  auto load_c = [&](int i) {
+#pragma unroll
+    for (int j = 0; j < WN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) cv[4 * j + r] = Cz[static_cast<long>(col0 + j * 16 + 4 * r) * ldc + row0 + i * 16];
+  };
+  v4d acc[4][WN];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < WN; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
   const int ntile = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
-  double ra[8], rb[8];
+  double ra[BK * BM / NT], rb[BK * TN / NT];
+  auto load = [&](int k0) {
+    if (interior && k0 + BK <= kend) {
+      load_tile<TA, false, BM>(ra, A, lda, M, m0, k0, kend);
+      load_tile<true, false, TN>(rb, B, ldb, N, n0, k0, kend);
+    } else {
+      load_tile<TA, true, BM>(ra, A, lda, M, m0, k0, kend);
+      load_tile<true, true, TN>(rb, B, ldb, N, n0, k0, kend);
+    }
+  };
   if (ntile > 0) {
-    load_a<TA>(ra, A, lda, M, m0, kbeg, kend);
-    load_b(rb, B, ldb, N, n0, kbeg, kend);
-    store_a<TA>(As[0], ra);
-    store_b(Bs[0], rb);
+    load(kbeg);
+    store_tile<TA, BM>(As[0], ra);
+    store_tile<true, TN>(Bs[0], rb);
   }
   __syncthreads();
   for (int t = 0; t < ntile; ++t) {
     const int cur = t & 1;
-    if (t + 1 < ntile) {
-      load_a<TA>(ra, A, lda, M, m0, kbeg + (t + 1) * BK, kend);
-      load_b(rb, B, ldb, N, n0, kbeg + (t + 1) * BK, kend);
-    }
+    if (t + 1 < ntile) load(kbeg + (t + 1) * BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       const int kr = kk + (lane >> 4);
-      double a[4], b[4];
+      double a[4], b[WN];
 #pragma unroll
       for (int i = 0; i < 4; ++i) a[i] = As[cur][kr][wm * 64 + i * 16 + (lane & 15)];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) b[j] = Bs[cur][kr][wn * 64 + j * 16 + (lane & 15)];
+      for (int j = 0; j < WN; ++j) b[j] = Bs[cur][kr][wn * (16 * WN) + j * 16 + (lane & 15)];
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < WN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[j], a[i], acc[i][j], 0, 0, 0);
     }
     if (t + 1 < ntile) {
-      store_a<TA>(As[cur ^ 1], ra);
-      store_b(Bs[cur ^ 1], rb);
+      store_tile<TA, BM>(As[cur ^ 1], ra);
+      store_tile<true, TN>(Bs[cur ^ 1], rb);
     }
     __syncthreads();
   }
-  const bool partial = gz > 1;
-  double* Cz = C + (partial ? static_cast<long>(kz) * pstride : 0L);
+  // block (i, j) holds the transposed product: register r of lane l is
+  // C(row i*16 + l%16, column j*16 + l/16 + 4r), so lanes 0..15 (and each
+  // further quarter wave) touch 16 consecutive rows of one column of C
+  if (rmw) {
+    // row block i is combined in its accumulators, the next block's C loads
+    // are issued, then block i is stored
+    load_c(0);
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 4; ++i) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int col = n0 + wn * 64 + j * 16 + (lane & 15);
+      for (int j = 0; j < WN; ++j)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm * 64 + i * 16 + (lane >> 4) + 4 * r;
-        if (row < M && col < N) {
-          double* c = Cz + static_cast<long>(col) * ldc + row;
-          if (partial) {
-            *c = acc[i][j][r];
-          } else {
-            const double v = alpha * acc[i][j][r];
-            *c = (beta == 0.0) ? v : v + beta * (*c);
-          }
-        }
-      }
+        for (int r = 0; r < 4; ++r) acc[i][j][r] = alpha * acc[i][j][r] + beta * cv[4 * j + r];
+      if (i + 1 < 4) load_c(i + 1);
+#pragma unroll
+      for (int j = 0; j < WN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Cz[static_cast<long>(col0 + j * 16 + 4 * r) * ldc + row0 + i * 16] = acc[i][j][r];
     }
+  } else if (partial || beta == 0.0) {
+    const double f = partial ? 1.0 : alpha;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < WN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = row0 + i * 16, col = col0 + j * 16 + 4 * r;
+          if (interior || (row < M && col < N)) Cz[static_cast<long>(col) * ldc + row] = f * acc[i][j][r];
+        }
+  } else {
+    // edge tile of C = alpha AB + beta C
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int row = row0 + i * 16;
+#pragma unroll
+      for (int j = 0; j < WN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int col = col0 + j * 16 + 4 * r;
+          cv[4 * j + r] = (row < M && col < N) ? Cz[static_cast<long>(col) * ldc + row] : 0.0;
+        }
+#pragma unroll
+      for (int j = 0; j < WN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int col = col0 + j * 16 + 4 * r;
+          if (row < M && col < N) Cz[static_cast<long>(col) * ldc + row] = alpha * acc[i][j][r] + beta * cv[4 * j + r];
+        }
+    }
+  }
 }
 
 // Bra product with few rows (M <= MB, e.g. the bands of a small molecule with
@@ -323,19 +330,25 @@ __global__ void k_g0_scale(int M, int N, double g0, const double* A, long lda, c
   C[static_cast<long>(j) * ldc + i] = v;
 }
 
+// Grows on demand; before a larger buffer replaces the old one, the streams
+// that used it (this host thread's calls only) are drained, so no other
+// system's stream is waited for.
 struct Scratch {
   double* p = nullptr;
   size_t n = 0;
-  double* get(size_t count) {
+  std::vector<hipStream_t> users;
+  double* get(size_t count, hipStream_t s) {
     if (count > n) {
       if (p) {
-        WH_HIP(hipDeviceSynchronize());
+        for (hipStream_t u : users) WH_HIP(hipStreamSynchronize(u));
         WH_HIP(hipFree(p));
       }
+      users.clear();
       p = nullptr;
       WH_HIP(hipMalloc(&p, sizeof(double) * count));
       n = count;
     }
+    if (std::find(users.begin(), users.end(), s) == users.end()) users.push_back(s);
     return p;
   }
 };
@@ -348,10 +361,68 @@ Scratch& partials() {
 
 }  // namespace
 
+template <bool TA, int WN>
+static void launch_tiles(int M, int N, int K, int kchunk, int gz, double alpha, const double* A, long lda,
+                         const double* B, long ldb, double beta, double* C, long ldc, long pstride, hipStream_t s) {
+  const int gx = ceil_div(M, BM), gy = ceil_div(N, 32 * WN);
+  k_dgemm<TA, WN><<<xcd_blocks(gx * gy * gz), NT, 0, s>>>(M, N, K, kchunk, alpha, A, lda, B, ldb, beta, C, ldc,
+                                                          pstride, gx, gy, gz);
+  WH_HIP(hipGetLastError());
+}
+
+// tile width variants: WN = 4, 2, 1 (128-, 64- and 32-column tiles)
+template <bool TA>
+static void launch_wn(int v, int M, int N, int K, int kchunk, int gz, double alpha, const double* A, long lda,
+                      const double* B, long ldb, double beta, double* C, long ldc, long pstride, hipStream_t s) {
+  if (v == 0) launch_tiles<TA, 4>(M, N, K, kchunk, gz, alpha, A, lda, B, ldb, beta, C, ldc, pstride, s);
+  else if (v == 1) launch_tiles<TA, 2>(M, N, K, kchunk, gz, alpha, A, lda, B, ldb, beta, C, ldc, pstride, s);
+  else launch_tiles<TA, 1>(M, N, K, kchunk, gz, alpha, A, lda, B, ldb, beta, C, ldc, pstride, s);
+}
+constexpr int kWnVariants = 3;
+inline int wn_of(int v) { return 4 >> v; }
+
+// The variant of a product shape is chosen once, by timing every variant on
+// the first call's own arguments (two rounds; the first warms the code up);
+// results never depend on the choice beyond rounding of a fixed order.
+using VariantKey = std::tuple<int, int, int, int, int>;
+static std::map<VariantKey, int>& chosen_variants() {
+  static std::map<VariantKey, int> m;
+  return m;
+}
+static std::mutex g_variant_mutex;
+
+static int time_variants(int nv, const std::function<void(int)>& run, hipStream_t s, float* t) {
+  hipEvent_t a, b;
+  WH_HIP(hipEventCreate(&a));
+  WH_HIP(hipEventCreate(&b));
+  for (int rep = 0; rep < 2; ++rep)
+    for (int v = 0; v < nv; ++v) {
+      WH_HIP(hipEventRecord(a, s));
+      run(v);
+      WH_HIP(hipEventRecord(b, s));
+      WH_HIP(hipEventSynchronize(b));
+      float ms = 0.f;
+      WH_HIP(hipEventElapsedTime(&ms, a, b));
+      if (rep == 1) t[v] = ms;
+    }
+  WH_HIP(hipEventDestroy(a));
+  WH_HIP(hipEventDestroy(b));
+  int best = 0;
+  for (int v = 1; v < nv; ++v)
+    if (t[v] < t[best]) best = v;
+  return best;
+}
+
+static int log2_bucket(int n) {
+  int lg = 0;
+  while ((2 << lg) <= n) ++lg;
+  return lg;
+}
+
 void dgemm(bool transa, int M, int N, int K, double alpha, const double* A, long lda, const double* B, long ldb,
            double beta, double* C, long ldc, hipStream_t s) {
   if (M <= 0 || N <= 0) return;
-  if (!transa && K > 0 && K <= 32 && N <= 256 && backend() != Backend::Rocblas) {
+  if (!transa && K > 0 && K <= 32 && N <= 256) {
     // inner dimension of a few bands: a streaming kernel (the matrix-core tiles would be nearly empty)
     dim3 grid(ceil_div(M, NTS), N);
     if (K <= 8) k_ket_skinny<8><<<grid, NTS, 0, s>>>(M, K, alpha, A, lda, B, ldb, beta, C, ldc);
@@ -360,27 +431,41 @@ void dgemm(bool transa, int M, int N, int K, double alpha, const double* A, long
     WH_HIP(hipGetLastError());
     return;
   }
-  if (backend() != Backend::Own && K > 0) {
-    WH_REQUIRE(rocblas_dgemm(blas(s), transa ? rocblas_operation_transpose : rocblas_operation_none,
-                             rocblas_operation_none, M, N, K, &alpha, A, lda, B, ldb, &beta, C, ldc) ==
-                   rocblas_status_success,
-               "rocblas_dgemm failed");
-    return;
+  const int K0 = std::max(K, 0), kc = std::max(K, 1);
+  auto run = [&](int v, double* Cv) {
+    if (transa) launch_wn<true>(v, M, N, K0, kc, 1, alpha, A, lda, B, ldb, beta, Cv, ldc, 0, s);
+    else launch_wn<false>(v, M, N, K0, kc, 1, alpha, A, lda, B, ldb, beta, Cv, ldc, 0, s);
+  };
+  // tile width, timed once per (op, M, log2 K, log2 N)
+  const VariantKey key{0, transa ? 1 : 0, M, log2_bucket(std::max(K, 1)), log2_bucket(N)};
+  std::unique_lock<std::mutex> lk(g_variant_mutex);
+  auto it = chosen_variants().find(key);
+  if (it == chosen_variants().end()) {
+    // timed on a scratch copy of C's shape: C += AB must run once only
+    double* tmp = nullptr;
+    WH_HIP(hipMalloc(&tmp, sizeof(double) * static_cast<size_t>(ldc) * N));
+    WH_HIP(hipMemsetAsync(tmp, 0, sizeof(double) * static_cast<size_t>(ldc) * N, s));
+    float t[kWnVariants] = {};
+    const int best = time_variants(kWnVariants, [&](int v) { run(v, tmp); }, s, t);
+    WH_HIP(hipStreamSynchronize(s));
+    WH_HIP(hipFree(tmp));
+    if (tune_log())
+      std::fprintf(stderr, "[west_hip] dgemm%s M=%d N=%d K=%d: tiles 128x128 %.3f 128x64 %.3f 128x32 %.3f ms -> %d\n",
+                   transa ? "_t" : "", M, N, K, t[0], t[1], t[2], best);
+    it = chosen_variants().emplace(key, best).first;
   }
-  const int gx = ceil_div(M, BM), gy = ceil_div(N, BN);
-  const int nb = xcd_blocks(gx * gy);
-  const int kc = std::max(K, 1);
-  if (transa)
-    k_dgemm<true><<<nb, NT, 0, s>>>(M, N, std::max(K, 0), kc, alpha, A, lda, B, ldb, beta, C, ldc, 0, gx, gy, 1);
-  else
-    k_dgemm<false><<<nb, NT, 0, s>>>(M, N, std::max(K, 0), kc, alpha, A, lda, B, ldb, beta, C, ldc, 0, gx, gy, 1);
-  WH_HIP(hipGetLastError());
+  const int pick = it->second;
+  lk.unlock();
+  run(pick, C);
 }
 
-// wg_target: workgroups the K split aims at (512: two per compute unit)
+// v = 3 * d + w: K split aimed at 512 << d workgroups (512: two per compute
+// unit), tile width variant w
 static void dgemm_bra_own(int M, int N, int K, double alpha, const double* A, long lda, const double* B, long ldb,
-                          double g0, const double* scale, double* C, long ldc, hipStream_t s, int wg_target = 512) {
-  const int tiles = ceil_div(M, BM) * ceil_div(N, BN);
+                          double g0, const double* scale, double* C, long ldc, hipStream_t s, int v = 0) {
+  const int w = v % kWnVariants;
+  const int wg_target = 512 << (v / kWnVariants);
+  const int tiles = ceil_div(M, BM) * ceil_div(N, 32 * wn_of(w));
   // enough workgroups for the compute units (few output tiles, as in the
   // narrow products of a partial batch, need a deep split), each K chunk at
   // least 16 steps long
@@ -389,11 +474,8 @@ static void dgemm_bra_own(int M, int N, int K, double alpha, const double* A, lo
   const int kchunk = ceil_div(ceil_div(std::max(K, 1), split), BK) * BK;
   split = ceil_div(std::max(K, 1), kchunk);
   const long pstride = static_cast<long>(ldc) * N;
-  double* P = partials().get(static_cast<size_t>(split) * pstride);
-  const int gx = ceil_div(M, BM), gy = ceil_div(N, BN);
-  k_dgemm<true><<<xcd_blocks(gx * gy * split), NT, 0, s>>>(M, N, K, kchunk, 1.0, A, lda, B, ldb, 0.0, P, ldc,
-                                                           pstride, gx, gy, split);
-  WH_HIP(hipGetLastError());
+  double* P = partials().get(static_cast<size_t>(split) * pstride, s);
+  launch_wn<true>(w, M, N, K, kchunk, split, 1.0, A, lda, B, ldb, 0.0, P, ldc, pstride, s);
   const long n = static_cast<long>(M) * N;
   k_reduce_g0<<<ceil_div(n, 256), 256, 0, s>>>(M, N, split, P, pstride, alpha, g0, A, lda, B, ldb, scale, C, ldc);
   WH_HIP(hipGetLastError());
@@ -406,7 +488,7 @@ static void dgemm_bra_skinny(int M, int N, int K, double alpha, const double* A,
   const int kchunk = ceil_div(std::max(K, 1), split);
   split = ceil_div(std::max(K, 1), kchunk);
   const long pstride = static_cast<long>(ldc) * N;
-  double* P = partials().get(static_cast<size_t>(split) * pstride);
+  double* P = partials().get(static_cast<size_t>(split) * pstride, s);
   dim3 grid(N, split);
   if (M <= 8) k_bra_skinny<8><<<grid, NTS, 0, s>>>(M, K, kchunk, A, lda, B, ldb, P, ldc, pstride);
   else if (M <= 16) k_bra_skinny<16><<<grid, NTS, 0, s>>>(M, K, kchunk, A, lda, B, ldb, P, ldc, pstride);
@@ -417,63 +499,32 @@ static void dgemm_bra_skinny(int M, int N, int K, double alpha, const double* A,
   WH_HIP(hipGetLastError());
 }
 
-static void dgemm_bra_lib(int M, int N, int K, double alpha, const double* A, long lda, const double* B, long ldb,
-                          double g0, const double* scale, double* C, long ldc, hipStream_t s) {
-  const double zero = 0.0;
-  WH_REQUIRE(rocblas_dgemm(blas(s), rocblas_operation_transpose, rocblas_operation_none, M, N, K, &alpha, A, lda, B,
-                           ldb, &zero, C, ldc) == rocblas_status_success,
-             "rocblas_dgemm failed");
-  g0_fix_scale(M, N, g0, A, lda, B, ldb, scale, C, ldc, s);
-}
-
 void dgemm_bra(int M, int N, int K, double alpha, const double* A, long lda, const double* B, long ldb, double g0,
                const double* scale, double* C, long ldc, hipStream_t s) {
   if (M <= 0 || N <= 0) return;
-  const Backend be = backend();
-  if (be == Backend::Rocblas && K > 0) return dgemm_bra_lib(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s);
   // a few rows and columns: the matrix-core tiles (128 x 128) would be almost
   // empty (with many columns the tiles' reuse of A wins again)
   if (M <= 32 && N <= 256 && K > 0) return dgemm_bra_skinny(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s);
-  if (be == Backend::Own || K <= 0) return dgemm_bra_own(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s);
-  // variants: 0..2 the kernel with its K split aimed at 512, 1024, 2048 workgroups; 3 rocBLAS
-  static std::map<std::tuple<int, int, int>, int> chosen;   // (M, K, log2 N) -> variant
-  static std::mutex chosen_mutex;
-  std::unique_lock<std::mutex> lk(chosen_mutex);
-  int lg = 0;
-  while ((2 << lg) <= N) ++lg;
-  const auto key = std::make_tuple(M, K, lg);
-  auto run = [&](int v) {
-    if (v == 3) dgemm_bra_lib(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s);
-    else dgemm_bra_own(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s, 512 << v);
-  };
-  auto it = chosen.find(key);
-  if (it == chosen.end() && N >= 32) {
-    // time every variant on this call's arguments (a first round warms them up)
-    hipEvent_t a, b;
-    WH_HIP(hipEventCreate(&a));
-    WH_HIP(hipEventCreate(&b));
-    float t[4] = {0.f, 0.f, 0.f, 0.f};
-    for (int rep = 0; rep < 2; ++rep)
-      for (int v = 0; v < 4; ++v) {
-        WH_HIP(hipEventRecord(a, s));
-        run(v);
-        WH_HIP(hipEventRecord(b, s));
-        WH_HIP(hipEventSynchronize(b));
-        float ms = 0.f;
-        WH_HIP(hipEventElapsedTime(&ms, a, b));
-        if (rep == 1) t[v] = ms;
-      }
-    WH_HIP(hipEventDestroy(a));
-    WH_HIP(hipEventDestroy(b));
-    int best = 0;
-    for (int v = 1; v < 4; ++v)
-      if (t[v] < t[best]) best = v;
-    if (tune_log())
-      std::fprintf(stderr, "[west_hip] dgemm_bra M=%d N=%d K=%d: own512 %.3f own1024 %.3f own2048 %.3f rocblas %.3f ms -> %d\n",
-                   M, N, K, t[0], t[1], t[2], t[3], best);
-    it = chosen.emplace(key, best).first;
+  if (K <= 0 || N < 32) return dgemm_bra_own(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s);
+  // variants: K split depth (3) x tile width (3), timed once per (M, K, log2 N);
+  // C is written (not accumulated), so the timing runs on the call's own output
+  constexpr int nv = 3 * kWnVariants;
+  auto run = [&](int v) { dgemm_bra_own(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s, v); };
+  const VariantKey key{1, 1, M, K, log2_bucket(N)};
+  std::unique_lock<std::mutex> lk(g_variant_mutex);
+  auto it = chosen_variants().find(key);
+  if (it == chosen_variants().end()) {
+    float t[nv] = {};
+    const int best = time_variants(nv, run, s, t);
+    if (tune_log()) {
+      std::fprintf(stderr, "[west_hip] dgemm_bra M=%d N=%d K=%d:", M, N, K);
+      for (int v = 0; v < nv; ++v) std::fprintf(stderr, " split%d/tile128x%d %.3f", 512 << (v / kWnVariants),
+                                                32 * wn_of(v % kWnVariants), t[v]);
+      std::fprintf(stderr, " ms -> %d\n", best);
+    }
+    it = chosen_variants().emplace(key, best).first;
   }
-  const int pick = it != chosen.end() ? it->second : 0;
+  const int pick = it->second;
   lk.unlock();
   // recomputed with the chosen variant, so the result does not depend on the timing order
   run(pick);

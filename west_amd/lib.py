@@ -53,8 +53,9 @@ SIGNATURES = {
     "wh_linsolve_sternheimer_m_wfcts": ([I, I, P, P, P, P, D, P], I),
     "wh_dfpt": ([I, P, P, D, I, P], I),
     "wh_apply_operator": ([I, P, P, D, I], I),
-    "wh_davidson_diago": ([P, P, P, P, P, I, P], I),
+    "wh_davidson_diago": ([P, P, P, P, P, I, P, P, P], I),
     "wh_get_counters": ([P, P], I),
+    "wh_get_operator_columns": ([P], I),
     "wh_get_precision_counters": ([P, P, P], I),
     "wh_profile": ([I], I),
     "wh_profile_query": ([C.c_char_p, P, P], I),
