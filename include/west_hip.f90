@@ -125,6 +125,17 @@ MODULE west_hip
        COMPLEX(C_DOUBLE_COMPLEX), INTENT(IN) :: alpha
      END FUNCTION
      !
+     ! no reference counterpart: the operator without device allocation (device arrays only)
+     INTEGER(C_INT) FUNCTION wh_apply_sternheimerop_to_m_wfcs_noalloc(nbndval, psi, hpsi, e, alpha, m) &
+          & BIND(C, NAME="wh_apply_sternheimerop_to_m_wfcs_noalloc")
+       IMPORT :: C_INT, C_DOUBLE, C_DOUBLE_COMPLEX
+       INTEGER(C_INT), VALUE :: nbndval, m
+       COMPLEX(C_DOUBLE_COMPLEX), INTENT(IN) :: psi(*)
+       COMPLEX(C_DOUBLE_COMPLEX), INTENT(OUT) :: hpsi(*)
+       REAL(C_DOUBLE), INTENT(IN) :: e(*)
+       COMPLEX(C_DOUBLE_COMPLEX), INTENT(IN) :: alpha
+     END FUNCTION
+     !
      INTEGER(C_INT) FUNCTION wh_linsolve_sternheimer_m_wfcts(nbndval, m, b, x, e, eprec, tr2, ierr) &
           & BIND(C, NAME="wh_linsolve_sternheimer_m_wfcts")
        IMPORT :: C_INT, C_DOUBLE, C_DOUBLE_COMPLEX

@@ -186,6 +186,22 @@ int wh_davidson_diago(const wh_davidson_params* params, double* ev, int* conv, d
  * are 0. */
 int wh_set_davidson_restart(const char* wstat_restart_dir, int n_steps_write_restart, int restart_from);
 
+/* ---- allocation-free operator -------------------------------------- */
+
+/* wh_apply_sternheimerop_to_m_wfcs with no device allocation (and no variant
+ * timing) inside the call: for callers that cannot allocate -- code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc.  No
+ * counterpart in the reference.  psi, hpsi (npwx, m) and e (m) must be device
+ * arrays; alpha is one complex scalar (host or device).  Every workspace
+ * comes from a fixed 64 MiB device array per GPU (held until the process
+ * ends); a call whose workspaces do not fit (about 2 m npw complex values plus
+ * the projection and FFT buffers) fails with an error and launches nothing.
+ * FP64 only (precision mode 0).  Calls that overlap in time on one GPU share
+ * that array and race: serialise them.  Synchronous, like the other entry
+ * points. */
+int wh_apply_sternheimerop_to_m_wfcs_noalloc(int nbndval, const double* psi, double* hpsi, const double* e,
+                                             const double* alpha, int m);
+
 /* ---- counters -------------------------------------------------------- */
 
 /* Operator calls so far as the reference's 'stern' clock counts them (one

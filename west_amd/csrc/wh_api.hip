@@ -212,7 +212,7 @@ void glbrak_q(int row0, int nq, const z64* f, int m, double* cmat) {
 //   (DFPT_kernel/apply_sternheimerop_to_m_wfcs.f90:49-76; QE h_psi_ at Gamma:
 //    kinetic + vloc_psi + calbec/add_vuspsi)
 // ---------------------------------------------------------------------------
-void stern_apply(const z64* x, z64* t, const double* d_e, int m, double alpha) {
+void stern_apply(const z64* x, z64* t, const double* d_e, int m, double alpha, const SternWs* ws) {
   Context& c = ctx();
   if (m <= 0) return;
   hipStream_t s = c.stream;
@@ -222,14 +222,14 @@ void stern_apply(const z64* x, z64* t, const double* d_e, int m, double alpha) {
     WH_TIMED("kinetic", kinetic_minus_e(m, npw, x, t, ld, c.d_g2kin, d_e, s));
   } else {
     const int npair = (m + 1) / 2;
-    const int cap = c.items_capacity();
-    FftItem* items = c.w_items.as<FftItem>(npair);
-    fill_pairs(items, m, 0, s);
-    FftItem* ident = c.w_items_id.as<FftItem>(std::min(cap, npair));
-    fill_ident(ident, std::min(cap, npair), s);
+    const int cap = ws ? ws->chunk : c.items_capacity();
     const int chunk = std::min(cap, npair);
-    z64* sa = c.w_sticks_a.as<z64>(static_cast<size_t>(chunk) * c.plan.sticks_elems());
-    z64* sb = c.w_sticks_b.as<z64>(static_cast<size_t>(chunk) * c.plan.sticks_elems());
+    FftItem* items = ws ? ws->items : c.w_items.as<FftItem>(npair);
+    fill_pairs(items, m, 0, s);
+    FftItem* ident = ws ? ws->ident : c.w_items_id.as<FftItem>(chunk);
+    fill_ident(ident, chunk, s);
+    z64* sa = ws ? ws->sa : c.w_sticks_a.as<z64>(static_cast<size_t>(chunk) * c.plan.sticks_elems());
+    z64* sb = ws ? ws->sb : c.w_sticks_b.as<z64>(static_cast<size_t>(chunk) * c.plan.sticks_elems());
     ZfwdArgs za{ZFWD_PAIR_STERN, 1.0, c.d_g2kin, d_e, nullptr, 0};
     for (int q0 = 0; q0 < npair; q0 += chunk) {
       const int nq = std::min(chunk, npair - q0);
@@ -241,15 +241,22 @@ void stern_apply(const z64* x, z64* t, const double* d_e, int m, double alpha) {
   // projector part: rows [evc ; vkb] (or evc only for the kinetic-only operator)
   const int nq = c.l_kinetic_only ? c.nbndval : c.nbndval + c.nkb;
   if (nq > 0) {
-    set_qscale_alpha(alpha);
     const long k2 = 2L * npw;
-    double* cm = c.w_cmat.as<double>(static_cast<size_t>(nq) * m);
     const double* q = reinterpret_cast<const double*>(c.d_q);
     const double* xr = reinterpret_cast<const double*>(x);
-    WH_TIMED("gemm_bra", dgemm_bra(nq, m, static_cast<int>(k2), 2.0, q, k2, xr, k2, c.gstart == 2 ? 1.0 : 0.0,
-                                   c.d_qscale, cm, nq, s));
-    WH_TIMED("gemm_ket", dgemm(false, static_cast<int>(k2), m, nq, 1.0, q, k2, cm, nq, 1.0,
-                               reinterpret_cast<double*>(t), k2, s));
+    if (ws) {
+      WH_TIMED("gemm_bra", dgemm_bra_into(nq, m, static_cast<int>(k2), 2.0, q, k2, xr, k2, c.gstart == 2 ? 1.0 : 0.0,
+                                          ws->qscale, ws->cmat, nq, ws->partials, ws->bra_variant, s));
+      WH_TIMED("gemm_ket", dgemm_untimed(false, static_cast<int>(k2), m, nq, 1.0, q, k2, ws->cmat, nq, 1.0,
+                                         reinterpret_cast<double*>(t), k2, s));
+    } else {
+      set_qscale_alpha(alpha);
+      double* cm = c.w_cmat.as<double>(static_cast<size_t>(nq) * m);
+      WH_TIMED("gemm_bra", dgemm_bra(nq, m, static_cast<int>(k2), 2.0, q, k2, xr, k2, c.gstart == 2 ? 1.0 : 0.0,
+                                     c.d_qscale, cm, nq, s));
+      WH_TIMED("gemm_ket", dgemm(false, static_cast<int>(k2), m, nq, 1.0, q, k2, cm, nq, 1.0,
+                                 reinterpret_cast<double*>(t), k2, s));
+    }
   }
   c.nstern += m;
 }
@@ -913,6 +920,7 @@ int wh_init(int device) {
   }
   g_sys.push_back(new_context(device));
   g_device = device;
+  noalloc_init(device);
   return 0;
   WH_API_END
 }

@@ -159,6 +159,8 @@ struct Context {
 };
 
 Context& ctx();
+// takes this device's address of the allocation-free operator's scratch array (wh_noalloc.hip)
+void noalloc_init(int device);
 void require_ready(bool need_states);
 
 // timed launch: the statement runs between two events when profiling is on
@@ -177,7 +179,22 @@ void require_ready(bool need_states);
   } while (0)
 
 // operations on device data in stick order (rows of npw complex, ld = npw)
-void stern_apply(const z64* x, z64* t, const double* d_e, int m, double alpha);
+// Buffers handed to stern_apply by a caller that must not allocate: FFT item
+// lists (npair and chunk items), two stick buffers of chunk items, the
+// projection coefficients (nq x m), the bra product's split-K partial sums
+// and its variant, and the projector row scales with alpha in the evc rows.
+struct SternWs {
+  FftItem* items;
+  FftItem* ident;
+  z64* sa;
+  z64* sb;
+  int chunk;
+  double* cmat;
+  double* partials;
+  int bra_variant;
+  const double* qscale;
+};
+void stern_apply(const z64* x, z64* t, const double* d_e, int m, double alpha, const SternWs* ws = nullptr);
 // maxiter <= 0: n_dfpt_maxiter
 int linsolve_dev(int m, z64* b, z64* x, const std::vector<double>& e_col, const double* d_e_col,
                  const double* d_eprec_col, double tr2, std::vector<int>* ierr_col, int maxiter = 0, int group = 0,

@@ -460,35 +460,47 @@ void dgemm(bool transa, int M, int N, int K, double alpha, const double* A, long
 }
 
 // v = 3 * d + w: K split aimed at 512 << d workgroups (512: two per compute
-// unit), tile width variant w
-static void dgemm_bra_own(int M, int N, int K, double alpha, const double* A, long lda, const double* B, long ldb,
-                          double g0, const double* scale, double* C, long ldc, hipStream_t s, int v = 0) {
+// unit), tile width variant w.  Returns the split depth and K chunk.
+static void bra_split(int M, int N, int K, int v, int& split, int& kchunk) {
   const int w = v % kWnVariants;
   const int wg_target = 512 << (v / kWnVariants);
   const int tiles = ceil_div(M, BM) * ceil_div(N, 32 * wn_of(w));
   // enough workgroups for the compute units (few output tiles, as in the
   // narrow products of a partial batch, need a deep split), each K chunk at
   // least 16 steps long
-  int split = std::max(1, std::min(64, ceil_div(wg_target, tiles)));
+  split = std::max(1, std::min(64, ceil_div(wg_target, tiles)));
   split = std::max(1, std::min(split, K / (16 * BK)));
-  const int kchunk = ceil_div(ceil_div(std::max(K, 1), split), BK) * BK;
+  kchunk = ceil_div(ceil_div(std::max(K, 1), split), BK) * BK;
   split = ceil_div(std::max(K, 1), kchunk);
+}
+
+// P: room for the split-K partial sums (null: the per-thread scratch)
+static void dgemm_bra_own(int M, int N, int K, double alpha, const double* A, long lda, const double* B, long ldb,
+                          double g0, const double* scale, double* C, long ldc, hipStream_t s, int v = 0,
+                          double* P = nullptr) {
+  int split = 1, kchunk = 1;
+  bra_split(M, N, K, v, split, kchunk);
   const long pstride = static_cast<long>(ldc) * N;
-  double* P = partials().get(static_cast<size_t>(split) * pstride, s);
-  launch_wn<true>(w, M, N, K, kchunk, split, 1.0, A, lda, B, ldb, 0.0, P, ldc, pstride, s);
+  if (!P) P = partials().get(static_cast<size_t>(split) * pstride, s);
+  launch_wn<true>(v % kWnVariants, M, N, K, kchunk, split, 1.0, A, lda, B, ldb, 0.0, P, ldc, pstride, s);
   const long n = static_cast<long>(M) * N;
   k_reduce_g0<<<ceil_div(n, 256), 256, 0, s>>>(M, N, split, P, pstride, alpha, g0, A, lda, B, ldb, scale, C, ldc);
   WH_HIP(hipGetLastError());
 }
 
-static void dgemm_bra_skinny(int M, int N, int K, double alpha, const double* A, long lda, const double* B, long ldb,
-                             double g0, const double* scale, double* C, long ldc, hipStream_t s) {
+static void skinny_split(int N, int K, int& split, int& kchunk) {
   // about 2048 workgroups, K chunks of at least 1024
-  int split = std::max(1, std::min(ceil_div(2048, N), K / 1024));
-  const int kchunk = ceil_div(std::max(K, 1), split);
+  split = std::max(1, std::min(ceil_div(2048, N), K / 1024));
+  kchunk = ceil_div(std::max(K, 1), split);
   split = ceil_div(std::max(K, 1), kchunk);
+}
+
+static void dgemm_bra_skinny(int M, int N, int K, double alpha, const double* A, long lda, const double* B, long ldb,
+                             double g0, const double* scale, double* C, long ldc, hipStream_t s, double* P = nullptr) {
+  int split = 1, kchunk = 1;
+  skinny_split(N, K, split, kchunk);
   const long pstride = static_cast<long>(ldc) * N;
-  double* P = partials().get(static_cast<size_t>(split) * pstride, s);
+  if (!P) P = partials().get(static_cast<size_t>(split) * pstride, s);
   dim3 grid(N, split);
   if (M <= 8) k_bra_skinny<8><<<grid, NTS, 0, s>>>(M, K, kchunk, A, lda, B, ldb, P, ldc, pstride);
   else if (M <= 16) k_bra_skinny<16><<<grid, NTS, 0, s>>>(M, K, kchunk, A, lda, B, ldb, P, ldc, pstride);
@@ -528,6 +540,57 @@ void dgemm_bra(int M, int N, int K, double alpha, const double* A, long lda, con
   lk.unlock();
   // recomputed with the chosen variant, so the result does not depend on the timing order
   run(pick);
+}
+
+// ---- products for callers that neither allocate nor time -------------------
+// The variant an earlier timed call chose for the shape, else a fixed one.
+static int chosen_or(const VariantKey& key, int dflt) {
+  std::lock_guard<std::mutex> lk(g_variant_mutex);
+  auto it = chosen_variants().find(key);
+  return it != chosen_variants().end() ? it->second : dflt;
+}
+
+static int bra_untimed_variant(int M, int N, int K) {
+  // default: K split aimed at 2048 workgroups on 128 x 64 tiles (the most
+  // frequent winner of the timing)
+  return (K <= 0 || N < 32) ? 0 : chosen_or(VariantKey{1, 1, M, K, log2_bucket(N)}, 7);
+}
+
+long bra_partials_doubles(int M, int N, int K, long ldc, int* variant) {
+  *variant = 0;
+  if (M <= 0 || N <= 0) return 0;
+  int split = 1, kchunk = 1;
+  if (M <= 32 && N <= 256 && K > 0) {
+    skinny_split(N, K, split, kchunk);
+  } else {
+    *variant = bra_untimed_variant(M, N, K);
+    bra_split(M, N, K, *variant, split, kchunk);
+  }
+  return static_cast<long>(split) * ldc * N;
+}
+
+void dgemm_bra_into(int M, int N, int K, double alpha, const double* A, long lda, const double* B, long ldb,
+                    double g0, const double* scale, double* C, long ldc, double* P, int variant, hipStream_t s) {
+  if (M <= 0 || N <= 0) return;
+  if (M <= 32 && N <= 256 && K > 0) return dgemm_bra_skinny(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s, P);
+  dgemm_bra_own(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s, variant, P);
+}
+
+void dgemm_untimed(bool transa, int M, int N, int K, double alpha, const double* A, long lda, const double* B,
+                   long ldb, double beta, double* C, long ldc, hipStream_t s) {
+  if (M <= 0 || N <= 0) return;
+  if (!transa && K > 0 && K <= 32 && N <= 256) return dgemm(transa, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+  // default tile width: the widest that still gives two workgroups per compute unit
+  int dflt = kWnVariants - 1;
+  for (int v = 0; v < kWnVariants; ++v)
+    if (static_cast<long>(ceil_div(M, BM)) * ceil_div(N, 32 * wn_of(v)) >= 512) {
+      dflt = v;
+      break;
+    }
+  const int v = chosen_or(VariantKey{0, transa ? 1 : 0, M, log2_bucket(std::max(K, 1)), log2_bucket(N)}, dflt);
+  const int K0 = std::max(K, 0), kc = std::max(K, 1);
+  if (transa) launch_wn<true>(v, M, N, K0, kc, 1, alpha, A, lda, B, ldb, beta, C, ldc, 0, s);
+  else launch_wn<false>(v, M, N, K0, kc, 1, alpha, A, lda, B, ldb, beta, C, ldc, 0, s);
 }
 
 void g0_fix_scale(int M, int N, double g0, const double* A, long lda, const double* B, long ldb,

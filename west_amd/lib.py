@@ -50,6 +50,7 @@ SIGNATURES = {
     "wh_set_davidson_restart": ([C.c_char_p, I, I], I),
     "wh_set_precision": ([I, D], I),
     "wh_apply_sternheimerop_to_m_wfcs": ([I, P, P, P, P, I], I),
+    "wh_apply_sternheimerop_to_m_wfcs_noalloc": ([I, P, P, P, P, I], I),
     "wh_linsolve_sternheimer_m_wfcts": ([I, I, P, P, P, P, D, P], I),
     "wh_dfpt": ([I, P, P, D, I, P], I),
     "wh_apply_operator": ([I, P, P, D, I], I),
