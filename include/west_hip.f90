@@ -125,6 +125,24 @@ MODULE west_hip
        COMPLEX(C_DOUBLE_COMPLEX), INTENT(IN) :: alpha
      END FUNCTION
      !
+     ! alpha_s(NLSTEPS, NRHS), beta_s(NLSTEPS-1, NRHS), q_s(npwx, NRHS, NLSTEPS), bnorm(NRHS)
+     INTEGER(C_INT) FUNCTION wh_solve_deflated_lanczos_w_full_ortho(nbnd_to_deflate, nrhs, nlsteps, b, alpha_s, &
+          & beta_s, q_s, bnorm) BIND(C, NAME="wh_solve_deflated_lanczos_w_full_ortho")
+       IMPORT :: C_INT, C_DOUBLE, C_DOUBLE_COMPLEX
+       INTEGER(C_INT), VALUE :: nbnd_to_deflate, nrhs, nlsteps
+       COMPLEX(C_DOUBLE_COMPLEX), INTENT(IN) :: b(*)
+       REAL(C_DOUBLE), INTENT(OUT) :: alpha_s(*), bnorm(*)
+       REAL(C_DOUBLE), INTENT(OUT), OPTIONAL :: beta_s(*)
+       COMPLEX(C_DOUBLE_COMPLEX), INTENT(OUT), OPTIONAL :: q_s(*)
+     END FUNCTION
+     !
+     INTEGER(C_INT) FUNCTION wh_glbrak_gamma(na, a, nb, b, c) BIND(C, NAME="wh_glbrak_gamma")
+       IMPORT :: C_INT, C_DOUBLE, C_DOUBLE_COMPLEX
+       INTEGER(C_INT), VALUE :: na, nb
+       COMPLEX(C_DOUBLE_COMPLEX), INTENT(IN) :: a(*), b(*)
+       REAL(C_DOUBLE), INTENT(OUT) :: c(*)
+     END FUNCTION
+     !
      ! no reference counterpart: the operator without device allocation (device arrays only)
      INTEGER(C_INT) FUNCTION wh_apply_sternheimerop_to_m_wfcs_noalloc(nbndval, psi, hpsi, e, alpha, m) &
           & BIND(C, NAME="wh_apply_sternheimerop_to_m_wfcs_noalloc")

@@ -186,6 +186,26 @@ int wh_davidson_diago(const wh_davidson_params* params, double* ev, int* conv, d
  * are 0. */
 int wh_set_davidson_restart(const char* wstat_restart_dir, int n_steps_write_restart, int restart_from);
 
+/* ---- Wfreq: chi(i omega) by deflated Lanczos chains ------------------ */
+
+/* NLSTEPS Lanczos steps of H restricted to the complement of the first
+ * nbnd_to_deflate registered states (h_psi followed by P_c), for NRHS start
+ * vectors b(npwx, NRHS) at once, with full re-orthogonalisation (modified
+ * Gram-Schmidt against all earlier vectors of the chain).  Outputs
+ * alpha_s(NLSTEPS, NRHS), beta_s(NLSTEPS-1, NRHS), the basis
+ * q_s(npwx, NRHS, NLSTEPS) (may be NULL) and bnorm(NRHS) = |b|.
+ * nbnd_to_deflate may not exceed the registered nbndval.  FP64.
+ * Replaces solve_deflated_lanczos_w_full_ortho
+ * (Wfreq/solve_deflated_lanczos_w_full_ortho.f90:14). */
+int wh_solve_deflated_lanczos_w_full_ortho(int nbnd_to_deflate, int nrhs, int nlsteps, const double* b,
+                                           double* alpha_s, double* beta_s, double* q_s, double* bnorm);
+
+/* c(na, nb) = 2 Re <a_i|b_j> - Re a_i(G=0) Re b_j(G=0) over the registered
+ * sphere, for a(npwx, na), b(npwx, nb) complex: the Gamma inner-product
+ * matrix (Hamiltonian_kernel/glbrak.f90:14-43) of get_brak_hyper_parallel
+ * (Wfreq/get_brak_hyper_parallel.f90:14), on the matrix cores. */
+int wh_glbrak_gamma(int na, const double* a, int nb, const double* b, double* c);
+
 /* ---- allocation-free operator -------------------------------------- */
 
 /* wh_apply_sternheimerop_to_m_wfcs with no device allocation (and no variant

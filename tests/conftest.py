@@ -66,6 +66,29 @@ def random_wfc(gs, m, seed):
     return a
 
 
+def dense_states(gs, vrs=None, nb=None):
+    """All (or the lowest nb) eigenpairs of H(vrs) on the sphere, by dense
+    diagonalisation of H in the real representation of Gamma wave functions."""
+    from oracle.hamiltonian import h_psi
+    from west_amd.groundstate import from_real, to_real
+    b = gs.basis
+    n = b.npw
+    dim = 2 * n - 1 if b.gstart == 2 else 2 * n
+
+    class G:
+        pass
+
+    g = G()
+    for k in ("basis", "npwx", "vkb", "dvan", "l_kinetic_only"):
+        setattr(g, k, getattr(gs, k))
+    g.vrs = gs.vrs if vrs is None else vrs
+    cols = from_real(np.eye(dim), n, gs.npwx, b.gstart)
+    H = to_real(h_psi(g, cols), n, b.gstart)
+    w, U = np.linalg.eigh(0.5 * (H + H.T))
+    nb = dim if nb is None else nb
+    return w[:nb], from_real(U[:, :nb].T, n, gs.npwx, b.gstart)
+
+
 @pytest.fixture(scope="session")
 def gpu():
     """Initialises the library on device 0 (GPU tests only)."""

@@ -78,7 +78,7 @@ static void free_context(Context* cp) {
                     &c.w_epcol, &c.w_dscr, &c.m_vrs32, &c.m_q32, &c.m_qp[0], &c.m_qp[1], &c.m_qp[2], &c.m_qtp[0],
                     &c.m_qtp[1], &c.m_qtp[2], &c.m_g, &c.m_h, &c.m_hold, &c.m_dx, &c.m_bc, &c.m_tc, &c.m_cm,
                     &c.m_work, &c.m_u2i, &c.m_rho64, &c.m_eu64, &c.m_fb, &c.m_fx, &c.m_fe, &c.m_fep,
-                    &c.w_rhob, &c.w_cgpk})
+                    &c.w_rhob, &c.w_cgpk, &c.lz_q, &c.lz_r, &c.lz_sc})
     b->release();
   for (HostBuf* b : {&c.h_l2i, &c.h_beta, &c.h_eu, &c.h_rho, &c.hm_l2i, &c.hm_beta, &c.hm_eu, &c.hm_rho, &c.hm_eu64,
                      &c.hm_u, &c.h_cgpk})
@@ -147,6 +147,8 @@ static bool on_device(const void* p) {
   }
   return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
 }
+
+bool on_device_mem(const void* p) { return on_device(p); }
 
 // device view of an input array (copied if it is host memory)
 template <typename T>

@@ -133,6 +133,7 @@ struct Context {
   HostBuf h_l2i, h_beta, h_eu, h_rho;   // pinned staging of the FP64 CG loop
   // the FP64 CG's per-iteration upload, one copy: [beta (nc) | e (nc) | list (nc ints)]
   DevBuf w_cgpk;
+  DevBuf lz_q, lz_r, lz_sc;   // Lanczos basis, residual and scalars (wh_lanczos.hip)
   HostBuf h_cgpk;
   HostBuf hm_l2i, hm_beta, hm_eu, hm_rho, hm_eu64, hm_u;   // and of the mixed-precision loop
   // mixed precision: FP32 operator with split-BF16 projectors, FP64 refinement
@@ -162,6 +163,8 @@ Context& ctx();
 // takes this device's address of the allocation-free operator's scratch array (wh_noalloc.hip)
 void noalloc_init(int device);
 void require_ready(bool need_states);
+// true for device (or managed) memory, false for host memory
+bool on_device_mem(const void* p);
 
 // timed launch: the statement runs between two events when profiling is on
 #define WH_TIMED(NAME, STMT)                                                       \

@@ -55,6 +55,8 @@ SIGNATURES = {
     "wh_dfpt": ([I, P, P, D, I, P], I),
     "wh_apply_operator": ([I, P, P, D, I], I),
     "wh_davidson_diago": ([P, P, P, P, P, I, P, P, P], I),
+    "wh_solve_deflated_lanczos_w_full_ortho": ([I, I, I, P, P, P, P, P], I),
+    "wh_glbrak_gamma": ([I, P, I, P, P], I),
     "wh_get_counters": ([P, P], I),
     "wh_get_operator_columns": ([P], I),
     "wh_get_precision_counters": ([P, P, P], I),
