@@ -87,7 +87,7 @@ def main():
                     help="run a plain streaming kernel of known size first (torch.neg over 1 GiB of float32: "
                          "1 GiB read, 1 GiB written) to scale HBM byte counters of a profiler run")
     ap.add_argument("--no-davidson", action="store_true",
-                    help="skip the Wstat Davidson wall-time run on the 6 H2 workload")
+                    help="skip the Wstat Davidson wall-time runs (6 H2, 32 PDEP; Si64, 256 PDEP)")
     ap.add_argument("--gs-cache", default=None,
                     help="npz file: load the ground state from it if present, else write it there "
                          "(lets a profiled run skip the ground-state kernels)")
@@ -157,8 +157,9 @@ def main():
     n1, n2, n3 = b.nr
     model = _byte_model(b, gs, npert, head)
     dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
-    roof = _roofline(dom, kernels[dom], model, args.steps, args.workload, _item_bytes(b))
+    roof = _roofline(dom, kernels[dom], model, args.steps, args.workload)
     e2e = _end_to_end(b, nv, value, head)
+    kbytes = _kernel_bytes_end_to_end(model, ms_step)
 
     # ---- mixed precision beside the FP64 line: same batch, same tr2
     mixed = None
@@ -180,9 +181,9 @@ def main():
                  "fp64_fallback_bands_per_step": mx["fallback_per_step"],
                  "cg_iterations_per_step": mx["iters_per_step"],
                  "kernels_ms_per_step": {k: round(v["ms_per_step"], 3) for k, v in mx["kernels"].items()},
-                 "roofline": _roofline(mdom, mx["kernels"][mdom], mmodel, args.steps, args.workload,
-                                       _item_bytes(b)),
-                 "roofline_end_to_end": _end_to_end(b, nv, mx["value"], mx)}
+                 "roofline": _roofline(mdom, mx["kernels"][mdom], mmodel, args.steps, args.workload),
+                 "kernel_bytes_end_to_end": _kernel_bytes_end_to_end(mmodel, mx["ms_per_step"]),
+                 "chi0_model_end_to_end": _end_to_end(b, nv, mx["value"], mx)}
         log(f"[bench] mixed: {mx['value']:.1f} pairs/s ({mixed['speedup_vs_fp64']}x), dng L2 rel err {l2:.2e}")
 
     # ---- CPU baseline: the FP64 oracle on a bounded sample of the same batch
@@ -198,11 +199,23 @@ def main():
         apply_operator_ref(gs, sq, dvg_h[:1], args.tr2, bands=np.arange(nb))
         tc = time.perf_counter() - t
         cpu = {"value": round(nb / tc, 4), "unit": "pairs/s", "cores": threads, "kind": "port",
+               "cpu_model": _cpu_model(),
                "sample": f"perturbation 1 of the batch x occupied bands 1..{nb} (of {nv}), tr2={args.tr2:g}, "
                          f"FP64 oracle (SciPy FFT on {threads} threads, OpenBLAS GEMM), {tc:.1f}s"}
 
-    # ---- Wstat Davidson wall time (6 H2 molecules, 32 PDEP, tr2_dfpt 1e-12), FP64 and mixed
-    dav = None if args.no_davidson else _davidson_h2(wstat, torch, args)
+    # ---- Wstat Davidson wall time, 256 PDEP of this Si64 ground state, FP64 and mixed
+    dav64 = None
+    if not args.no_davidson and args.workload == "si64":
+        wstat.set_ground_state(gs, sqvc=sq)
+        dav64 = _davidson_si64_measure(wstat, torch, args)
+
+    # ---- Wstat Davidson wall time (6 H2 molecules, 32 PDEP, tr2_dfpt 1e-12), FP64 and mixed,
+    # and the same Davidson run by the FP64 oracle on the host cores
+    dav = None
+    if not args.no_davidson:
+        dav, h2 = _davidson_h2(wstat, torch, args)
+        if cpu is not None:
+            cpu["davidson_6h2"] = _davidson_h2_cpu(h2, dav)
 
     rec = {
         "metric": "chi0.dv applies/s (pert x band), Wstat dfpt at tr2_dfpt",
@@ -221,13 +234,18 @@ def main():
         "config": {"workload": f"{args.workload}: {system.name}, npw={b.npw}, fft={n1}x{n2}x{n3}, "
                                f"nbndval={nv}, nkb={system.nkb}, {npert} perturbations/batch",
                    "pairs_per_step": pairs, "tr2_dfpt": args.tr2,
+                   "ground_state": {"scf_steps": gs.meta.get("scf_steps"),
+                                    "scf_density_residual": gs.meta.get("scf_density_residual"),
+                                    "gap_ry": round(float(gs.et[nv] - gs.et[nv - 1]), 5) if len(gs.et) > nv else None},
                    "sternheimer_applications_per_step": stern_per_step, "cg_iterations_per_step": iters_per_step},
         "kernels_ms_per_step": {k: round(v["ms_per_step"], 3) for k, v in kernels.items()},
         "roofline": roof,
-        "roofline_end_to_end": e2e,
+        "kernel_bytes_end_to_end": kbytes,
+        "chi0_model_end_to_end": e2e,
         "cpu_baseline": cpu,
         "mixed_precision": mixed,
         "davidson": dav,
+        "davidson_si64": dav64,
     }
     print(json.dumps(rec), flush=True)
 
@@ -327,10 +345,23 @@ def _batch_c4(wstat, torch, args):
 
 
 def _davidson_si64(wstat, torch, args, system, gs, npdep=256):
+    """The si64dav workload's own JSON line (see _davidson_si64_measure)."""
+    b = system.basis
+    res = _davidson_si64_measure(wstat, torch, args, npdep)
+    rec = {"metric": "Wstat Davidson wall time, 256 PDEP of the 64-atom Si cell",
+           "value": res["fp64"]["converged_setting"]["wall_s"], "unit": "s", "n_gpus": 1,
+           "higher_is_better": False, "dtype": "f64", "data": "synthetic (seeded Si supercell ground state)",
+           "config": {"workload": f"si64dav: {system.name}, npw={b.npw}, fft={'x'.join(map(str, b.nr))}, "
+                                  f"nbndval={gs.nbndval}, nkb={system.nkb}, {npdep} PDEP, n_pdep_times 4",
+                      "davidson_default": DAV_DEFAULT, "davidson_converged": dict(DAV_CONVERGED, n_pdep_maxiter=100)},
+           **res}
+    print(json.dumps(rec), flush=True)
+
+
+def _davidson_si64_measure(wstat, torch, args, npdep=256):
     """Wstat Davidson for 256 PDEP of the 64-atom Si cell (n_pdep_times 4,
     nvecx 1024), FP64 and mixed, with the default and the converged settings;
     eigenvalue errors against the converged FP64 run."""
-    b = system.basis
     res, evs = {}, {}
     for mode in ("fp64", args.mixed_mode):
         wstat.set_precision(mode)
@@ -354,18 +385,14 @@ def _davidson_si64(wstat, torch, args, system, gs, npdep=256):
     ref = evs["fp64", "converged_setting"]
     for key in ("fp64", "mixed"):
         res[key]["default"]["ev_max_rel_err_vs_converged_fp64"] = _max_rel(evs[key, "default"], ref)
-    rec = {"metric": "Wstat Davidson wall time, 256 PDEP of the 64-atom Si cell",
-           "value": res["fp64"]["converged_setting"]["wall_s"], "unit": "s", "n_gpus": 1,
-           "higher_is_better": False, "dtype": "f64", "data": "synthetic (seeded Si supercell ground state)",
-           "config": {"workload": f"si64dav: {system.name}, npw={b.npw}, fft={'x'.join(map(str, b.nr))}, "
-                                  f"nbndval={gs.nbndval}, nkb={system.nkb}, {npdep} PDEP, n_pdep_times 4",
-                      "davidson_default": DAV_DEFAULT, "davidson_converged": dict(DAV_CONVERGED, n_pdep_maxiter=100)},
-           "fp64": res["fp64"],
-           "mixed_precision": dict(res["mixed"], mode=args.mixed_mode,
-                                   ev_max_rel_err_vs_fp64_converged_setting=_max_rel(
-                                       evs["mixed", "converged_setting"], ref)),
-           "ev_fp64_lowest": [round(float(x), 8) for x in ref[:4]]}
-    print(json.dumps(rec), flush=True)
+    mixed_err = _max_rel(evs["mixed", "converged_setting"], ref)
+    return {"fp64": res["fp64"],
+            "mixed_precision": dict(res["mixed"], mode=args.mixed_mode,
+                                    ev_max_rel_err_vs_fp64_converged_setting=mixed_err,
+                                    speedup_vs_fp64_converged_setting=round(
+                                        res["fp64"]["converged_setting"]["wall_s"]
+                                        / res["mixed"]["converged_setting"]["wall_s"], 3)),
+            "ev_fp64_lowest": [round(float(x), 8) for x in ref[:4]]}
 
 
 def _davidson_h2(wstat, torch, args):
@@ -405,11 +432,44 @@ def _davidson_h2(wstat, torch, args):
     ref = evs["fp64", "converged_setting"]
     for key in ("fp64", "mixed"):
         res[key]["default"]["ev_max_rel_err_vs_converged_fp64"] = _max_rel(evs[key, "default"], ref)
+    state = {"gs": gs, "ev_fp64_default": evs["fp64", "default"]}
     return {"workload": f"{s.name}: npw={s.basis.npw}, fft={'x'.join(map(str, s.basis.nr))}, nbndval={gs.nbndval}",
             "n_pdep_eigen": 32, "n_pdep_times": 4, "davidson_default": DAV_DEFAULT,
             "davidson_converged": DAV_CONVERGED, "fp64": res["fp64"], "mixed": res["mixed"],
             "ev_max_rel_err_mixed_vs_fp64_converged_setting": _max_rel(evs["mixed", "converged_setting"], ref),
-            "ev_fp64_lowest": [round(float(x), 8) for x in ref[:4]]}
+            "ev_fp64_lowest": [round(float(x), 8) for x in ref[:4]]}, state
+
+
+def _cpu_model():
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def _davidson_h2_cpu(state, dav):
+    """The 6-H2 Davidson (32 PDEP, the default settings) by the FP64 oracle on
+    the host cores, from the same ground state: wall time against the GPU's,
+    and the PDEP eigenvalues of the FP64 GPU run against it."""
+    import oracle.fft_at_gamma as ofa
+    from oracle.davidson import davidson_diago_gamma
+    from oracle.dfpt import sqvc_init
+    gs = state["gs"]
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    ofa.WORKERS = threads
+    t = time.perf_counter()
+    log_ref = []
+    ev, conv, _ = davidson_diago_gamma(gs, sqvc_init(gs.basis, gs.npwx), 32, n_pdep_times=4, log=log_ref,
+                                       **DAV_DEFAULT)
+    wall = time.perf_counter() - t
+    log(f"[bench] davidson 6H2 on the host oracle: {wall:.1f}s, {len(log_ref)} iterations")
+    return {"wall_s": round(wall, 2), "iterations": len(log_ref), "converged": int(np.sum(conv)),
+            "cores": threads, "gpu_fp64_wall_s": dav["fp64"]["default"]["wall_s"],
+            "ev_max_rel_err_gpu_fp64_vs_cpu": _max_rel(state["ev_fp64_default"], ev),
+            "settings": DAV_DEFAULT}
 
 
 def _run_mode(wstat, torch, args, mode, npert, dvg, dng):
@@ -475,19 +535,16 @@ def _ncols(b):
     return len({(int(m[0]), int(m[1])) for m in b.mill})
 
 
-def _item_bytes(b):
-    """algorithmic bytes per transform item of the plane kernels (sticks read + written, V read)"""
-    st64 = b.nr[2] * (2 * _ncols(b) - 1) * 16
-    return {"plane_vloc": 2 * st64 + 8 * b.nnr, "plane_vloc32": st64 + 4 * b.nnr}
-
-
 def _byte_model(b, gs, npert, run):
-    """Compulsory HBM bytes (or MFMA flops) per step of each kernel class.
+    """Compulsory HBM bytes (or MFMA flops) per step of each kernel class:
+    (bound, per-step amount, amount added per launch, transform items per step).
 
     Units: a transform item is one band pair (two real bands packed in one
     complex transform); its stick block is n3 x (2*ncols - 1) complex numbers
     (16 bytes in FP64, 8 in FP32).
-    plane_vloc : read + write one stick block, read V(r) (nnr reals) per item
+    plane_vloc : read + write one stick block per item, read the real-space
+                 potential once per launch (nnr reals; the right-hand-side
+                 launches read each perturbation's dv(r) once more)
     zinv       : read two bands (2 npw complex), write one stick block
     zfwd       : read one stick block, read two bands, write two bands
     gemm_bra / gemm_ket : 2 * (2 npw) * nq flops per Sternheimer application
@@ -507,28 +564,28 @@ def _byte_model(b, gs, npert, run):
     it32 = n32 / 2.0
     items_rhs = npert * ((nv + 1) // 2)
     return {
-        "plane_vloc": ("hbm", (it64 + items_rhs) * (2 * st64 + 8 * b.nnr)),
-        "zinv": ("hbm", it64 * (2 * 16 * npw + st64)),
-        "zfwd": ("hbm", it64 * (st64 + 4 * 16 * npw)),
-        "gemm_bra": ("mfma", 2.0 * 2 * npw * nq * n64),
-        "gemm_ket": ("mfma", 2.0 * 2 * npw * nq * n64),
-        "cg_update": ("hbm", 6 * 16 * npw * n64),
-        "plane_vloc32": ("hbm", it32 * (2 * st32 + 4 * b.nnr)),
-        "zinv32": ("hbm", it32 * (2 * 8 * npw + st32)),
-        "zfwd32": ("hbm", it32 * (st32 + 4 * 8 * npw)),
-        "cg_update32": ("hbm", 6 * 8 * npw * n32),
+        "plane_vloc": ("hbm", (it64 + items_rhs) * 2 * st64 + npert * 8 * b.nnr, 8 * b.nnr, it64 + items_rhs),
+        "zinv": ("hbm", it64 * (2 * 16 * npw + st64), 0, it64),
+        "zfwd": ("hbm", it64 * (st64 + 4 * 16 * npw), 0, it64),
+        "gemm_bra": ("mfma", 2.0 * 2 * npw * nq * n64, 0, 0),
+        "gemm_ket": ("mfma", 2.0 * 2 * npw * nq * n64, 0, 0),
+        "cg_update": ("hbm", 6 * 16 * npw * n64, 0, 0),
+        "plane_vloc32": ("hbm", it32 * 2 * st32, 4 * b.nnr, it32),
+        "zinv32": ("hbm", it32 * (2 * 8 * npw + st32), 0, it32),
+        "zfwd32": ("hbm", it32 * (st32 + 4 * 8 * npw), 0, it32),
+        "cg_update32": ("hbm", 6 * 8 * npw * n32, 0, 0),
     }
 
 
-def _roofline(name, k, model, steps, workload=None, item_bytes=None):
-    """achieved = algorithmic bytes (flops) per launch / average launch time;
+def _roofline(name, k, model, steps, workload=None):
+    """achieved = compulsory bytes (flops) per launch / average launch time;
     traffic = measured HBM bytes per item x items per launch, where measured."""
     if name not in model:
         return {"kernel": name, "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": None, "traffic": None}
-    bound, per_step = model[name]
+    bound, per_step, per_launch_extra, items_per_step = model[name]
     launches_per_step = k["launches"] / steps
-    per_launch = per_step / launches_per_step
+    per_launch = per_step / launches_per_step + per_launch_extra
     avg_s = k["avg_ms"] * 1e-3
     if bound == "mfma":
         ach = per_launch / avg_s / 1e12
@@ -538,15 +595,28 @@ def _roofline(name, k, model, steps, workload=None, item_bytes=None):
     ach = per_launch / avg_s / 1e9
     traffic = None
     meas = TRAFFIC_PER_ITEM.get(workload, {}).get(name)
-    if meas and item_bytes and item_bytes.get(name):
-        traffic = int(meas * per_launch / item_bytes[name])
+    if meas and items_per_step:
+        traffic = int(meas * items_per_step / launches_per_step)
     return {"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "algorithmic_bytes_per_launch": int(per_launch),
-            "avg_launch_ms": k["avg_ms"]}
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "compulsory_bytes_per_launch": int(per_launch), "avg_launch_ms": k["avg_ms"],
+            "bytes": "stick blocks read and written once per item, the potential read once per launch"}
+
+
+def _kernel_bytes_end_to_end(model, ms_per_step):
+    """Sum of the HBM-bound kernels' compulsory bytes per step over the step
+    time: the HBM bandwidth the whole chi0 step sustains on bytes it must move."""
+    total = sum(v[1] for v in model.values() if v[0] == "hbm")
+    ach = total / (ms_per_step * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "bytes_per_step": int(total)}
 
 
 def _end_to_end(b, nv, pairs_per_s, run):
-    """Bytes per (pert, band) pair of the chi0 model, b bytes per complex and
+    """The chi0 byte model with one full-grid HBM round trip per 3D transform
+    (which the pruned stick transforms here never make, so this is a model
+    equivalent, not a measured bandwidth).  Bytes per (pert, band) pair, b
+    bytes per complex and
     s per real (FP64: b = 16, s = 8; FP32: 8, 4):
     B_iter = 2b N_r + (s/2) N_r + 16 b N_pw per operator application and CG step,
     B_pair = n_it B_iter + 4b N_r + 1.5 s N_r + 4b N_pw (FP64 right-hand side and

@@ -214,6 +214,7 @@ def make_ground_state(system: System, apply_h_factory, nbnd: int | None = None, 
                      nbndval=nbndval, occ=np.ones(nbndval), meta={"system": system.name})
     x0 = None
     hist_in, hist_res = [], []
+    res_norm = None
     for it in range(scf_steps + 1):
         gs.vrs = vrs
         et, evc = solve_bands(b, apply_h_factory(gs), nbnd, npwx, tol=tol, x0=x0, verbose=verbose)
@@ -227,6 +228,7 @@ def make_ground_state(system: System, apply_h_factory, nbnd: int | None = None, 
             rho_r += 2.0 * psir ** 2 / b.omega
         n_out = real_to_grid(b, rho_r)
         res = n_out - n_g
+        res_norm = float(np.linalg.norm(res))
         hist_in.append(n_g.copy())
         hist_res.append(res)
         if len(hist_in) > 5:
@@ -251,6 +253,8 @@ def make_ground_state(system: System, apply_h_factory, nbnd: int | None = None, 
     gs.et = et
     gs.alpha_pv = max(2.0 * (float(et[:nbndval].max()) - float(et.min())), 0.01)
     gs.meta["scf_steps"] = scf_steps
+    # |n_out - n_in| of the last mixing step (density Fourier components, 1/bohr^3)
+    gs.meta["scf_density_residual"] = res_norm
     return gs
 
 
