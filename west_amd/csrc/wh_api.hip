@@ -78,10 +78,10 @@ static void free_context(Context* cp) {
                     &c.w_epcol, &c.w_dscr, &c.m_vrs32, &c.m_q32, &c.m_qp[0], &c.m_qp[1], &c.m_qp[2], &c.m_qtp[0],
                     &c.m_qtp[1], &c.m_qtp[2], &c.m_g, &c.m_h, &c.m_hold, &c.m_dx, &c.m_bc, &c.m_tc, &c.m_cm,
                     &c.m_work, &c.m_u2i, &c.m_rho64, &c.m_eu64, &c.m_fb, &c.m_fx, &c.m_fe, &c.m_fep,
-                    &c.w_rhob, &c.w_cgpk, &c.lz_q, &c.lz_r, &c.lz_sc})
+                    &c.w_rhob, &c.w_cgpk, &c.lz_q, &c.lz_r, &c.lz_sc, &c.w_conv})
     b->release();
   for (HostBuf* b : {&c.h_l2i, &c.h_beta, &c.h_eu, &c.h_rho, &c.hm_l2i, &c.hm_beta, &c.hm_eu, &c.hm_rho, &c.hm_eu64,
-                     &c.hm_u, &c.h_cgpk})
+                     &c.hm_u, &c.h_cgpk, &c.h_cgrec})
     b->release();
   for (DevBuf& b : c.dav) b.release();
   (void)hipStreamDestroy(c.stream);
@@ -214,7 +214,8 @@ void glbrak_q(int row0, int nq, const z64* f, int m, double* cmat) {
 //   (DFPT_kernel/apply_sternheimerop_to_m_wfcs.f90:49-76; QE h_psi_ at Gamma:
 //    kinetic + vloc_psi + calbec/add_vuspsi)
 // ---------------------------------------------------------------------------
-void stern_apply(const z64* x, z64* t, const double* d_e, int m, double alpha, const SternWs* ws) {
+void stern_apply(const z64* x, z64* t, const double* d_e, int m, double alpha, const SternWs* ws,
+                 const int* d_nc) {
   Context& c = ctx();
   if (m <= 0) return;
   hipStream_t s = c.stream;
@@ -227,16 +228,25 @@ void stern_apply(const z64* x, z64* t, const double* d_e, int m, double alpha, c
     const int cap = ws ? ws->chunk : c.items_capacity();
     const int chunk = std::min(cap, npair);
     FftItem* items = ws ? ws->items : c.w_items.as<FftItem>(npair);
-    fill_pairs(items, m, 0, s);
-    FftItem* ident = ws ? ws->ident : c.w_items_id.as<FftItem>(chunk);
-    fill_ident(ident, chunk, s);
+    FftItem* ident = nullptr;
+    if (d_nc) {
+      // items of the inactive tail are switched off on the device
+      fill_pairs_dev(items, npair, d_nc, s);
+      ident = c.w_items_id.as<FftItem>(npair);
+      fill_ident_dev(ident, npair, chunk, d_nc, s);
+    } else {
+      fill_pairs(items, m, 0, s);
+      ident = ws ? ws->ident : c.w_items_id.as<FftItem>(chunk);
+      fill_ident(ident, chunk, s);
+    }
     z64* sa = ws ? ws->sa : c.w_sticks_a.as<z64>(static_cast<size_t>(chunk) * c.plan.sticks_elems());
     z64* sb = ws ? ws->sb : c.w_sticks_b.as<z64>(static_cast<size_t>(chunk) * c.plan.sticks_elems());
     ZfwdArgs za{ZFWD_PAIR_STERN, 1.0, c.d_g2kin, d_e, nullptr, 0};
     for (int q0 = 0; q0 < npair; q0 += chunk) {
       const int nq = std::min(chunk, npair - q0);
       WH_TIMED("zinv", launch_zinv(c.plan, items + q0, nq, x, ld, sa, ZINV_PAIR, s));
-      WH_TIMED("plane_vloc", launch_plane_vloc(c.plan, ident, nq, sa, sb, c.d_vrs, 0, 1.0 / c.nnr, s));
+      WH_TIMED("plane_vloc", launch_plane_vloc(c.plan, d_nc ? ident + q0 : ident, nq, sa, sb, c.d_vrs, 0,
+                                               1.0 / c.nnr, s));
       WH_TIMED("zfwd", launch_zfwd(c.plan, items + q0, nq, sb, x, ld, t, ld, za, s));
     }
   }
@@ -255,12 +265,12 @@ void stern_apply(const z64* x, z64* t, const double* d_e, int m, double alpha, c
       set_qscale_alpha(alpha);
       double* cm = c.w_cmat.as<double>(static_cast<size_t>(nq) * m);
       WH_TIMED("gemm_bra", dgemm_bra(nq, m, static_cast<int>(k2), 2.0, q, k2, xr, k2, c.gstart == 2 ? 1.0 : 0.0,
-                                     c.d_qscale, cm, nq, s));
+                                     c.d_qscale, cm, nq, s, d_nc));
       WH_TIMED("gemm_ket", dgemm(false, static_cast<int>(k2), m, nq, 1.0, q, k2, cm, nq, 1.0,
-                                 reinterpret_cast<double*>(t), k2, s));
+                                 reinterpret_cast<double*>(t), k2, s, d_nc));
     }
   }
-  c.nstern += m;
+  if (!d_nc) c.nstern += m;   // device-driven callers add their counts themselves
 }
 
 void count_calls(int n, const int* cols, int group, const int* col_id) {
@@ -283,6 +293,7 @@ void count_calls(int n, const int* cols, int group, const int* col_id) {
 int linsolve_dev(int m, z64* b, z64* x, const std::vector<double>& e_col, const double* d_e_col,
                  const double* d_eprec_col, double tr2, std::vector<int>* ierr_col, int maxiter, int group,
                  const int* col_id) {
+  (void)e_col;
   Context& c = ctx();
   hipStream_t s = c.stream;
   const int npw = c.npw;
@@ -293,90 +304,79 @@ int linsolve_dev(int m, z64* b, z64* x, const std::vector<double>& e_col, const 
   z64* hold = c.w_hold.as<z64>(vec);
   z64* bc = c.w_bc.as<z64>(vec);
   z64* tc = c.w_tc.as<z64>(vec);
-  // each iteration's active list, beta and e go to the device in one copy,
-  // laid out for that iteration's nc: [beta (nc) | e (nc) | list (nc ints)]
-  double* d_pk = c.w_cgpk.as<double>(static_cast<size_t>(5) * m / 2 + 1);
-  double* pk = c.h_cgpk.as<double>(static_cast<size_t>(5) * m / 2 + 1);
-  int* d_l2i = c.w_l2i.as<int>(m);   // the first residual's list; then inside d_pk
-  double* d_beta = nullptr;
-  double* d_eu = nullptr;
-  // rho of each band's current g, by band: from prec_rho for the first
-  // residual, then from the update kernel of the previous iteration
-  double* d_rhob = c.w_rhob.as<double>(m);
+  // the iteration's active list and coefficients, built on the device
+  int* d_l2i = c.w_l2i.as<int>(m);
+  double* d_beta = c.w_beta.as<double>(m);
+  double* d_eu = c.w_eu.as<double>(m);
+  double* d_rhob = c.w_rhob.as<double>(m);   // rho by band: prec_rho first, then the update kernel
+  double* d_rhoold = c.w_rho.as<double>(m);
+  int* d_conv = c.w_conv.as<int>(static_cast<size_t>(m) + 4);
+  int* d_nc = d_conv + m;       // active count of the current iteration
+  int* d_rec = d_conv + m + 1;  // (active count, perturbations among them)
   const RhoOut rho_out{c.d_g2kin, d_eprec_col, d_rhob};
-  // step 1: g = A x - b
+  const int itmax = maxiter > 0 ? maxiter : c.n_dfpt_maxiter;
+  // one record per iteration, read by the host two iterations late
+  int* rec = c.h_cgrec.as<int>(2 * static_cast<size_t>(itmax + 1));
+  WH_HIP(hipMemsetAsync(d_conv, 0, sizeof(int) * (static_cast<size_t>(m) + 4), s));
+  // step 1: g = A x - b, and rho of every band
   stern_apply(x, g, d_e_col, m, c.alpha_pv);
   count_calls(m, nullptr, group, col_id);
   axpby(static_cast<long>(vec), -1.0, b, 1.0, g, s);
-  std::vector<char> is_conv(m, 0);
-  // pinned: the lists and coefficients go to the device with asynchronous copies
-  int* l2i = c.h_l2i.as<int>(m);
-  double* rho_c = c.h_rho.as<double>(m);
-  std::iota(l2i, l2i + m, 0);
-  std::vector<double> rho(m, 0.0), rhoold(m, 0.0);
-  int nc = m;
-  const int itmax = maxiter > 0 ? maxiter : c.n_dfpt_maxiter;
-  // Small batches (a molecule's few bands: every product below is a
-  // streaming kernel) are launch-bound: the device work of an iteration
-  // (direction, operator, update) is captured in a HIP graph the second time
-  // the same active-column count and h/hold assignment come up, and replayed
-  // while they stay.  All buffers were sized by the first, eager iteration
-  // (nc only shrinks), so nothing inside allocates.  WH_GRAPH=0 keeps every
-  // iteration eager; so do the profiler's timed mode and a process holding
-  // several systems (their threads' device-wide synchronisations may not
-  // meet a capture).
+  iota(d_l2i, m, s);
+  WH_TIMED("cg_prec", prec_rho(m, npw, c.gstart, d_l2i, g, ld, c.d_g2kin, d_eprec_col, d_rhob, s, true));
+  // The loop (linsolve_sternheimer_m_wfcts.f90:146-330) runs on the device:
+  // each iteration starts with cg_select (convergence test, compaction,
+  // beta), and its kernels are launched for a capacity -- the active count
+  // the host last read, rounded up to a power of two -- and skip the columns
+  // beyond the device's count.  The host never waits for the iteration it
+  // launches: it reads the active count of the iteration two back (already
+  // finished while the next one is queued), to size the launches and to stop
+  // once a count of zero is seen; iterations launched after the last band
+  // converged do nothing.
+  // Narrow batches (every product a streaming kernel, launch-bound) replay
+  // an iteration captured in a HIP graph the second time the same capacity
+  // and h/hold assignment come up; WH_GRAPH=0 keeps every iteration eager;
+  // so do the profiler's timed mode and a process holding several systems
+  // (their threads' device-wide synchronisations may not meet a capture).
   const int nqp = c.l_kinetic_only ? c.nbndval : c.nbndval + c.nkb;
   const char* genv = std::getenv("WH_GRAPH");
   const bool graphs = !(genv && genv[0] == '0') && !c.prof.on && nqp <= 32 && m <= 256 && single_system();
   struct CgGraph {
-    int nc;
+    int cap;
     const void* h;
     hipGraphExec_t exec;
   };
   std::vector<CgGraph> cg_graphs;
-  int eager_nc = -1;   // active-column count of the last eager iteration
+  std::vector<std::pair<int, const void*>> eager_seen;
+  hipEvent_t ev[3];
+  for (hipEvent_t& e : ev) WH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  int last = 0;   // iterations launched
+  int cap = m;
   for (int iter = 1; iter <= itmax; ++iter) {
-    // (from iteration 2 on, d_l2i already holds this list: it was sent with beta)
-    if (iter == 1) {
-      WH_HIP(hipMemcpyAsync(d_l2i, l2i, sizeof(int) * nc, hipMemcpyHostToDevice, s));
-      WH_TIMED("cg_prec", prec_rho(nc, npw, c.gstart, d_l2i, g, ld, c.d_g2kin, d_eprec_col, d_rhob, s, true));
+    if (iter >= 3) {
+      WH_HIP(hipEventSynchronize(ev[(iter - 2) % 3]));
+      const int nlag = rec[2 * (iter - 2)];
+      if (nlag == 0) break;
+      int p2 = 1;
+      while (p2 < nlag) p2 <<= 1;
+      cap = std::min(m, p2);
     }
-    WH_HIP(hipMemcpyAsync(rho_c, d_rhob, sizeof(double) * m, hipMemcpyDeviceToHost, s));
-    WH_HIP(hipStreamSynchronize(s));
-    ++c.cg_iters;
-    for (int lb = 0; lb < nc; ++lb) {
-      const int ib = l2i[lb];
-      rho[ib] = rho_c[ib];
-      if (rho[ib] < tr2) is_conv[ib] = 1;
-    }
-    int n2 = 0;
-    for (int ib = 0; ib < m; ++ib)
-      if (!is_conv[ib]) l2i[n2++] = ib;
-    nc = n2;
-    if (nc == 0) break;
-    int* pl = reinterpret_cast<int*>(pk + 2 * nc);
-    for (int lb = 0; lb < nc; ++lb) {
-      const int ib = l2i[lb];
-      pk[lb] = (iter != 1) ? rho[ib] / rhoold[ib] : 0.0;   // beta
-      pk[nc + lb] = e_col[ib];
-      pl[lb] = ib;
-    }
-    WH_HIP(hipMemcpyAsync(d_pk, pk, (2 * sizeof(double) + sizeof(int)) * nc, hipMemcpyHostToDevice, s));
-    count_calls(nc, l2i, group, col_id);
-    d_beta = d_pk;
-    d_eu = d_pk + nc;
-    d_l2i = reinterpret_cast<int*>(d_pk + 2 * nc);
+    const int first = iter == 1 ? 1 : 0;
     auto body = [&]() {
-      WH_TIMED("cg_dir", direction(nc, npw, d_l2i, d_beta, g, c.d_g2kin, d_eprec_col, h, hold, bc, ld, s));
-      stern_apply(bc, tc, d_eu, nc, c.alpha_pv);
-      WH_TIMED("cg_update", cg_update(nc, npw, c.gstart, d_l2i, h, g, tc, x, ld, nullptr, s, &rho_out));
+      WH_TIMED("cg_select", cg_select(m, tr2, first, group, d_rhob, d_rhoold, d_conv, d_e_col, d_l2i, d_beta, d_eu,
+                                      d_nc, d_rec, s));
+      WH_TIMED("cg_dir", direction(cap, npw, d_l2i, d_beta, g, c.d_g2kin, d_eprec_col, h, hold, bc, ld, s, d_nc));
+      stern_apply(bc, tc, d_eu, cap, c.alpha_pv, nullptr, d_nc);
+      WH_TIMED("cg_update", cg_update(cap, npw, c.gstart, d_l2i, h, g, tc, x, ld, nullptr, s, &rho_out, d_nc));
     };
     const CgGraph* cached = nullptr;
     for (const CgGraph& gr : cg_graphs)
-      if (gr.nc == nc && gr.h == h) cached = &gr;
-    if (cached) {
+      if (gr.cap == cap && gr.h == h) cached = &gr;
+    const std::pair<int, const void*> key{cap, h};
+    if (cached && !first) {
       WH_HIP(hipGraphLaunch(cached->exec, s));
-    } else if (graphs && eager_nc == nc) {
+    } else if (graphs && !first &&
+               std::find(eager_seen.begin(), eager_seen.end(), key) != eager_seen.end()) {
       hipGraph_t graph = nullptr;
       WH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
       body();
@@ -384,23 +384,39 @@ int linsolve_dev(int m, z64* b, z64* x, const std::vector<double>& e_col, const 
       hipGraphExec_t exec = nullptr;
       WH_HIP(hipGraphInstantiateWithFlags(&exec, graph, 0));
       WH_HIP(hipGraphDestroy(graph));
-      cg_graphs.push_back({nc, h, exec});
+      cg_graphs.push_back({cap, h, exec});
       WH_HIP(hipGraphLaunch(exec, s));
     } else {
       body();
-      eager_nc = nc;
+      if (!first) eager_seen.push_back(key);
     }
+    WH_HIP(hipMemcpyAsync(rec + 2 * iter, d_rec, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    WH_HIP(hipEventRecord(ev[iter % 3], s));
+    last = iter;
     std::swap(h, hold);   // hold <- h  (linsolve 319)
-    for (int lb = 0; lb < nc; ++lb) rhoold[l2i[lb]] = rho[l2i[lb]];
-    // the host lists sent above are next modified after the synchronisation
-    // that follows the next iteration's rho read, which orders those copies
   }
   WH_HIP(hipStreamSynchronize(s));
+  for (hipEvent_t e : ev) WH_HIP(hipEventDestroy(e));
   for (const CgGraph& gr : cg_graphs) WH_HIP(hipGraphExecDestroy(gr.exec));
-  if (ierr_col) {
-    ierr_col->assign(m, 0);
-    for (int lb = 0; lb < nc; ++lb) (*ierr_col)[l2i[lb]] = 1;
+  // counters: an iteration counts when it had active bands
+  for (int it = 1; it <= last; ++it) {
+    const int n = rec[2 * it];
+    if (n > 0) {
+      ++c.cg_iters;
+      c.nstern += n;
+      c.nstern_calls += rec[2 * it + 1];
+    }
   }
+  // ierr: the bands still active when the iterations ran out (linsolve 332)
+  std::vector<int> conv(m);
+  WH_HIP(hipMemcpy(conv.data(), d_conv, sizeof(int) * m, hipMemcpyDeviceToHost));
+  int nc = 0;
+  if (ierr_col) ierr_col->assign(m, 0);
+  for (int ib = 0; ib < m; ++ib)
+    if (!conv[ib]) {
+      ++nc;
+      if (ierr_col) (*ierr_col)[ib] = 1;
+    }
   return nc;
 }
 

@@ -134,6 +134,8 @@ struct Context {
   // the FP64 CG's per-iteration upload, one copy: [beta (nc) | e (nc) | list (nc ints)]
   DevBuf w_cgpk;
   DevBuf lz_q, lz_r, lz_sc;   // Lanczos basis, residual and scalars (wh_lanczos.hip)
+  DevBuf w_conv;               // device-driven CG: flags, count and record
+  HostBuf h_cgrec;             // per-iteration (active count, calls), pinned
   HostBuf h_cgpk;
   HostBuf hm_l2i, hm_beta, hm_eu, hm_rho, hm_eu64, hm_u;   // and of the mixed-precision loop
   // mixed precision: FP32 operator with split-BF16 projectors, FP64 refinement
@@ -197,7 +199,10 @@ struct SternWs {
   int bra_variant;
   const double* qscale;
 };
-void stern_apply(const z64* x, z64* t, const double* d_e, int m, double alpha, const SternWs* ws = nullptr);
+// d_nc (device int, may be null): m is a capacity and only the first *d_nc
+// columns are active (the device-driven CG); inactive columns are skipped.
+void stern_apply(const z64* x, z64* t, const double* d_e, int m, double alpha, const SternWs* ws = nullptr,
+                 const int* d_nc = nullptr);
 // maxiter <= 0: n_dfpt_maxiter
 int linsolve_dev(int m, z64* b, z64* x, const std::vector<double>& e_col, const double* d_e_col,
                  const double* d_eprec_col, double tr2, std::vector<int>* ierr_col, int maxiter = 0, int group = 0,

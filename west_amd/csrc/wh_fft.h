@@ -114,6 +114,8 @@ void plane_occ_launch(int& chosen, int max_occ, int nitems, Run run, hipStream_t
 
 // One transform item: bands a and b (row indices into a vector block; b < 0 -> zero),
 // pot = index of the real-space potential (per-item multiply) or -1 for the shared one.
+// a < 0 marks an inactive item: the kernels skip it (launches sized for a
+// capacity whose tail the device has switched off).
 struct FftItem {
   int a, b, pot, pad;
 };

@@ -168,6 +168,7 @@ __global__ __launch_bounds__(kNT) void k_zinv(StickPlanDev P, const FftItem* ite
   const int c1 = min(c0 + kCPW, P.ncs);
   const int ncl = c1 - c0;
   const FftItem item = items[it];
+  if (item.a < 0) return;   // inactive item (uniform per workgroup)
   const z64* A = psiA + static_cast<long>(item.a) * ldA;
   const z64* B = item.b >= 0 ? psiB + static_cast<long>(item.b) * ldB : nullptr;
   const z64* tws = reinterpret_cast<const z64*>(P.tw3);
@@ -272,6 +273,7 @@ __global__ __launch_bounds__(kNT) void k_plane_vloc(StickPlanDev P, const FftIte
   const int z = blockIdx.x;
   const int it = blockIdx.y;
   const FftItem item = items[it];
+  if (item.a < 0) return;   // inactive item (uniform per workgroup)
   const long sl = static_cast<long>(P.n3) * P.nct;
   plane_load(P, S, sin + static_cast<long>(item.a) * sl + static_cast<long>(z) * P.nct);
   lds_fft<double, kNT, MAXE>(S.pl, P.nrows, P.n2, P.ax2, S.tw2, +1);
@@ -305,6 +307,7 @@ __global__ __launch_bounds__(kNT) void k_plane_to_real(StickPlanDev P, const Fft
   const int z = blockIdx.x;
   const int it = blockIdx.y;
   const FftItem item = items[it];
+  if (item.a < 0) return;   // inactive item (uniform per workgroup)
   const long sl = static_cast<long>(P.n3) * P.nct;
   plane_load(P, S, sin + static_cast<long>(it) * sl + static_cast<long>(z) * P.nct);
   lds_fft<double, kNT, MAXE>(S.pl, P.nrows, P.n2, P.ax2, S.tw2, +1);
@@ -388,6 +391,7 @@ __global__ __launch_bounds__(kNT) void k_zfwd(StickPlanDev P, const FftItem* ite
   const int c1 = min(c0 + kCPW, P.ncs);
   const int ncl = c1 - c0;
   const FftItem item = items[it];
+  if (item.a < 0) return;   // inactive item (uniform per workgroup)
   const z64* tws = reinterpret_cast<const z64*>(P.tw3);
   for (int t = threadIdx.x; t < n3; t += kNT) tw[t] = tws[t];
   const z64* src = sticks + static_cast<long>(it) * n3 * P.nct;

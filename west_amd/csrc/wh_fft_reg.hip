@@ -108,6 +108,7 @@ __global__ __launch_bounds__((nt_reg<N, PZ>())) __attribute__((amdgpu_waves_per_
   const int z0 = blockIdx.x * PZ;
   const int nz = min(PZ, N - z0);
   const FftItem item = items[blockIdx.y];
+  if (item.a < 0) return;   // inactive item (uniform per workgroup)
   const long sl = static_cast<long>(N) * nct;
   const int tid = threadIdx.x;
 
@@ -262,6 +263,7 @@ __global__ __launch_bounds__(ZNT) __attribute__((amdgpu_waves_per_eu(2))) void k
   const int S0 = P.col_off[c0];
   const int npt = P.col_off[c1] - S0;
   const FftItem item = items[it];
+  if (item.a < 0) return;   // inactive item (uniform per workgroup)
   const z32* A = psiA + static_cast<long>(item.a) * ldA + S0;
   const z32* B = item.b >= 0 ? psiB + static_cast<long>(item.b) * ldB + S0 : nullptr;
   {
@@ -337,6 +339,7 @@ __global__ __launch_bounds__(ZNT) __attribute__((amdgpu_waves_per_eu(2))) void k
   const int c0 = blockIdx.x * kZregCols;
   const int c1 = min(c0 + kZregCols, P.ncs);
   const FftItem item = items[it];
+  if (item.a < 0) return;   // inactive item (uniform per workgroup)
   {
     const bool own = threadIdx.x < kZregCols;
     const int lc = own ? threadIdx.x : threadIdx.x - kZregCols;

@@ -266,6 +266,7 @@ __global__ __launch_bounds__(NT) void k_zinv_t(StickPlanDev P, const FftItem* it
   const int c1 = min(c0 + CPW, P.ncs);
   const int ncl = c1 - c0;
   const FftItem item = items[it];
+  if (item.a < 0) return;   // inactive item (uniform per workgroup)
   const cplx<V>* A = psiA + static_cast<long>(item.a) * ldA;
   const cplx<V>* B = item.b >= 0 ? psiB + static_cast<long>(item.b) * ldB : nullptr;
   load_tw<T, N>(tw, P.tw3);
@@ -323,6 +324,7 @@ __global__ __launch_bounds__(NT) void k_zfwd_t(StickPlanDev P, const FftItem* it
   const int c1 = min(c0 + CPW, P.ncs);
   const int ncl = c1 - c0;
   const FftItem item = items[it];
+  if (item.a < 0) return;   // inactive item (uniform per workgroup)
   load_tw<T, N>(tw, P.tw3);
   const cplx<T>* src = sticks + static_cast<long>(it) * N * P.nct;
   const bool single = A.mode == ZFWD_SINGLE;
@@ -439,6 +441,7 @@ __global__ __launch_bounds__(NTP) __attribute__((amdgpu_waves_per_eu(2 * OCC))) 
   const int z = blockIdx.x;
   const int it = blockIdx.y;
   const FftItem item = items[it];
+  if (item.a < 0) return;   // inactive item (uniform per workgroup)
   const long sl = static_cast<long>(N) * P.nct;
   const int imax = P.imax;
   load_tw<T, N>(S.tw, P.tw1);
@@ -490,6 +493,7 @@ __global__ __launch_bounds__(NTP) void k_plane_to_real_t(StickPlanDev P, const F
   const int z = blockIdx.x;
   const int it = blockIdx.y;
   const FftItem item = items[it];
+  if (item.a < 0) return;   // inactive item (uniform per workgroup)
   const long sl = static_cast<long>(N) * P.nct;
   const int imax = P.imax;
   load_tw<T, N>(S.tw, P.tw1);

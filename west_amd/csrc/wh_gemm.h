@@ -6,13 +6,15 @@
 namespace wh {
 
 // C (M x N) = alpha * op(A) * B + beta * C, column major; op(A) = A^T if transa.
+// d_n (device int, may be null): only the first min(N, *d_n) columns of B and
+// C are used (launches sized for a capacity whose tail the device switched off).
 void dgemm(bool transa, int M, int N, int K, double alpha, const double* A, long lda, const double* B, long ldb,
-           double beta, double* C, long ldc, hipStream_t s);
+           double beta, double* C, long ldc, hipStream_t s, const int* d_n = nullptr);
 
 // Gamma bra product: C(i, j) = scale_i * (alpha * (A^T B)(i, j) - g0 * A[i*lda] * B[j*ldb])
 // (A stored K x M); K is split across workgroups and summed in a fixed order.
 void dgemm_bra(int M, int N, int K, double alpha, const double* A, long lda, const double* B, long ldb, double g0,
-               const double* scale, double* C, long ldc, hipStream_t s);
+               const double* scale, double* C, long ldc, hipStream_t s, const int* d_n = nullptr);
 
 // C(i, j) = scale_i * (C(i, j) - g0 * A[i*lda] * B[j*ldb])   (scale may be null)
 void g0_fix_scale(int M, int N, double g0, const double* A, long lda, const double* B, long ldb,

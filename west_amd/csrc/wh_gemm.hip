@@ -107,8 +107,14 @@ template <bool TA, int WN>
 __global__ __launch_bounds__(NT, 2) void k_dgemm(int M, int N, int K, int kchunk, double alpha,
                                               const double* __restrict__ A, long lda, const double* __restrict__ B,
                                               long ldb, double beta, double* __restrict__ C, long ldc, long pstride,
-                                              int gx, int gy, int gz) {
+                                              int gx, int gy, int gz, const int* __restrict__ d_n) {
   constexpr int TN = 32 * WN;
+  if (d_n) {
+    // columns the device left active: the tiles of the active columns are
+    // renumbered over the whole grid, so they still spread over all XCDs
+    N = min(N, *d_n);
+    gy = min(gy, (N + TN - 1) / TN);
+  }
   __shared__ double As[2][BK][BM + PAD];
   __shared__ double Bs[2][BK][TN + PAD];
   const int tid = threadIdx.x;
@@ -117,10 +123,14 @@ __global__ __launch_bounds__(NT, 2) void k_dgemm(int M, int N, int K, int kchunk
   const int wm = wave & 1;
   const int wn = wave >> 1;
   const int tiles = gx * gy * gz;
+  // (with fewer active tiles than the grid was sized for, only the first
+  // xcd_blocks(tiles) workgroups take part, so each tile has exactly one)
+  if (static_cast<int>(blockIdx.x) >= xcd_blocks(tiles)) return;
   const int W = static_cast<int>(blockIdx.x % kXcds) * (xcd_blocks(tiles) / kXcds) + static_cast<int>(blockIdx.x / kXcds);
   if (W >= tiles) return;   // uniform per workgroup
   const int m0 = (W % gx) * BM;
   const int n0 = ((W / gx) % gy) * TN;
+  if (n0 >= N) return;      // uniform per workgroup
   const int kz = W / (gx * gy);
   const int kbeg = kz * kchunk;
   const int kend = min(K, kbeg + kchunk);
@@ -245,9 +255,10 @@ constexpr int NTS = 256;
 template <int MB>
 __global__ __launch_bounds__(NTS) void k_bra_skinny(int M, int K, int kchunk, const double* __restrict__ A, long lda,
                                                    const double* __restrict__ B, long ldb, double* __restrict__ P,
-                                                   long ldc, long pstride) {
+                                                   long ldc, long pstride, const int* __restrict__ d_n) {
   __shared__ double red[MB][NTS / 64];
   const int j = blockIdx.x;
+  if (d_n && j >= *d_n) return;
   const int z = blockIdx.y;
   const int kb = z * kchunk;
   const int ke = min(K, kb + kchunk);
@@ -286,9 +297,10 @@ __global__ __launch_bounds__(NTS) void k_bra_skinny(int M, int K, int kchunk, co
 template <int KB>
 __global__ __launch_bounds__(NTS) void k_ket_skinny(int M, int K, double alpha, const double* __restrict__ A, long lda,
                                                    const double* __restrict__ B, long ldb, double beta,
-                                                   double* __restrict__ C, long ldc) {
+                                                   double* __restrict__ C, long ldc, const int* __restrict__ d_n) {
   __shared__ double bj[KB];
   const int j = blockIdx.y;
+  if (d_n && j >= *d_n) return;
   if (threadIdx.x < K) bj[threadIdx.x] = B[static_cast<long>(j) * ldb + threadIdx.x];
   __syncthreads();
   const int r = blockIdx.x * NTS + threadIdx.x;
@@ -305,7 +317,8 @@ __global__ __launch_bounds__(NTS) void k_ket_skinny(int M, int K, double alpha, 
 //   P_z: split partials (ldc, stride pstride); a(1, i) = A[i*lda], b(1, j) = B[j*ldb]
 __global__ void k_reduce_g0(int M, int N, int nsplit, const double* P, long pstride, double alpha, double g0,
                             const double* A, long lda, const double* B, long ldb, const double* s, double* C,
-                            long ldc) {
+                            long ldc, const int* d_n) {
+  if (d_n) N = min(N, *d_n);
   const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (idx >= static_cast<long>(M) * N) return;
   const int i = static_cast<int>(idx % M);
@@ -363,20 +376,22 @@ Scratch& partials() {
 
 template <bool TA, int WN>
 static void launch_tiles(int M, int N, int K, int kchunk, int gz, double alpha, const double* A, long lda,
-                         const double* B, long ldb, double beta, double* C, long ldc, long pstride, hipStream_t s) {
+                         const double* B, long ldb, double beta, double* C, long ldc, long pstride, hipStream_t s,
+                         const int* d_n) {
   const int gx = ceil_div(M, BM), gy = ceil_div(N, 32 * WN);
   k_dgemm<TA, WN><<<xcd_blocks(gx * gy * gz), NT, 0, s>>>(M, N, K, kchunk, alpha, A, lda, B, ldb, beta, C, ldc,
-                                                          pstride, gx, gy, gz);
+                                                          pstride, gx, gy, gz, d_n);
   WH_HIP(hipGetLastError());
 }
 
 // tile width variants: WN = 4, 2, 1 (128-, 64- and 32-column tiles)
 template <bool TA>
 static void launch_wn(int v, int M, int N, int K, int kchunk, int gz, double alpha, const double* A, long lda,
-                      const double* B, long ldb, double beta, double* C, long ldc, long pstride, hipStream_t s) {
-  if (v == 0) launch_tiles<TA, 4>(M, N, K, kchunk, gz, alpha, A, lda, B, ldb, beta, C, ldc, pstride, s);
-  else if (v == 1) launch_tiles<TA, 2>(M, N, K, kchunk, gz, alpha, A, lda, B, ldb, beta, C, ldc, pstride, s);
-  else launch_tiles<TA, 1>(M, N, K, kchunk, gz, alpha, A, lda, B, ldb, beta, C, ldc, pstride, s);
+                      const double* B, long ldb, double beta, double* C, long ldc, long pstride, hipStream_t s,
+                      const int* d_n = nullptr) {
+  if (v == 0) launch_tiles<TA, 4>(M, N, K, kchunk, gz, alpha, A, lda, B, ldb, beta, C, ldc, pstride, s, d_n);
+  else if (v == 1) launch_tiles<TA, 2>(M, N, K, kchunk, gz, alpha, A, lda, B, ldb, beta, C, ldc, pstride, s, d_n);
+  else launch_tiles<TA, 1>(M, N, K, kchunk, gz, alpha, A, lda, B, ldb, beta, C, ldc, pstride, s, d_n);
 }
 constexpr int kWnVariants = 3;
 inline int wn_of(int v) { return 4 >> v; }
@@ -420,21 +435,21 @@ static int log2_bucket(int n) {
 }
 
 void dgemm(bool transa, int M, int N, int K, double alpha, const double* A, long lda, const double* B, long ldb,
-           double beta, double* C, long ldc, hipStream_t s) {
+           double beta, double* C, long ldc, hipStream_t s, const int* d_n) {
   if (M <= 0 || N <= 0) return;
   if (!transa && K > 0 && K <= 32 && N <= 256) {
     // inner dimension of a few bands: a streaming kernel (the matrix-core tiles would be nearly empty)
     dim3 grid(ceil_div(M, NTS), N);
-    if (K <= 8) k_ket_skinny<8><<<grid, NTS, 0, s>>>(M, K, alpha, A, lda, B, ldb, beta, C, ldc);
-    else if (K <= 16) k_ket_skinny<16><<<grid, NTS, 0, s>>>(M, K, alpha, A, lda, B, ldb, beta, C, ldc);
-    else k_ket_skinny<32><<<grid, NTS, 0, s>>>(M, K, alpha, A, lda, B, ldb, beta, C, ldc);
+    if (K <= 8) k_ket_skinny<8><<<grid, NTS, 0, s>>>(M, K, alpha, A, lda, B, ldb, beta, C, ldc, d_n);
+    else if (K <= 16) k_ket_skinny<16><<<grid, NTS, 0, s>>>(M, K, alpha, A, lda, B, ldb, beta, C, ldc, d_n);
+    else k_ket_skinny<32><<<grid, NTS, 0, s>>>(M, K, alpha, A, lda, B, ldb, beta, C, ldc, d_n);
     WH_HIP(hipGetLastError());
     return;
   }
   const int K0 = std::max(K, 0), kc = std::max(K, 1);
   auto run = [&](int v, double* Cv) {
-    if (transa) launch_wn<true>(v, M, N, K0, kc, 1, alpha, A, lda, B, ldb, beta, Cv, ldc, 0, s);
-    else launch_wn<false>(v, M, N, K0, kc, 1, alpha, A, lda, B, ldb, beta, Cv, ldc, 0, s);
+    if (transa) launch_wn<true>(v, M, N, K0, kc, 1, alpha, A, lda, B, ldb, beta, Cv, ldc, 0, s, d_n);
+    else launch_wn<false>(v, M, N, K0, kc, 1, alpha, A, lda, B, ldb, beta, Cv, ldc, 0, s, d_n);
   };
   // tile width, timed once per (op, M, log2 K, log2 N)
   const VariantKey key{0, transa ? 1 : 0, M, log2_bucket(std::max(K, 1)), log2_bucket(N)};
@@ -477,14 +492,15 @@ static void bra_split(int M, int N, int K, int v, int& split, int& kchunk) {
 // P: room for the split-K partial sums (null: the per-thread scratch)
 static void dgemm_bra_own(int M, int N, int K, double alpha, const double* A, long lda, const double* B, long ldb,
                           double g0, const double* scale, double* C, long ldc, hipStream_t s, int v = 0,
-                          double* P = nullptr) {
+                          double* P = nullptr, const int* d_n = nullptr) {
   int split = 1, kchunk = 1;
   bra_split(M, N, K, v, split, kchunk);
   const long pstride = static_cast<long>(ldc) * N;
   if (!P) P = partials().get(static_cast<size_t>(split) * pstride, s);
-  launch_wn<true>(v % kWnVariants, M, N, K, kchunk, split, 1.0, A, lda, B, ldb, 0.0, P, ldc, pstride, s);
+  launch_wn<true>(v % kWnVariants, M, N, K, kchunk, split, 1.0, A, lda, B, ldb, 0.0, P, ldc, pstride, s, d_n);
   const long n = static_cast<long>(M) * N;
-  k_reduce_g0<<<ceil_div(n, 256), 256, 0, s>>>(M, N, split, P, pstride, alpha, g0, A, lda, B, ldb, scale, C, ldc);
+  k_reduce_g0<<<ceil_div(n, 256), 256, 0, s>>>(M, N, split, P, pstride, alpha, g0, A, lda, B, ldb, scale, C, ldc,
+                                               d_n);
   WH_HIP(hipGetLastError());
 }
 
@@ -496,32 +512,35 @@ static void skinny_split(int N, int K, int& split, int& kchunk) {
 }
 
 static void dgemm_bra_skinny(int M, int N, int K, double alpha, const double* A, long lda, const double* B, long ldb,
-                             double g0, const double* scale, double* C, long ldc, hipStream_t s, double* P = nullptr) {
+                             double g0, const double* scale, double* C, long ldc, hipStream_t s, double* P = nullptr,
+                             const int* d_n = nullptr) {
   int split = 1, kchunk = 1;
   skinny_split(N, K, split, kchunk);
   const long pstride = static_cast<long>(ldc) * N;
   if (!P) P = partials().get(static_cast<size_t>(split) * pstride, s);
   dim3 grid(N, split);
-  if (M <= 8) k_bra_skinny<8><<<grid, NTS, 0, s>>>(M, K, kchunk, A, lda, B, ldb, P, ldc, pstride);
-  else if (M <= 16) k_bra_skinny<16><<<grid, NTS, 0, s>>>(M, K, kchunk, A, lda, B, ldb, P, ldc, pstride);
-  else k_bra_skinny<32><<<grid, NTS, 0, s>>>(M, K, kchunk, A, lda, B, ldb, P, ldc, pstride);
+  if (M <= 8) k_bra_skinny<8><<<grid, NTS, 0, s>>>(M, K, kchunk, A, lda, B, ldb, P, ldc, pstride, d_n);
+  else if (M <= 16) k_bra_skinny<16><<<grid, NTS, 0, s>>>(M, K, kchunk, A, lda, B, ldb, P, ldc, pstride, d_n);
+  else k_bra_skinny<32><<<grid, NTS, 0, s>>>(M, K, kchunk, A, lda, B, ldb, P, ldc, pstride, d_n);
   WH_HIP(hipGetLastError());
   const long n = static_cast<long>(M) * N;
-  k_reduce_g0<<<ceil_div(n, 256), 256, 0, s>>>(M, N, split, P, pstride, alpha, g0, A, lda, B, ldb, scale, C, ldc);
+  k_reduce_g0<<<ceil_div(n, 256), 256, 0, s>>>(M, N, split, P, pstride, alpha, g0, A, lda, B, ldb, scale, C, ldc,
+                                               d_n);
   WH_HIP(hipGetLastError());
 }
 
 void dgemm_bra(int M, int N, int K, double alpha, const double* A, long lda, const double* B, long ldb, double g0,
-               const double* scale, double* C, long ldc, hipStream_t s) {
+               const double* scale, double* C, long ldc, hipStream_t s, const int* d_n) {
   if (M <= 0 || N <= 0) return;
   // a few rows and columns: the matrix-core tiles (128 x 128) would be almost
   // empty (with many columns the tiles' reuse of A wins again)
-  if (M <= 32 && N <= 256 && K > 0) return dgemm_bra_skinny(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s);
-  if (K <= 0 || N < 32) return dgemm_bra_own(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s);
+  if (M <= 32 && N <= 256 && K > 0)
+    return dgemm_bra_skinny(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s, nullptr, d_n);
+  if (K <= 0 || N < 32) return dgemm_bra_own(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s, 0, nullptr, d_n);
   // variants: K split depth (3) x tile width (3), timed once per (M, K, log2 N);
   // C is written (not accumulated), so the timing runs on the call's own output
   constexpr int nv = 3 * kWnVariants;
-  auto run = [&](int v) { dgemm_bra_own(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s, v); };
+  auto run = [&](int v) { dgemm_bra_own(M, N, K, alpha, A, lda, B, ldb, g0, scale, C, ldc, s, v, nullptr, d_n); };
   const VariantKey key{1, 1, M, K, log2_bucket(N)};
   std::unique_lock<std::mutex> lk(g_variant_mutex);
   auto it = chosen_variants().find(key);

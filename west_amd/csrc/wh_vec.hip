@@ -39,9 +39,10 @@ __device__ __forceinline__ double gamma_dot_part(const z64* a, const z64* b, int
 template <typename V>
 __global__ __launch_bounds__(NT) void k_prec_rho(int npw, int gstart, const int* l2i, const cplx<V>* g, long ld,
                                                  const double* g2kin, const double* eprec, double* rho,
-                                                 bool by_band) {
+                                                 bool by_band, const int* d_nc = nullptr) {
   __shared__ double red[NT / 64];
   const int lb = blockIdx.x;
+  if (d_nc && lb >= *d_nc) return;   // uniform per workgroup
   const int ib = l2i[lb];
   const cplx<V>* gi = g + static_cast<long>(ib) * ld;
   const double ep = eprec[ib];
@@ -62,8 +63,10 @@ __global__ __launch_bounds__(NT) void k_prec_rho(int npw, int gstart, const int*
 template <typename V>
 __global__ __launch_bounds__(NT) void k_direction(int npw, const int* l2i, const double* beta, const cplx<V>* g,
                                                   const double* g2kin, const double* eprec, cplx<V>* h,
-                                                  const cplx<V>* hold, cplx<V>* bc, long ld, long ldc) {
+                                                  const cplx<V>* hold, cplx<V>* bc, long ld, long ldc,
+                                                  const int* d_nc) {
   const int lb = blockIdx.y;
+  if (d_nc && lb >= *d_nc) return;
   const int ib = l2i[lb];
   const int q = blockIdx.x * NT + threadIdx.x;
   if (q >= npw) return;
@@ -83,9 +86,11 @@ __global__ __launch_bounds__(NT) void k_direction(int npw, const int* l2i, const
 // a = <h|g>, c = <h|t>; x -= (a/c) h; g -= (a/c) t   (linsolve 253-321)
 template <typename V>
 __global__ __launch_bounds__(NT) void k_cg_update(int npw, int gstart, const int* l2i, const cplx<V>* h, cplx<V>* g,
-                                                  const cplx<V>* tc, cplx<V>* x, long ld, long ldc, double* ac_out) {
+                                                  const cplx<V>* tc, cplx<V>* x, long ld, long ldc, double* ac_out,
+                                                  const int* d_nc) {
   __shared__ double red[NT / 64];
   const int lb = blockIdx.x;
+  if (d_nc && lb >= *d_nc) return;
   const int ib = l2i[lb];
   const cplx<V>* hi = h + static_cast<long>(ib) * ld;
   cplx<V>* gi = g + static_cast<long>(ib) * ld;
@@ -131,9 +136,11 @@ constexpr int NTU = 1024;
 template <typename V, int E, bool CG_, bool RHO>
 __global__ __launch_bounds__(NTU) void k_cg_update_reg(int npw, int gstart, const int* l2i, const cplx<V>* h,
                                                       cplx<V>* g, const cplx<V>* tc, cplx<V>* x, long ld, long ldc,
-                                                      const double* g2kin, const double* eprec, double* rho) {
+                                                      const double* g2kin, const double* eprec, double* rho,
+                                                      const int* d_nc) {
   __shared__ double red[NTU / 64];
   const int lb = blockIdx.x;
+  if (d_nc && lb >= *d_nc) return;
   const int ib = l2i[lb];
   const cplx<V>* hi = h + static_cast<long>(ib) * ld;
   cplx<V>* gi = g + static_cast<long>(ib) * ld;
@@ -331,6 +338,104 @@ __global__ void k_fill_pairs(FftItem* items, int n, int m, int row0) {
   items[q] = it;
 }
 
+__global__ void k_fill_pairs_dev(FftItem* items, int cap, const int* nc) {
+  const int q = blockIdx.x * NT + threadIdx.x;
+  if (q >= cap) return;
+  const int m = *nc;
+  FftItem it;
+  it.a = (2 * q < m) ? 2 * q : -1;
+  it.b = (2 * q + 1 < m) ? 2 * q + 1 : -1;
+  it.pot = -1;
+  it.pad = 0;
+  items[q] = it;
+}
+
+__global__ void k_fill_ident_dev(FftItem* items, int cap, int chunk, const int* nc) {
+  const int q = blockIdx.x * NT + threadIdx.x;
+  if (q >= cap) return;
+  const int npair = (*nc + 1) / 2;
+  FftItem it;
+  it.a = q < npair ? q % chunk : -1;
+  it.b = -1;
+  it.pot = -1;
+  it.pad = 0;
+  items[q] = it;
+}
+
+__global__ void k_iota(int* v, int n) {
+  const int q = blockIdx.x * NT + threadIdx.x;
+  if (q < n) v[q] = q;
+}
+
+// Convergence test, compaction and coefficients of one CG iteration on the
+// device (linsolve_sternheimer_m_wfcts.f90:190-240): one workgroup walks the
+// bands in order, so the active list keeps the ascending band order of
+// map_lbnd2ibnd.
+constexpr int NTS_SEL = 1024;
+__global__ __launch_bounds__(NTS_SEL) void k_cg_select(int m, double tr2, int first, int group,
+                                                       const double* __restrict__ rhob, double* __restrict__ rhoold,
+                                                       int* __restrict__ conv, const double* __restrict__ e_col,
+                                                       int* __restrict__ l2i, double* __restrict__ beta,
+                                                       double* __restrict__ eu, int* __restrict__ nc_out,
+                                                       int* __restrict__ rec) {
+  __shared__ int wcount[NTS_SEL / 64];
+  __shared__ int base_s;
+  __shared__ int calls_w[NTS_SEL / 64];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int w = tid >> 6;
+  if (tid == 0) base_s = 0;
+  __syncthreads();
+  for (int c0 = 0; c0 < m; c0 += NTS_SEL) {
+    const int ib = c0 + tid;
+    bool act = false;
+    double r = 0.0;
+    if (ib < m && !conv[ib]) {
+      r = rhob[ib];
+      if (r < tr2) conv[ib] = 1;
+      else act = true;
+    }
+    const unsigned long long mask = __ballot(act);
+    const int before = __popcll(mask & ((1ull << lane) - 1ull));
+    if (lane == 0) wcount[w] = __popcll(mask);
+    __syncthreads();
+    int off = base_s;
+    for (int k = 0; k < w; ++k) off += wcount[k];
+    if (act) {
+      const int pos = off + before;
+      l2i[pos] = ib;
+      beta[pos] = first ? 0.0 : r / rhoold[ib];
+      rhoold[ib] = r;
+      eu[pos] = e_col[ib];
+    }
+    __syncthreads();
+    if (tid == 0) {
+      int tot = 0;
+      for (int k = 0; k < NTS_SEL / 64; ++k) tot += wcount[k];
+      base_s += tot;
+    }
+    __syncthreads();
+  }
+  const int nc = base_s;
+  // perturbations among the active bands (the list is ascending)
+  int calls = 0;
+  for (int pos = tid; pos < nc; pos += NTS_SEL) {
+    const int gp = group > 0 ? l2i[pos] / group : 0;
+    const int gq = pos > 0 ? (group > 0 ? l2i[pos - 1] / group : 0) : -1;
+    calls += gp != gq;
+  }
+  for (int o = 32; o > 0; o >>= 1) calls += __shfl_xor(calls, o, 64);
+  if (lane == 0) calls_w[w] = calls;
+  __syncthreads();
+  if (tid == 0) {
+    int tot = 0;
+    for (int k = 0; k < NTS_SEL / 64; ++k) tot += calls_w[k];
+    *nc_out = nc;
+    rec[0] = nc;
+    rec[1] = tot;
+  }
+}
+
 __global__ void k_fill_ident(FftItem* items, int n) {
   const int q = blockIdx.x * NT + threadIdx.x;
   if (q >= n) return;
@@ -370,10 +475,10 @@ void prec_rho(int nc, int npw, int gstart, const int* l2i, const z32* g, long ld
 }
 
 void direction(int nc, int npw, const int* l2i, const double* beta, const z64* g, const double* g2kin,
-               const double* eprec, z64* h, const z64* hold, z64* bc, long ld, hipStream_t s) {
+               const double* eprec, z64* h, const z64* hold, z64* bc, long ld, hipStream_t s, const int* d_nc) {
   if (nc <= 0) return;
   k_direction<double><<<dim3(ceil_div(npw, NT), nc), NT, 0, s>>>(npw, l2i, beta, g, g2kin, eprec, h, hold, bc, ld,
-                                                                  ld);
+                                                                  ld, d_nc);
   WH_HIP(hipGetLastError());
 }
 
@@ -381,44 +486,45 @@ void direction(int nc, int npw, const int* l2i, const double* beta, const z32* g
                const double* eprec, z32* h, const z32* hold, z32* bc, long ld, hipStream_t s) {
   if (nc <= 0) return;
   k_direction<float><<<dim3(ceil_div(npw, NT), nc), NT, 0, s>>>(npw, l2i, beta, g, g2kin, eprec, h, hold, bc, ld,
-                                                                 ld);
+                                                                 ld, nullptr);
   WH_HIP(hipGetLastError());
 }
 
 template <typename V, bool RHO>
 static void cg_update_reg(int nc, int npw, int gstart, const int* l2i, const cplx<V>* h, cplx<V>* g,
-                          const cplx<V>* tc, cplx<V>* x, long ld, const RhoOut& r, hipStream_t s) {
+                          const cplx<V>* tc, cplx<V>* x, long ld, const RhoOut& r, hipStream_t s, const int* d_nc) {
   // FP64 at 9 elements per thread keeps h and t only (g too would spill)
   if (npw <= 4 * NTU)
     k_cg_update_reg<V, 4, true, RHO><<<nc, NTU, 0, s>>>(npw, gstart, l2i, h, g, tc, x, ld, ld, r.g2kin, r.eprec,
-                                                         r.rho);
+                                                         r.rho, d_nc);
   else
     k_cg_update_reg<V, 9, (sizeof(V) == 4), RHO><<<nc, NTU, 0, s>>>(npw, gstart, l2i, h, g, tc, x, ld, ld, r.g2kin,
-                                                                    r.eprec, r.rho);
+                                                                    r.eprec, r.rho, d_nc);
 }
 
 template <typename V>
 static void cg_update_any(int nc, int npw, int gstart, const int* l2i, const cplx<V>* h, cplx<V>* g,
-                          const cplx<V>* tc, cplx<V>* x, long ld, double* ac_out, const RhoOut* r, hipStream_t s) {
+                          const cplx<V>* tc, cplx<V>* x, long ld, double* ac_out, const RhoOut* r, hipStream_t s,
+                          const int* d_nc) {
   if (nc <= 0) return;
   if (!ac_out && npw <= 9 * NTU) {
-    if (r) cg_update_reg<V, true>(nc, npw, gstart, l2i, h, g, tc, x, ld, *r, s);
-    else cg_update_reg<V, false>(nc, npw, gstart, l2i, h, g, tc, x, ld, RhoOut{}, s);
+    if (r) cg_update_reg<V, true>(nc, npw, gstart, l2i, h, g, tc, x, ld, *r, s, d_nc);
+    else cg_update_reg<V, false>(nc, npw, gstart, l2i, h, g, tc, x, ld, RhoOut{}, s, d_nc);
   } else {
-    k_cg_update<V><<<nc, NT, 0, s>>>(npw, gstart, l2i, h, g, tc, x, ld, ld, ac_out);
-    if (r) k_prec_rho<V><<<nc, NT, 0, s>>>(npw, gstart, l2i, g, ld, r->g2kin, r->eprec, r->rho, true);
+    k_cg_update<V><<<nc, NT, 0, s>>>(npw, gstart, l2i, h, g, tc, x, ld, ld, ac_out, d_nc);
+    if (r) k_prec_rho<V><<<nc, NT, 0, s>>>(npw, gstart, l2i, g, ld, r->g2kin, r->eprec, r->rho, true, d_nc);
   }
   WH_HIP(hipGetLastError());
 }
 
 void cg_update(int nc, int npw, int gstart, const int* l2i, const z64* h, z64* g, const z64* tc, z64* x, long ld,
-               double* ac_out, hipStream_t s, const RhoOut* rho) {
-  cg_update_any<double>(nc, npw, gstart, l2i, h, g, tc, x, ld, ac_out, rho, s);
+               double* ac_out, hipStream_t s, const RhoOut* rho, const int* d_nc) {
+  cg_update_any<double>(nc, npw, gstart, l2i, h, g, tc, x, ld, ac_out, rho, s, d_nc);
 }
 
 void cg_update(int nc, int npw, int gstart, const int* l2i, const z32* h, z32* g, const z32* tc, z32* x, long ld,
                double* ac_out, hipStream_t s, const RhoOut* rho) {
-  cg_update_any<float>(nc, npw, gstart, l2i, h, g, tc, x, ld, ac_out, rho, s);
+  cg_update_any<float>(nc, npw, gstart, l2i, h, g, tc, x, ld, ac_out, rho, s, nullptr);
 }
 
 void gather_sticks(int m, int npw, const int* perm, const z64* in, long ldin, z64* out, long ldout, const double* w,
@@ -510,6 +616,30 @@ void fill_pairs(FftItem* items, int m, int row0, hipStream_t s) {
   const int n = (m + 1) / 2;
   if (n <= 0) return;
   k_fill_pairs<<<ceil_div(n, NT), NT, 0, s>>>(items, n, m, row0);
+  WH_HIP(hipGetLastError());
+}
+
+void fill_pairs_dev(FftItem* items, int cap, const int* nc, hipStream_t s) {
+  if (cap <= 0) return;
+  k_fill_pairs_dev<<<ceil_div(cap, NT), NT, 0, s>>>(items, cap, nc);
+  WH_HIP(hipGetLastError());
+}
+
+void fill_ident_dev(FftItem* items, int cap, int chunk, const int* nc, hipStream_t s) {
+  if (cap <= 0) return;
+  k_fill_ident_dev<<<ceil_div(cap, NT), NT, 0, s>>>(items, cap, chunk, nc);
+  WH_HIP(hipGetLastError());
+}
+
+void iota(int* v, int n, hipStream_t s) {
+  if (n <= 0) return;
+  k_iota<<<ceil_div(n, NT), NT, 0, s>>>(v, n);
+  WH_HIP(hipGetLastError());
+}
+
+void cg_select(int m, double tr2, int first, int group, const double* rhob, double* rhoold, int* conv,
+               const double* e_col, int* l2i, double* beta, double* eu, int* nc, int* rec, hipStream_t s) {
+  k_cg_select<<<1, NTS_SEL, 0, s>>>(m, tr2, first, group, rhob, rhoold, conv, e_col, l2i, beta, eu, nc, rec);
   WH_HIP(hipGetLastError());
 }
 
