@@ -206,6 +206,49 @@ int wh_solve_deflated_lanczos_w_full_ortho(int nbnd_to_deflate, int nrhs, int nl
  * (Wfreq/get_brak_hyper_parallel.f90:14), on the matrix cores. */
 int wh_glbrak_gamma(int na, const double* a, int nb, const double* b, double* c);
 
+/* ---- q points: Wstat away from Gamma ------------------------------- */
+
+/* Ground state on a k grid (the gamma_only = .FALSE. inputs of dfpt,
+ * DFPT_kernel/dfpt_module.f90:163-209: QE's wk, npwx, the k-point states and
+ * projectors, nbnd_occ, alphapv_dfpt; the q grid of Tools/class_bz_grid.f90:71-131
+ * and igq_q of Tools/set_npwq.f90:50-85).  After wh_set_kgrid the system's
+ * operator, CG and chi0 entry points (wh_apply_sternheimerop_to_m_wfcs,
+ * wh_linsolve_sternheimer_m_wfcts, wh_dfpt, wh_apply_operator) act at k and q
+ * points: vectors are (npwx, m) on the |k+G| sphere of the current k point
+ * (wh_set_current_k) or (npwqx, m) on the |q+G| sphere of q point iq; iq of
+ * wh_dfpt / wh_apply_operator selects the q point (1..nqs).  wh_set_grid
+ * returns the system to Gamma.
+ *   wh_set_kgrid: FFT grid, cell volume, vrs (nr1*nr2*nr3, first index
+ *     fastest), nks k points with weights wk (sum 2 for nspin = 1), row length
+ *     npwx of k-point vectors, nqs q points with row length npwqx, occupied
+ *     bands, nkb projectors with D = dvan (Ry), alpha_pv.
+ *   wh_set_kpoint(ik): npw plane waves k+G with 1-based grid index nl
+ *     (dffts%nl(igk_k)), g2kin = |k+G|^2 (Ry), evc (npwx, nbndval) and et
+ *     (nbndval) at k, vkb (npwx, nkb) at k (init_us_2).
+ *   wh_set_qpoint(iq): npwq vectors q+G (nl, |q+G|^2 in Ry, 1-based global G
+ *     index ig_global, the counter of the random start vectors), sqvc (npwq,
+ *     may be NULL: then only wh_dfpt), and for each k the 1-based index ikq
+ *     of k - q on the k grid with g0 (3, nks) integer crystal components,
+ *     k - q = k_ikq + g0 (findp, class_bz_grid.f90:166-214). */
+int wh_set_kgrid(int nr1, int nr2, int nr3, double omega, const double* vrs, int nks, const double* wk, int npwx,
+                 int nqs, int npwqx, int nbndval, int nkb, const double* dvan, double alpha_pv);
+int wh_set_kpoint(int ik, int npw, const int* nl, const double* g2kin, const double* evc, const double* et,
+                  const double* vkb);
+int wh_set_qpoint(int iq, int npwq, const int* nl, const double* gq2kin, const int* ig_global, const double* sqvc,
+                  const int* ikq, const int* g0);
+/* k point (1..nks) of the operator and CG entry points: QE's current_k,
+ * set by dfpt before h_psi (dfpt_module.f90:149). */
+int wh_set_current_k(int ik);
+
+/* One q point of the Davidson at q points (the QPOINTS_LOOP body of
+ * davidson_diago_k, Wstat/davidson_diago.f90:551-894): start vectors of
+ * do_randomize_q (1186-1250, the counter generator of wh_davidson_diago drawn
+ * per global G, divided by |q+G|^2 + 1 in Ry), complex Gram-Schmidt and
+ * subspace products, rocSOLVER zheevd.  Arguments as wh_davidson_diago;
+ * dvg_out is (npwqx, n_pdep_eigen) on the q sphere.  No restart files. */
+int wh_davidson_diago_k(int iq, const wh_davidson_params* params, double* ev, int* conv, double* dvg_out,
+                        wh_davidson_step* log, int log_capacity, int* log_len, double* ev_log, int* conv_log);
+
 /* ---- allocation-free operator -------------------------------------- */
 
 /* wh_apply_sternheimerop_to_m_wfcs with no device allocation (and no variant

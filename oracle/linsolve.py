@@ -7,9 +7,13 @@ import numpy as np
 from .hamiltonian import apply_sternheimerop
 
 
-def _gamma_dot(a, b, n, gstart):
-    """Per-column 2 Re<a|b> - Re a(1) Re b(1)  (``linsolve...f90:162-181, 253-276``)."""
-    r = 2.0 * np.sum(a[:, :n].real * b[:, :n].real + a[:, :n].imag * b[:, :n].imag, axis=1)
+def _gamma_dot(a, b, n, gstart, gamma=True):
+    """Per-column 2 Re<a|b> - Re a(1) Re b(1)  (``linsolve...f90:162-181, 253-276``);
+    away from Gamma the plain Re<a|b> over the full sphere (183-205, 277-305)."""
+    r = np.sum(a[:, :n].real * b[:, :n].real + a[:, :n].imag * b[:, :n].imag, axis=1)
+    if not gamma:
+        return r
+    r = 2.0 * r
     if gstart == 2:
         r -= a[:, 0].real * b[:, 0].real
     return r
@@ -21,10 +25,17 @@ def linsolve_sternheimer_m_wfcts(gs, nbndval, b, x, e, eprec, tr2, n_dfpt_maxite
     unconverged columns.  ``stats['nstern']`` counts operator applications
     (columns), ``stats['ncalls']`` calls of ``apply_sternheimerop`` (the
     reference's 'stern' clock, lines 127 and 251), ``stats['iters']``
-    iterations."""
+    iterations.  A ``gs`` with ``gamma = False`` (one k point) supplies its
+    operator as ``gs.sternheimer(psi, e, alpha)`` and uses full-sphere
+    products."""
     bas = gs.basis
     n = bas.npw
     gstart = bas.gstart
+    gamma = getattr(gs, "gamma", True)
+    op = getattr(gs, "sternheimer", None) if not gamma else None
+    if op is None:
+        def op(psi, ee, al):
+            return apply_sternheimerop(gs, psi, ee, al)
     m = b.shape[0]
     e = np.asarray(e, dtype=np.float64)
     eprec = np.asarray(eprec, dtype=np.float64)
@@ -38,7 +49,7 @@ def linsolve_sternheimer_m_wfcts(gs, nbndval, b, x, e, eprec, tr2, n_dfpt_maxite
     nstern = 0
     ncalls = 1
     # Step 1 (127-138): g = A x - b
-    g[:] = apply_sternheimerop(gs, x, e, alpha)
+    g[:] = op(x, e, alpha)
     nstern += m
     g[:, :n] -= b[:, :n]
     l2i = np.flatnonzero(~is_conv)                     # map_lbnd2ibnd (356-382)
@@ -48,7 +59,7 @@ def linsolve_sternheimer_m_wfcts(gs, nbndval, b, x, e, eprec, tr2, n_dfpt_maxite
         nc = l2i.size
         # preconditioning (150-160), only ig <= npw
         h[l2i, :n] = g[l2i, :n] / np.maximum(1.0, bas.g2kin[None, :n] / eprec[l2i, None])
-        rho_c = _gamma_dot(h[l2i], g[l2i], n, gstart)   # (162-181), compacted
+        rho_c = _gamma_dot(h[l2i], g[l2i], n, gstart, gamma)   # (162-181), compacted
         # un-compact rho and test convergence (208-218)
         rho[l2i] = rho_c
         is_conv[l2i] = rho[l2i] < tr2
@@ -64,11 +75,11 @@ def linsolve_sternheimer_m_wfcts(gs, nbndval, b, x, e, eprec, tr2, n_dfpt_maxite
         b[:nc] = h[l2i]
         eu = e[l2i]
         # t = A h on the compacted columns (251)
-        t[:nc] = apply_sternheimerop(gs, b[:nc], eu, alpha)
+        t[:nc] = op(b[:nc], eu, alpha)
         nstern += nc
         ncalls += 1
-        a_c = _gamma_dot(h[l2i], g[l2i], n, gstart)     # (253-276)
-        c_c = _gamma_dot(h[l2i], t[:nc], n, gstart)
+        a_c = _gamma_dot(h[l2i], g[l2i], n, gstart, gamma)     # (253-276)
+        c_c = _gamma_dot(h[l2i], t[:nc], n, gstart, gamma)
         step = (a_c / c_c)[:, None]
         x[l2i] -= step * h[l2i]                         # (313-321)
         g[l2i] -= step * t[:nc]

@@ -84,6 +84,8 @@ static void free_context(Context* cp) {
                      &c.hm_u, &c.h_cgpk, &c.h_cgrec})
     b->release();
   for (DevBuf& b : c.dav) b.release();
+  free_kpath(c.kpath);
+  c.kpath = nullptr;
   (void)hipStreamDestroy(c.stream);
   delete cp;
 }
@@ -1016,6 +1018,9 @@ int wh_set_grid(int nr1, int nr2, int nr3, int npw, int npwx, int gstart, const 
   WH_HIP(hipMemcpy(c.d_perm, c.plan.perm.data(), sizeof(int) * npw, hipMemcpyHostToDevice));
   WH_HIP(hipMemcpy(c.d_g2kin, c.g2kin_s.data(), sizeof(double) * npw, hipMemcpyHostToDevice));
   c.have_grid = true;
+  // a Gamma grid replaces a k grid registered before in this system
+  free_kpath(c.kpath);
+  c.kpath = nullptr;
   c.mixed_valid = false;
   c.have_pot = false;
   c.have_states = false;
@@ -1139,6 +1144,7 @@ int wh_set_davidson_restart(const char* wstat_restart_dir, int n_steps_write_res
 int wh_apply_sternheimerop_to_m_wfcs(int nbndval, const double* psi, double* hpsi, const double* e,
                                      const double* alpha, int m) {
   WH_API_BEGIN
+  if (kpath_on()) return kpath_apply_sternheimerop(nbndval, psi, hpsi, e, alpha, m);
   Context& c = ctx();
   // without registered states (nbndval = 0) this is H - e, used to find the states themselves
   require_ready(false);
@@ -1179,6 +1185,7 @@ int wh_apply_sternheimerop_to_m_wfcs(int nbndval, const double* psi, double* hps
 int wh_linsolve_sternheimer_m_wfcts(int nbndval, int m, double* b, double* x, const double* e, const double* eprec,
                                     double tr2, int* ierr) {
   WH_API_BEGIN
+  if (kpath_on()) return kpath_linsolve(nbndval, m, b, x, e, eprec, tr2, ierr);
   Context& c = ctx();
   require_ready(true);
   WH_REQUIRE(nbndval == c.nbndval, "nbndval differs from the registered states");
@@ -1213,9 +1220,10 @@ int wh_linsolve_sternheimer_m_wfcts(int nbndval, int m, double* b, double* x, co
 }
 
 static int dfpt_entry(int m, const double* dvg, double* dng, double tr2, int iq, int* ierr_pert, bool coulomb) {
+  if (kpath_on()) return kpath_dfpt(m, dvg, dng, tr2, iq, ierr_pert, coulomb);
   Context& c = ctx();
   require_ready(true);
-  WH_REQUIRE(iq == 1, "only the Gamma point (iq = 1) is supported");
+  WH_REQUIRE(iq == 1, "at Gamma iq must be 1 (q points need a k grid: wh_set_kgrid)");
   WH_REQUIRE(m >= 0, "m must be >= 0");
   if (coulomb) WH_REQUIRE(c.have_sqvc, "wh_set_coulomb has not been called");
   hipStream_t s = c.stream;

@@ -92,4 +92,21 @@ inline bool tune_log() {
 
 inline int ceil_div(long a, long b) { return static_cast<int>((a + b - 1) / b); }
 
+// Start vectors of the Davidson drivers: SplitMix64's finaliser, and the two
+// uniforms in [0, 1) of basis vector ivec (1-based) at plane wave ig (1-based
+// counter): u_k = mix64(seed ^ (ivec << 32) ^ (2 ig + k)) / 2^64, 53 bits.
+constexpr unsigned long long kRngSeed = 0x5EED0F1E57ULL;
+__device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
+  x += 0x9E3779B97F4A7C15ULL;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+  return x ^ (x >> 31);
+}
+__device__ __forceinline__ void start_uniforms(unsigned long long ivec, unsigned long long ig, double& u0,
+                                               double& u1) {
+  const unsigned long long base = kRngSeed ^ (ivec << 32);
+  u0 = static_cast<double>(mix64(base ^ (2ULL * ig)) >> 11) * (1.0 / 9007199254740992.0);
+  u1 = static_cast<double>(mix64(base ^ (2ULL * ig + 1ULL)) >> 11) * (1.0 / 9007199254740992.0);
+}
+
 }  // namespace wh

@@ -84,6 +84,8 @@ struct Profiler {
   void clear();
 };
 
+struct KPath;   // the q/k-point path's state (wh_kpoint.hip)
+
 struct Context {
   int device = -1;
   hipStream_t stream = nullptr;
@@ -156,12 +158,24 @@ struct Context {
   long nfallback = 0;  // bands the mixed solve handed to the FP64 CG
   long cg_iters = 0;
   Profiler prof;
+  // k-point ground state (wh_set_kgrid): when present, the operator, CG and
+  // chi0 entry points act at k points / q points instead of Gamma
+  KPath* kpath = nullptr;
 
   long sticks_bytes() const { return plan.sticks_elems() * static_cast<long>(sizeof(z64)); }
   int items_capacity() const;   // transform items per launch chunk
 };
 
 Context& ctx();
+// the q/k-point path (wh_kpoint.hip): state release, and the C-ABI hot-path
+// entries when a k grid is registered in the current system
+void free_kpath(KPath* k);
+bool kpath_on();
+int kpath_apply_sternheimerop(int nbndval, const double* psi, double* hpsi, const double* e, const double* alpha,
+                              int m);
+int kpath_linsolve(int nbndval, int m, double* b, double* x, const double* e, const double* eprec, double tr2,
+                   int* ierr);
+int kpath_dfpt(int m, const double* dvg, double* dng, double tr2, int iq, int* ierr_pert, bool coulomb);
 // takes this device's address of the allocation-free operator's scratch array (wh_noalloc.hip)
 void noalloc_init(int device);
 void require_ready(bool need_states);

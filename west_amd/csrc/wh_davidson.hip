@@ -39,15 +39,6 @@ namespace wh {
 namespace {
 
 constexpr int NT = 256;
-constexpr unsigned long long kRngSeed = 0x5EED0F1E57ULL;
-
-__device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
-  x += 0x9E3779B97F4A7C15ULL;
-  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
-  x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
-  return x ^ (x >> 31);
-}
-
 // do_randomize (davidson_diago.f90:1156-1177): vector ivec (1-based), plane
 // wave ig (1-based, caller order): amplitude u0, phase 2 pi u1, divided by
 // |G|^2 + 1 in (2 pi / alat)^2 units; G = 0 left at zero.
@@ -58,9 +49,8 @@ __global__ void k_randomize(int npw, int ivec0, int nvec, const int* perm, const
   if (s >= npw || r >= nvec) return;
   const unsigned long long ivec = static_cast<unsigned long long>(ivec0 + r);
   const unsigned long long ig = static_cast<unsigned long long>(perm[s]) + 1ULL;
-  const unsigned long long base = kRngSeed ^ (ivec << 32);
-  const double u0 = static_cast<double>(mix64(base ^ (2ULL * ig)) >> 11) * (1.0 / 9007199254740992.0);
-  const double u1 = static_cast<double>(mix64(base ^ (2ULL * ig + 1ULL)) >> 11) * (1.0 / 9007199254740992.0);
+  double u0, u1;
+  start_uniforms(ivec, ig, u0, u1);
   z64 v = mk(0.0, 0.0);
   if (s != 0) {
     const double arg = 2.0 * 3.14159265358979323846 * u1;

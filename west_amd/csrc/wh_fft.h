@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <vector>
 
 #include "wh_common.h"
 
@@ -139,6 +140,22 @@ struct ZfwdArgs {
   const double* weight;   // per-G weight (SINGLE), may be null
   int zero_g0;
 };
+
+// Full complex grids of the k-point path (QE layout, first index fastest):
+// dir = +1 is QE's invfft (unnormalised exp(+iGr)), dir = -1 the forward sum
+// without the 1/N (the caller scales).  A batch is `batch` grids back to back.
+struct GridFft {
+  int n1 = 0, n2 = 0, n3 = 0;
+  FftAxis ax1{}, ax2{}, ax3{};
+  const double* tw1 = nullptr;
+  const double* tw2 = nullptr;
+  const double* tw3 = nullptr;
+  std::vector<void*> allocations;
+  void build(int nr1, int nr2, int nr3);
+  void release();
+  long nnr() const { return static_cast<long>(n1) * n2 * n3; }
+};
+void launch_grid_fft(const GridFft& g, z64* data, int batch, int dir, hipStream_t s);
 
 // kernel launchers (double precision)
 void launch_zinv(const StickPlan& p, const FftItem* d_items, int nitems, const z64* psi, long ldpsi,

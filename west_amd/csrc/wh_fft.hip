@@ -211,6 +211,59 @@ __global__ __launch_bounds__(kNT) void k_zinv(StickPlanDev P, const FftItem* ite
 }
 
 // --------------------------------------------------------------------------
+// full complex grids (k points): batches of grids in QE layout (first index
+// fastest).  A workgroup transforms a block of lines in LDS; along the
+// second and third axes the block is nrun consecutive positions of the
+// first index, so that the loads and stores are contiguous runs.
+// --------------------------------------------------------------------------
+constexpr int kGridE = 16;   // lines x length per workgroup <= kGridE * kNT
+
+// lines along the first axis: contiguous, lpb lines per workgroup
+__global__ __launch_bounds__(kNT) void k_grid_fft_x(z64* data, long nlines, int n, FftAxis ax, const double* tws,
+                                                    int dir, int lpb) {
+  extern __shared__ char smem_raw[];
+  z64* tw = reinterpret_cast<z64*>(smem_raw);
+  z64* buf = tw + n;
+  const long l0 = static_cast<long>(blockIdx.x) * lpb;
+  const int nl = static_cast<int>(min(static_cast<long>(lpb), nlines - l0));
+  for (int t = threadIdx.x; t < n; t += kNT) tw[t] = reinterpret_cast<const z64*>(tws)[t];
+  z64* src = data + l0 * n;
+  for (int t = threadIdx.x; t < nl * n; t += kNT) buf[t] = src[t];
+  __syncthreads();
+  lds_fft<double, kNT, kGridE>(buf, nl, n, ax, tw, dir);
+  for (int t = threadIdx.x; t < nl * n; t += kNT) src[t] = buf[t];
+}
+
+// lines of n points at stride `stride`; blockIdx.x = o selects the line
+// family at offset (o / inner) * outer_stride + (o % inner) * inner_stride,
+// blockIdx.y the run of nrun lines starting at first-axis position x0
+__global__ __launch_bounds__(kNT) void k_grid_fft_s(z64* data, int n, long stride, int n1, int nrun, int inner,
+                                                    long inner_stride, long outer_stride, FftAxis ax,
+                                                    const double* tws, int dir) {
+  extern __shared__ char smem_raw[];
+  z64* tw = reinterpret_cast<z64*>(smem_raw);
+  z64* buf = tw + n;
+  const int ls = n + 1;   // LDS line stride
+  const int x0 = blockIdx.y * nrun;
+  const int nl = min(nrun, n1 - x0);
+  const long o = blockIdx.x;
+  z64* base = data + (o / inner) * outer_stride + (o % inner) * inner_stride + x0;
+  for (int t = threadIdx.x; t < n; t += kNT) tw[t] = reinterpret_cast<const z64*>(tws)[t];
+  for (int t = threadIdx.x; t < nl * n; t += kNT) {
+    const int i = t / nl;
+    const int l = t - i * nl;
+    buf[l * ls + i] = base[l + i * stride];
+  }
+  __syncthreads();
+  lds_fft<double, kNT, kGridE>(buf, nl, ls, ax, tw, dir);
+  for (int t = threadIdx.x; t < nl * n; t += kNT) {
+    const int i = t / nl;
+    const int l = t - i * nl;
+    base[l + i * stride] = buf[l * ls + i];
+  }
+}
+
+// --------------------------------------------------------------------------
 // per-plane 2D part: shared helpers
 // --------------------------------------------------------------------------
 struct PlaneSmem {
@@ -749,6 +802,60 @@ void launch_zfwd(const StickPlan& p, const FftItem* d_items, int nitems, const z
     default: set_smem(k_zfwd<32>, smem); k_zfwd<32><<<grid, kNT, smem, s>>>(P, d_items, sticks, in, ldin, out, ldout, args); break;
   }
   WH_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// full-grid transforms
+// ---------------------------------------------------------------------------
+void GridFft::build(int nr1, int nr2, int nr3) {
+  release();
+  WH_REQUIRE(nr1 > 0 && nr2 > 0 && nr3 > 0, "grid dimensions must be positive");
+  WH_REQUIRE(std::max({nr1, nr2, nr3}) <= kGridE * kNT, "grid dimension too large for the grid transforms");
+  n1 = nr1;
+  n2 = nr2;
+  n3 = nr3;
+  ax1 = make_axis(nr1);
+  ax2 = make_axis(nr2);
+  ax3 = make_axis(nr3);
+  tw1 = upload_twiddles(nr1, allocations);
+  tw2 = upload_twiddles(nr2, allocations);
+  tw3 = upload_twiddles(nr3, allocations);
+}
+
+void GridFft::release() {
+  for (void* p : allocations) (void)hipFree(p);
+  allocations.clear();
+  n1 = n2 = n3 = 0;
+}
+
+void launch_grid_fft(const GridFft& g, z64* data, int batch, int dir, hipStream_t s) {
+  if (batch <= 0) return;
+  WH_REQUIRE(g.n1 > 0, "grid transforms used before GridFft::build");
+  const long nnr = static_cast<long>(g.n1) * g.n2 * g.n3;
+  {
+    const int lpb = std::max(1, kGridE * kNT / g.n1);
+    const long nlines = static_cast<long>(g.n2) * g.n3 * batch;
+    const size_t sm = sizeof(z64) * (g.n1 + static_cast<size_t>(lpb) * g.n1);
+    set_smem(k_grid_fft_x, sm);
+    k_grid_fft_x<<<static_cast<unsigned>((nlines + lpb - 1) / lpb), kNT, sm, s>>>(data, nlines, g.n1, g.ax1, g.tw1,
+                                                                                   dir, lpb);
+    WH_HIP(hipGetLastError());
+  }
+  auto strided = [&](int n, long stride, int inner, long inner_stride, long outer_stride, long nfam,
+                     const FftAxis& ax, const double* tw) {
+    const int nrun = std::max(1, std::min(g.n1, kGridE * kNT / n));
+    const size_t sm = sizeof(z64) * (n + static_cast<size_t>(nrun) * (n + 1));
+    set_smem(k_grid_fft_s, sm);
+    WH_REQUIRE(nfam < (1L << 31), "too many grid lines for one launch");
+    k_grid_fft_s<<<dim3(static_cast<unsigned>(nfam), ceil_div(g.n1, nrun)), kNT, sm, s>>>(
+        data, n, stride, g.n1, nrun, inner, inner_stride, outer_stride, ax, tw, dir);
+    WH_HIP(hipGetLastError());
+  };
+  const long n12 = static_cast<long>(g.n1) * g.n2;
+  // second axis: one family per (grid, third-axis plane), planes contiguous
+  strided(g.n2, g.n1, 1 << 30, n12, 0, static_cast<long>(g.n3) * batch, g.ax2, g.tw2);
+  // third axis: one family per (grid, second-axis row)
+  strided(g.n3, n12, g.n2, g.n1, nnr, static_cast<long>(g.n2) * batch, g.ax3, g.tw3);
 }
 
 }  // namespace wh

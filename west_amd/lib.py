@@ -57,6 +57,11 @@ SIGNATURES = {
     "wh_davidson_diago": ([P, P, P, P, P, I, P, P, P], I),
     "wh_solve_deflated_lanczos_w_full_ortho": ([I, I, I, P, P, P, P, P], I),
     "wh_glbrak_gamma": ([I, P, I, P, P], I),
+    "wh_set_kgrid": ([I, I, I, D, P, I, P, I, I, I, I, I, P, D], I),
+    "wh_set_kpoint": ([I, I, P, P, P, P, P], I),
+    "wh_set_qpoint": ([I, I, P, P, P, P, P, P], I),
+    "wh_set_current_k": ([I], I),
+    "wh_davidson_diago_k": ([I, P, P, P, P, P, I, P, P, P], I),
     "wh_get_counters": ([P, P], I),
     "wh_get_operator_columns": ([P], I),
     "wh_get_precision_counters": ([P, P, P], I),

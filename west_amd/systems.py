@@ -253,8 +253,18 @@ def kb_projectors(system: System):
     """vkb (npw, nkb) complex128 on the wave-function half sphere and the
     diagonal D (nkb,) in Ry.  Order: atom-major, then channel, then m."""
     b = system.basis
-    g = b.gvec
-    q = np.sqrt(np.maximum(b.g2kin, 0.0))
+    vkb, dvals = kb_projectors_at(system, b.gvec, b.omega)
+    # G = 0 of a real function is real: make it exactly so
+    if b.gstart == 2 and vkb.shape[1]:
+        vkb[0, :] = vkb[0, :].real
+    return vkb, dvals
+
+
+def kb_projectors_at(system: System, g: np.ndarray, omega: float):
+    """Projectors at the wave vectors ``g`` (rows, Cartesian bohr^-1; k + G at
+    a k point, as QE's ``init_us_2``): beta(|g|) Y_lm(g) (-i)^l e^{-i g.tau}
+    / sqrt(omega), shape (len(g), nkb), and D (nkb,) in Ry."""
+    q = np.sqrt(np.einsum("ij,ij->i", g, g))
     cols, dvals = [], []
     for ia in range(system.nat):
         sp = system.species[system.ityp[ia]]
@@ -266,17 +276,13 @@ def kb_projectors(system: System):
                 p = 8.0 * np.sqrt(rl ** 5 / 3.0) * np.pi ** 1.25 * q * np.exp(-0.5 * (q * rl) ** 2)
             else:
                 raise ValueError("only s and p projectors are supported")
-            p = p / np.sqrt(b.omega)
+            p = p / np.sqrt(omega)
             for y in _real_ylm(l, g):
                 cols.append(((-1j) ** l) * p * y * phase)
                 dvals.append(2.0 * h)
     if not cols:
-        return np.zeros((b.npw, 0), dtype=np.complex128), np.zeros(0)
-    vkb = np.stack(cols, axis=1)
-    # G = 0 of a real function is real: make it exactly so
-    if b.gstart == 2:
-        vkb[0, :] = vkb[0, :].real
-    return np.ascontiguousarray(vkb), np.array(dvals)
+        return np.zeros((g.shape[0], 0), dtype=np.complex128), np.zeros(0)
+    return np.ascontiguousarray(np.stack(cols, axis=1)), np.array(dvals)
 
 
 def make_h2_clusters(nsys: int = 24, seed: int = 2024, ecutwfc: float = 25.0, min_sep: float = 4.0,

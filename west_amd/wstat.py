@@ -9,6 +9,8 @@ Names, argument meaning and error behaviour follow the reference:
 * :func:`dfpt`                          -- ``DFPT_kernel/dfpt_module.f90:22``
 * :func:`apply_operator`                -- ``Wstat/apply_operator.f90:14``
 * :func:`davidson_diago`                -- ``Wstat/davidson_diago.f90:31``
+* :func:`davidson_diago_k`              -- ``Wstat/davidson_diago.f90:441`` (one q point),
+  after :func:`set_k_ground_state`; :func:`dfpt` / :func:`apply_operator` then take ``iq``
 * :func:`set_davidson_restart`          -- ``Wstat/davidson_restart.f90`` (inputs of ``Modules/westcom.f90``)
 
 Arrays are numpy ``(m, npwx)`` complex128 (the memory layout of the
@@ -166,6 +168,83 @@ def davidson_diago(n_pdep_eigen: int, n_pdep_times: int = 4, n_pdep_maxiter: int
     rc = check(lib.wh_davidson_diago(C.addressof(prm), ptr(ev), ptr(conv), ptr(dvg) if dvg is not None else None,
                                      C.addressof(log), cap, C.addressof(nlog), ptr(ev_log), ptr(conv_log)),
                allow_positive=True)
+    recs = [{"dav_iter": s.dav_iter, "ev": ev_log[i].tolist(), "conv": [bool(v) for v in conv_log[i]],
+             "notcnv": s.notcnv, "time_elap:sec": s.time_elap, "time_iter:sec": s.time_iter,
+             "sternop_ncalls": s.sternop_ncalls, "dfpt_tr2": s.dfpt_tr2, "dfpt_dim": s.dfpt_dim,
+             "diago_dim": s.diago_dim} for i, s in enumerate(log[:nlog.value])]
+    if rc > 0:
+        warnings.warn(f"WARNING : {rc} eigenvalues not converged in chidiago")
+    return ev, conv.astype(bool), dvg, recs
+
+
+# ---------------------------------------------------------------------------
+# q points (Wstat away from Gamma)
+# ---------------------------------------------------------------------------
+_kgs_by_sys: dict = {}
+
+
+def set_k_ground_state(kgs, n_dfpt_maxiter: int = 250, l_kinetic_only: bool = False):
+    """Register a :class:`west_amd.kpoint.KGroundState` (the inputs of the
+    non-Gamma branches of dfpt: per k the sphere, g2kin, evc, et and vkb; per q
+    the |q+G| sphere, the Coulomb factor and the k - q map).  The system's
+    operator, CG and chi0 entry points then act at k and q points
+    (:func:`set_current_k`; ``iq`` of :func:`dfpt` / :func:`apply_operator`)."""
+    from .coulomb import sqvc_init
+    lib = load()
+    nr = kgs.nr
+    nks, nqs = kgs.nks, len(kgs.qpts)
+    wk = np.ascontiguousarray([k.weight for k in kgs.kpts], dtype=np.float64)
+    dvan = np.ascontiguousarray(kgs.dvan, dtype=np.float64)
+    check(lib.wh_set_kgrid(nr[0], nr[1], nr[2], float(kgs.omega), ptr(np.ascontiguousarray(kgs.vrs, dtype=np.float64)),
+                           nks, ptr(wk), int(kgs.npwx), nqs, int(kgs.npwqx), int(kgs.nbndval), int(kgs.nkb),
+                           ptr(dvan) if kgs.nkb else None, float(kgs.alpha_pv)))
+    check(lib.wh_set_params(int(n_dfpt_maxiter), int(bool(l_kinetic_only))))
+    nbv = kgs.nbndval
+    for ik, kp in enumerate(kgs.kpts):
+        b = kp.basis
+        evc = np.ascontiguousarray(kp.evc[:nbv], dtype=np.complex128)
+        et = np.ascontiguousarray(kp.et[:nbv], dtype=np.float64)
+        vkb = np.ascontiguousarray(kp.vkb, dtype=np.complex128)
+        check(lib.wh_set_kpoint(ik + 1, b.npw, ptr(np.ascontiguousarray(b.nl, dtype=np.int32)),
+                                ptr(np.ascontiguousarray(b.g2kin, dtype=np.float64)), ptr(evc), ptr(et),
+                                ptr(vkb) if kgs.nkb else None))
+    for iq, qp in enumerate(kgs.qpts):
+        b = qp.basis
+        sq = sqvc_init(b)
+        check(lib.wh_set_qpoint(iq + 1, b.npw, ptr(np.ascontiguousarray(b.nl, dtype=np.int32)),
+                                ptr(np.ascontiguousarray(b.g2kin, dtype=np.float64)),
+                                ptr(np.ascontiguousarray(qp.ig_global, dtype=np.int32)), ptr(sq),
+                                ptr(np.ascontiguousarray(qp.ikq + 1, dtype=np.int32)),
+                                ptr(np.ascontiguousarray(qp.g0, dtype=np.int32))))
+    _kgs_by_sys[_cur()] = kgs
+    _gs_by_sys.pop(_cur(), None)
+    _grid_key_by_sys[_cur()] = None
+
+
+def set_current_k(ik: int):
+    """k point (1-based, QE's ``current_k``) of the operator and CG calls."""
+    check(load().wh_set_current_k(int(ik)))
+
+
+def davidson_diago_k(iq: int, n_pdep_eigen: int, n_pdep_times: int = 4, n_pdep_maxiter: int = 100,
+                     trev_pdep: float = 1e-3, trev_pdep_rel: float = 0.1, tr2_dfpt: float = 1e-16,
+                     want_vectors: bool = True):
+    """Wstat Davidson at q point ``iq`` (1-based; ``davidson_diago_k``); returns
+    (ev, conv, dvg or None, davitr records)."""
+    lib = load()
+    kgs = _kgs_by_sys[_cur()]
+    prm = DavidsonParams(n_pdep_eigen, n_pdep_times, n_pdep_maxiter, trev_pdep, trev_pdep_rel, tr2_dfpt)
+    ev = np.zeros(n_pdep_eigen)
+    conv = np.zeros(n_pdep_eigen, dtype=np.int32)
+    dvg = np.zeros((n_pdep_eigen, kgs.npwqx), dtype=np.complex128) if want_vectors else None
+    cap = n_pdep_maxiter + 2
+    log = (DavidsonStep * cap)()
+    ev_log = np.zeros((cap, n_pdep_eigen))
+    conv_log = np.zeros((cap, n_pdep_eigen), dtype=np.int32)
+    nlog = C.c_int(0)
+    rc = check(lib.wh_davidson_diago_k(int(iq), C.addressof(prm), ptr(ev), ptr(conv),
+                                       ptr(dvg) if dvg is not None else None, C.addressof(log), cap,
+                                       C.addressof(nlog), ptr(ev_log), ptr(conv_log)), allow_positive=True)
     recs = [{"dav_iter": s.dav_iter, "ev": ev_log[i].tolist(), "conv": [bool(v) for v in conv_log[i]],
              "notcnv": s.notcnv, "time_elap:sec": s.time_elap, "time_iter:sec": s.time_iter,
              "sternop_ncalls": s.sternop_ncalls, "dfpt_tr2": s.dfpt_tr2, "dfpt_dim": s.dfpt_dim,
