@@ -44,6 +44,8 @@ WORKLOADS = {
     "c4": (None, None),
     # Wstat Davidson for 256 PDEP on the si64 ground state (nvecx 1024), FP64 and mixed
     "si64dav": ({"nrep": 2, "a": 10.26, "ecutwfc": 25.0, "sigma": 0.05, "seed": 2024}, 64),
+    # chi0 at a q point: H2 molecular crystal on a 3x3x3 k grid (the default line's "qpoint" field alone)
+    "qpoint": (None, None),
 }
 
 
@@ -88,6 +90,8 @@ def main():
                          "1 GiB read, 1 GiB written) to scale HBM byte counters of a profiler run")
     ap.add_argument("--no-davidson", action="store_true",
                     help="skip the Wstat Davidson wall-time runs (6 H2, 32 PDEP; Si64, 256 PDEP)")
+    ap.add_argument("--no-qpoint", action="store_true",
+                    help="skip the q-point chi0 measurement (H2 molecular crystal on a 3x3x3 k grid)")
     ap.add_argument("--gs-cache", default=None,
                     help="npz file: load the ground state from it if present, else write it there "
                          "(lets a profiled run skip the ground-state kernels)")
@@ -113,6 +117,12 @@ def main():
 
     if args.workload == "c4":
         _batch_c4(wstat, torch, args)
+        return
+    if args.workload == "qpoint":
+        wstat.init(0)
+        print(json.dumps({"metric": "chi0(q).dv applies/s (pert x band x k), Wstat dfpt at q",
+                          "qpoint": _qpoint_measure(wstat, torch, args, with_cpu=not args.no_cpu_baseline)}),
+              flush=True)
         return
 
     kw, npert = WORKLOADS[args.workload]
@@ -217,6 +227,11 @@ def main():
         if cpu is not None:
             cpu["davidson_6h2"] = _davidson_h2_cpu(h2, dav)
 
+    # ---- chi0 at a q point (k-point ground state), FP64, with the oracle beside it
+    qpt = None
+    if not args.no_qpoint:
+        qpt = _qpoint_measure(wstat, torch, args, with_cpu=not args.no_cpu_baseline)
+
     rec = {
         "metric": "chi0.dv applies/s (pert x band), Wstat dfpt at tr2_dfpt",
         "value": round(value, 3),
@@ -246,8 +261,86 @@ def main():
         "mixed_precision": mixed,
         "davidson": dav,
         "davidson_si64": dav64,
+        "qpoint": qpt,
     }
     print(json.dumps(rec), flush=True)
+
+
+def _qpoint_measure(wstat, torch, args, with_cpu=True, nk=(3, 3, 3), npert=16, ecut=25.0):
+    """chi0(q) dv away from Gamma: the H2 molecular crystal (2 H2, s projectors)
+    at ecutwfc 25 Ry on an unshifted 3x3x3 k grid (27 k points, every one
+    explicit), one batch of 16 perturbations at the last q point
+    (2/3, 2/3, 2/3), tr2 = args.tr2.  A pair is (perturbation, band, k point).
+    Runs on a system of its own.  The grid-FFT figure counts 3 passes x
+    (read + write) of every transformed grid column."""
+    from oracle import kpoint as K
+    from west_amd.kpoint import make_h2_crystal, make_k_ground_state
+    t0 = time.perf_counter()
+    kgs = make_k_ground_state(make_h2_crystal(ecut), nk, nbnd=4)
+    iq = len(kgs.qpts) - 1
+    qp = kgs.qpts[iq]
+    qb = qp.basis
+    nb, nks = kgs.nbndval, kgs.nks
+    nnr = int(np.prod(kgs.nr))
+    log(f"[bench] qpoint: {kgs.meta['system']} k grid {nk}, npwq={qb.npw}, fft={kgs.nr}, "
+        f"ground state {time.perf_counter() - t0:.1f}s, gap {kgs.meta.get('gap_ry', float('nan')):.3f} Ry")
+    h = wstat.system_create()
+    wstat.system_select(h)
+    try:
+        wstat.set_k_ground_state(kgs)
+        rng = np.random.default_rng(17)
+        dvg_h = np.zeros((npert, kgs.npwqx), complex)
+        dvg_h[:, :qb.npw] = (rng.normal(size=(npert, qb.npw)) + 1j * rng.normal(size=(npert, qb.npw))) / \
+            (1.0 + qb.g2kin)[None, :]
+        dvg = torch.from_numpy(dvg_h).to("cuda:0")
+        dng = torch.zeros_like(dvg)
+        wstat.dfpt(npert, dvg, args.tr2, iq=iq + 1, dng=dng)          # warm-up (kernel variants, workspaces)
+        torch.cuda.synchronize()
+        wstat.profile(True)
+        cols0 = wstat.counters()[0]
+        steps = 3
+        t = time.perf_counter()
+        for _ in range(steps):
+            wstat.dfpt(npert, dvg, args.tr2, iq=iq + 1, dng=dng)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / steps
+        kern = {n: wstat.profile_query(n) for n in ("kfft", "gemm_bra", "gemm_ket")}
+        op_cols = (wstat.counters()[0] - cols0) / steps
+        wstat.profile(False)
+        pairs = npert * nb * nks
+        fft_cols = npert + nks * (nb + 2 * npert * nb + npert) + 2 * op_cols
+        fft_bytes = fft_cols * 6 * 16 * nnr
+        kfft_ms = kern["kfft"][0] / steps
+        rec = {"workload": f"H2 molecular crystal (2 H2, 6.5x6.0x5.5 bohr triclinic, s projectors), {ecut:g} Ry, "
+                           f"k grid {nk[0]}x{nk[1]}x{nk[2]} ({nks} k points), q = {qp.cryst.round(4).tolist()}, "
+                           f"npwq={qb.npw}, fft={kgs.nr[0]}x{kgs.nr[1]}x{kgs.nr[2]}, {nb} bands, "
+                           f"{npert} perturbations, tr2={args.tr2:g}",
+               "value": round(pairs / dt, 2), "unit": "pairs/s (pert x band x k)", "ms_per_batch": round(dt * 1e3, 2),
+               "operator_columns_per_batch": op_cols,
+               "kernels_ms_per_batch": {n: round(v[0] / steps, 3) for n, v in kern.items()},
+               "grid_fft": {"bound": "hbm", "achieved": round(fft_bytes / (kfft_ms * 1e-3) / 1e9, 1) if kfft_ms > 0 else None,
+                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(fft_bytes / (kfft_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if kfft_ms > 0 else None,
+                            "bytes_per_batch": int(fft_bytes)}}
+        out = dng.cpu().numpy()
+        ref = K.dfpt_q(kgs, iq, dvg_h[:1], args.tr2)
+        rec["rel_err_vs_oracle_pert1"] = float(np.abs(out[:1] - ref).max() / np.abs(ref).max())
+        if with_cpu:
+            import oracle.fft_at_gamma as ofa
+            threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+            ofa.WORKERS = threads
+            tc = time.perf_counter()
+            K.dfpt_q(kgs, iq, dvg_h[:4], args.tr2)
+            tc = time.perf_counter() - tc
+            rec["cpu_baseline"] = {"value": round(4 * nb * nks / tc, 3), "unit": "pairs/s", "cores": threads,
+                                   "kind": "port", "sample": f"perturbations 1-4 of the batch, all k points and "
+                                                             f"bands, FP64 oracle, {tc:.1f}s"}
+        log(f"[bench] qpoint: {rec['value']} pairs/s, {rec['ms_per_batch']} ms per batch, "
+            f"err vs oracle {rec['rel_err_vs_oracle_pert1']:.1e}")
+        return rec
+    finally:
+        wstat.system_select(0)
+        wstat.system_destroy(h)
 
 
 # Davidson settings: the reference's defaults (trev_pdep 1e-3, trev_pdep_rel 0.1)

@@ -183,6 +183,55 @@ MODULE west_hip
      !
      ! ev_log(n_pdep_eigen, log_capacity), conv_log(n_pdep_eigen, log_capacity):
      ! the ev and conv of each step, as output_ev_and_time records them
+     ! q points (Wstat away from Gamma): dfpt_module.f90 non-Gamma branches,
+     ! davidson_diago_k (Wstat/davidson_diago.f90:441-905)
+     INTEGER(C_INT) FUNCTION wh_set_kgrid(nr1, nr2, nr3, omega, vrs, nks, wk, npwx, nqs, npwqx, nbndval, nkb, &
+          & dvan, alpha_pv) BIND(C, NAME="wh_set_kgrid")
+       IMPORT :: C_INT, C_DOUBLE
+       INTEGER(C_INT), VALUE :: nr1, nr2, nr3, nks, npwx, nqs, npwqx, nbndval, nkb
+       REAL(C_DOUBLE), VALUE :: omega, alpha_pv
+       REAL(C_DOUBLE), INTENT(IN) :: vrs(*), wk(*)
+       REAL(C_DOUBLE), INTENT(IN), OPTIONAL :: dvan(*)
+     END FUNCTION
+
+     INTEGER(C_INT) FUNCTION wh_set_kpoint(ik, npw, nl, g2kin, evc, et, vkb) BIND(C, NAME="wh_set_kpoint")
+       IMPORT :: C_INT, C_DOUBLE, C_DOUBLE_COMPLEX
+       INTEGER(C_INT), VALUE :: ik, npw
+       INTEGER(C_INT), INTENT(IN) :: nl(*)
+       REAL(C_DOUBLE), INTENT(IN) :: g2kin(*), et(*)
+       COMPLEX(C_DOUBLE_COMPLEX), INTENT(IN) :: evc(*)
+       COMPLEX(C_DOUBLE_COMPLEX), INTENT(IN), OPTIONAL :: vkb(*)
+     END FUNCTION
+
+     INTEGER(C_INT) FUNCTION wh_set_qpoint(iq, npwq, nl, gq2kin, ig_global, sqvc, ikq, g0) &
+          & BIND(C, NAME="wh_set_qpoint")
+       IMPORT :: C_INT, C_DOUBLE
+       INTEGER(C_INT), VALUE :: iq, npwq
+       INTEGER(C_INT), INTENT(IN) :: nl(*), ig_global(*), ikq(*), g0(*)
+       REAL(C_DOUBLE), INTENT(IN) :: gq2kin(*)
+       REAL(C_DOUBLE), INTENT(IN), OPTIONAL :: sqvc(*)
+     END FUNCTION
+
+     INTEGER(C_INT) FUNCTION wh_set_current_k(ik) BIND(C, NAME="wh_set_current_k")
+       IMPORT :: C_INT
+       INTEGER(C_INT), VALUE :: ik
+     END FUNCTION
+
+     INTEGER(C_INT) FUNCTION wh_davidson_diago_k(iq, params, ev, conv, dvg_out, log, log_capacity, log_len, &
+          & ev_log, conv_log) BIND(C, NAME="wh_davidson_diago_k")
+       IMPORT :: C_INT, C_DOUBLE, C_DOUBLE_COMPLEX, wh_davidson_params, wh_davidson_step
+       INTEGER(C_INT), VALUE :: iq
+       TYPE(wh_davidson_params), INTENT(IN) :: params
+       REAL(C_DOUBLE), INTENT(OUT) :: ev(*)
+       INTEGER(C_INT), INTENT(OUT), OPTIONAL :: conv(*)
+       COMPLEX(C_DOUBLE_COMPLEX), INTENT(OUT), OPTIONAL :: dvg_out(*)
+       TYPE(wh_davidson_step), INTENT(OUT), OPTIONAL :: log(*)
+       INTEGER(C_INT), VALUE :: log_capacity
+       INTEGER(C_INT), INTENT(OUT), OPTIONAL :: log_len
+       REAL(C_DOUBLE), INTENT(OUT), OPTIONAL :: ev_log(*)
+       INTEGER(C_INT), INTENT(OUT), OPTIONAL :: conv_log(*)
+     END FUNCTION
+
      INTEGER(C_INT) FUNCTION wh_davidson_diago(params, ev, conv, dvg_out, log, log_capacity, log_len, &
           & ev_log, conv_log) BIND(C, NAME="wh_davidson_diago")
        IMPORT :: C_INT, C_DOUBLE, C_DOUBLE_COMPLEX, wh_davidson_params, wh_davidson_step

@@ -22,4 +22,9 @@ Parity status per function (what pins it; see ``tests/test_oracle_*.py``):
 * ``davidson_diago_gamma``: ``numpy.linalg.eigh`` of the explicitly assembled
   chi0~ matrix on tiny cells; iteration-control semantics restated line by line.
 * G-sphere counts: the reference test-suite fixtures (npw/ngm/FFT grid).
+* ``kpoint`` (chi0(q), the k-point operator and CG, ``davidson_diago_k``):
+  supercell folding -- the Gamma chi0 of a doubled cell equals chi0(q) of the
+  primitive cell on the matching k grid, sector by sector, and the union of
+  the q-point Davidson spectra equals the supercell's; dense H_k; Hermiticity
+  and negativity of chi0~(q).
 """
