@@ -428,7 +428,7 @@ def test_edge_cases_and_errors(gpu, gs_si8):
     out = gpu.apply_sternheimerop_to_m_wfcs(gs.nbndval, psi, e, gs.alpha_pv, m=3)
     assert _rel(out[:3], apply_sternheimerop(gs, psi[:3], e[:3], gs.alpha_pv)) < 1e-12
     assert not out[3:].any()
-    with pytest.raises(WestHipError, match="iq = 1"):
+    with pytest.raises(WestHipError, match="iq must be 1"):
         gpu.dfpt(1, psi[:1], 1e-12, iq=2)
     with pytest.raises(WestHipError, match="nbndval"):
         gpu.apply_sternheimerop_to_m_wfcs(gs.nbndval + 1, psi, e, gs.alpha_pv)
