@@ -113,3 +113,44 @@ def test_davidson_at_q(gpu, ksys):
         n = kgs.qpts[iq].basis.npw
         gram = dvg[:, :n].conj() @ dvg[:, :n].T
         assert np.abs(gram - np.eye(6)).max() < 1e-8
+
+
+@pytest.mark.gpu
+def test_kinetic_only_and_device_arrays_at_k(gpu):
+    """l_kinetic_only at a k point (k_psi plus alpha P_v) and device (torch)
+    arrays through the k-point entries."""
+    import torch
+    from oracle import kpoint as K
+    kgs = make_k_ground_state(make_h2_crystal(10.0), (2, 1, 1), nbnd=4)
+    h = gpu.system_create()
+    gpu.system_select(h)
+    try:
+        print("case kpoint kinetic-only operator", flush=True)
+        kgs.l_kinetic_only = True
+        gpu.set_k_ground_state(kgs, l_kinetic_only=True)
+        v = K.KView(kgs, 1)
+        psi = _smooth(v.basis.npw, kgs.npwx, 3, v.basis.g2kin, 31)
+        e = np.array([-0.3, -0.1, 0.2])
+        gpu.set_current_k(2)
+        ref = K.apply_sternheimerop_k(v, psi, e, kgs.alpha_pv)
+        assert _rel(gpu.apply_sternheimerop_to_m_wfcs(kgs.nbndval, psi, e, kgs.alpha_pv), ref) < 1e-12
+        print("case kpoint device arrays", flush=True)
+        kgs.l_kinetic_only = False
+        gpu.set_k_ground_state(kgs)
+        qb = kgs.qpts[1].basis
+        dvg = _smooth(qb.npw, kgs.npwqx, 2, qb.g2kin, 41)
+        dvg_d = torch.from_numpy(dvg).to("cuda:0")
+        out = gpu.dfpt(2, dvg_d, 1e-18, iq=2).cpu().numpy()
+        assert _rel(out, K.dfpt_q(kgs, 1, dvg, 1e-18)) < 1e-9
+        # back to Gamma on the same system: the k grid is dropped
+        from oracle.dfpt import sqvc_init
+        from west_amd.groundstate import make_ground_state
+        from oracle.hamiltonian import h_psi
+        gs = make_ground_state(make_h2_crystal(10.0), lambda g: (lambda p: h_psi(g, p)), nbnd=3, tol=1e-10)
+        gpu.set_ground_state(gs, sqvc=sqvc_init(gs.basis, gs.npwx))
+        from west_amd.lib import WestHipError
+        with pytest.raises(WestHipError, match="iq must be 1"):
+            gpu.dfpt(1, np.zeros((1, gs.npwx), complex), -1.0, iq=2)
+    finally:
+        gpu.system_select(0)
+        gpu.system_destroy(h)
