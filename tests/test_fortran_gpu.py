@@ -1,4 +1,5 @@
-"""The boundary as the Fortran host uses it: tests/fortran_hotpath.f90 is
+"""The boundary as the Fortran host uses it: tests/fortran_hotpath.f90 (Gamma) and
+tests/fortran_qpoint.f90 (q points) are
 built with amdflang against include/west_hip.f90 and libwesthip.so, reads a
 ground state, and calls wh_set_*, wh_apply_operator, wh_dfpt (optional ierr
 array absent) and wh_davidson_diago (per-step ev/conv log) on the GPU; the
@@ -90,3 +91,66 @@ def test_fortran_host_calls_the_hot_path(tmp_path, key, request):
     for row, rec in zip(ev_log, log_ref):
         assert np.max(np.abs(row - np.asarray(rec["ev"])) / np.abs(np.asarray(rec["ev"]))) < 1e-7
     assert np.all(ncalls > 0)
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not Path(FC).exists(), reason="amdflang not installed")
+def test_fortran_host_calls_the_qpoint_path(tmp_path):
+    """tests/fortran_qpoint.f90: the k-grid registration, apply_operator at q
+    point 2 and davidson_diago_k at q point 2, from Fortran, against the oracle."""
+    from oracle import kpoint as K
+    from west_amd import build_lib
+    from west_amd.coulomb import sqvc_init
+    from west_amd.kpoint import make_h2_crystal, make_k_ground_state
+    kgs = make_k_ground_state(make_h2_crystal(10.0), (2, 1, 1), nbnd=4)
+    m, nvec, tr2 = 2, 4, 1e-20
+    qb = kgs.qpts[1].basis
+    rng = np.random.default_rng(71)
+    dvg = np.zeros((m, kgs.npwqx), complex)
+    dvg[:, :qb.npw] = (rng.normal(size=(m, qb.npw)) + 1j * rng.normal(size=(m, qb.npw))) / (1.0 + qb.g2kin)
+    with open(tmp_path / "kgs.bin", "wb") as f:
+        np.array([*kgs.nr, kgs.nks, kgs.npwx, len(kgs.qpts), kgs.npwqx, kgs.nbndval, kgs.nkb, m, nvec],
+                 dtype=np.int32).tofile(f)
+        np.array([kgs.omega, kgs.alpha_pv, tr2]).tofile(f)
+        np.asarray(kgs.vrs, dtype=np.float64).tofile(f)
+        np.array([k.weight for k in kgs.kpts]).tofile(f)
+        if kgs.nkb:
+            np.asarray(kgs.dvan, dtype=np.float64).tofile(f)
+        for kp in kgs.kpts:
+            np.array([kp.basis.npw], dtype=np.int32).tofile(f)
+            np.asarray(kp.basis.nl, dtype=np.int32).tofile(f)
+            np.asarray(kp.basis.g2kin, dtype=np.float64).tofile(f)
+            np.ascontiguousarray(kp.evc[:kgs.nbndval]).tofile(f)
+            np.asarray(kp.et[:kgs.nbndval], dtype=np.float64).tofile(f)
+            if kgs.nkb:
+                np.ascontiguousarray(kp.vkb).tofile(f)
+        for qp in kgs.qpts:
+            b = qp.basis
+            np.array([b.npw], dtype=np.int32).tofile(f)
+            np.asarray(b.nl, dtype=np.int32).tofile(f)
+            np.asarray(b.g2kin, dtype=np.float64).tofile(f)
+            np.asarray(qp.ig_global, dtype=np.int32).tofile(f)
+            sqvc_init(b).tofile(f)
+            np.asarray(qp.ikq + 1, dtype=np.int32).tofile(f)
+            np.asarray(qp.g0, dtype=np.int32).tofile(f)
+        np.ascontiguousarray(dvg).tofile(f)
+    lib = build_lib.LIB
+    subprocess.run([FC, "-J", str(tmp_path), "-c", str(ROOT / "include" / "west_hip.f90"), "-o",
+                    str(tmp_path / "west_hip.o")], check=True, capture_output=True)
+    exe = tmp_path / "qpt"
+    subprocess.run([FC, "-J", str(tmp_path), str(ROOT / "tests" / "fortran_qpoint.f90"), str(tmp_path / "west_hip.o"),
+                    str(lib), f"-Wl,-rpath,{lib.parent}", "-o", str(exe)], check=True, capture_output=True)
+    print("case fortran qpoint: running the driver", flush=True)
+    r = subprocess.run([str(exe)], cwd=tmp_path, capture_output=True, text=True, timeout=100)
+    assert r.returncode == 0, r.stdout + r.stderr
+    with open(tmp_path / "kout.bin", "rb") as f:
+        dng = np.fromfile(f, dtype=np.complex128, count=m * kgs.npwqx).reshape(m, kgs.npwqx)
+        ev = np.fromfile(f, dtype=np.float64, count=nvec)
+        conv = np.fromfile(f, dtype=np.int32, count=nvec)
+    sq = sqvc_init(qb, kgs.npwqx)
+    ref = K.apply_operator_q(kgs, 1, sq, dvg, tr2)
+    assert float(np.abs(dng - ref).max() / np.abs(ref).max()) < 1e-9
+    ev_ref, conv_ref, _ = K.davidson_diago_k(kgs, 1, sq, nvec, n_pdep_times=4, n_pdep_maxiter=40, trev_pdep=1e-8,
+                                             trev_pdep_rel=1e-7, tr2_dfpt=1e-18)
+    assert np.max(np.abs(ev - ev_ref) / np.abs(ev_ref)) < 1e-8
+    assert np.array_equal(conv.astype(bool), conv_ref)
