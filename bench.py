@@ -46,6 +46,8 @@ WORKLOADS = {
     "si64dav": ({"nrep": 2, "a": 10.26, "ecutwfc": 25.0, "sigma": 0.05, "seed": 2024}, 64),
     # chi0 at a q point: H2 molecular crystal on a 3x3x3 k grid (the default line's "qpoint" field alone)
     "qpoint": (None, None),
+    # one FP64 Davidson of the 6-H2 system at the converged settings, warm-up then timed (for traces)
+    "c1dav": (None, None),
 }
 
 
@@ -117,6 +119,10 @@ def main():
 
     if args.workload == "c4":
         _batch_c4(wstat, torch, args)
+        return
+    if args.workload == "c1dav":
+        wstat.init(0)
+        print(json.dumps(_davidson_c1_fp64(wstat, torch, args)), flush=True)
         return
     if args.workload == "qpoint":
         wstat.init(0)
@@ -531,6 +537,27 @@ def _davidson_h2(wstat, torch, args):
             "davidson_converged": DAV_CONVERGED, "fp64": res["fp64"], "mixed": res["mixed"],
             "ev_max_rel_err_mixed_vs_fp64_converged_setting": _max_rel(evs["mixed", "converged_setting"], ref),
             "ev_fp64_lowest": [round(float(x), 8) for x in ref[:4]]}, state
+
+
+def _davidson_c1_fp64(wstat, torch, args):
+    """The 6-H2 FP64 Davidson (32 PDEP) at the converged settings: one warm-up
+    run, then one timed run between device synchronisations."""
+    from west_amd.coulomb import sqvc_init
+    from west_amd.groundstate import make_ground_state
+    from west_amd.systems import make_h2_system
+    s = make_h2_system(nmol=6, L=18.0, ecutwfc=25.0, seed=2024)
+    gs = make_ground_state(s, wstat.gpu_h_factory, nbnd=s.nbnd_occ + 4, scf_steps=args.scf, tol=1e-7)
+    wstat.set_ground_state(gs, sqvc=sqvc_init(s.basis, gs.npwx))
+    wstat.davidson_diago(32, n_pdep_times=4, want_vectors=False, **DAV_CONVERGED)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    ev, conv, _, recs = wstat.davidson_diago(32, n_pdep_times=4, want_vectors=False, **DAV_CONVERGED)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t
+    log(f"[bench] 6H2 davidson fp64 converged_setting: {wall:.4f}s, {len(recs)} iterations")
+    return {"metric": "Wstat Davidson wall time, 6 H2, 32 PDEP, converged settings", "value": round(wall, 4),
+            "unit": "s", "higher_is_better": False, "iterations": len(recs), "converged": int(conv.sum()),
+            "ev_lowest": [round(float(x), 10) for x in ev[:4]]}
 
 
 def _cpu_model():

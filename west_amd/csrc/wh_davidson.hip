@@ -26,6 +26,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <map>
+#include <mutex>
 #include <sstream>
 #include <string>
 
@@ -106,6 +108,29 @@ Solver& solver() {
   }
   WH_REQUIRE(rocblas_set_stream(s.h, ctx().stream) == rocblas_status_success, "rocblas_set_stream failed");
   return s;
+}
+
+// The subspace eigensolve (matdiago_dsy, Tools/linear_algebra_kernel.f90:84-181)
+// on rocSOLVER: divide and conquer (dsyevd), QR iteration (dsyev) or Jacobi
+// (dsyevj, tolerance at machine precision).  On matrices of a few hundred
+// rows they differ several-fold in time; the first call in each size class
+// (n rounded up to 32) times all three on copies of its matrix and keeps the
+// fastest, and every call then runs that one.
+std::mutex g_eig_mutex;
+std::map<int, int>& eig_choice() {
+  static std::map<int, int> m;
+  return m;
+}
+const char* const kEigName[3] = {"dsyevd", "dsyev", "dsyevj"};
+
+void run_syev(rocblas_handle h, int v, int n, double* A, double* D, double* E, rocblas_int* info, double* resid,
+              rocblas_int* nsw) {
+  rocblas_status st;
+  if (v == 0) st = rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, n, A, n, D, E, info);
+  else if (v == 1) st = rocsolver_dsyev(h, rocblas_evect_original, rocblas_fill_upper, n, A, n, D, E, info);
+  else st = rocsolver_dsyevj(h, rocblas_esort_ascending, rocblas_evect_original, rocblas_fill_upper, n, A, n, 0.0, resid,
+                             100, nsw, D, info);
+  WH_REQUIRE(st == rocblas_status_success, std::string("rocsolver_") + kEigName[v] + " failed");
 }
 
 // glbrak of rows: C (na x nb, col major ldc) = <A_i|B_j>
@@ -377,19 +402,52 @@ extern "C" int wh_davidson_diago(const wh_davidson_params* prm, double* ev_out, 
       double* A = b_hr.as<double>(static_cast<size_t>(n) * n);
       double* D = b_w.as<double>(n);
       double* E = b_e.as<double>(n);
-      rocblas_int* info = b_info.as<rocblas_int>(1);
+      rocblas_int* info = b_info.as<rocblas_int>(4);
+      double* resid = reinterpret_cast<double*>(info + 2);
+      rocblas_int* nsw = info + 1;
       std::vector<double> a(static_cast<size_t>(n) * n);
       for (int j = 0; j < n; ++j) std::memcpy(&a[static_cast<size_t>(j) * n], &hr[static_cast<size_t>(j) * nvecx], sizeof(double) * n);
+      rocblas_handle h = solver().h;
+      int pick = 0;
+      {
+        std::lock_guard<std::mutex> lk(g_eig_mutex);
+        const int cls = (n + 31) / 32;
+        auto it = eig_choice().find(cls);
+        if (it == eig_choice().end()) {
+          // time the three on copies (each run once untimed first: its workspace)
+          float t[3] = {0.f, 0.f, 0.f};
+          hipEvent_t e0, e1;
+          WH_HIP(hipEventCreate(&e0));
+          WH_HIP(hipEventCreate(&e1));
+          for (int v = 0; v < 3; ++v)
+            for (int rep = 0; rep < 2; ++rep) {
+              WH_HIP(hipMemcpyAsync(A, a.data(), sizeof(double) * a.size(), hipMemcpyHostToDevice, s));
+              WH_HIP(hipEventRecord(e0, s));
+              run_syev(h, v, n, A, D, E, info, resid, nsw);
+              WH_HIP(hipEventRecord(e1, s));
+              WH_HIP(hipEventSynchronize(e1));
+              WH_HIP(hipEventElapsedTime(&t[v], e0, e1));
+            }
+          WH_HIP(hipEventDestroy(e0));
+          WH_HIP(hipEventDestroy(e1));
+          for (int v = 1; v < 3; ++v)
+            if (t[v] < t[pick]) pick = v;
+          if (tune_log())
+            std::fprintf(stderr, "[west_hip] subspace eigensolve n=%d: dsyevd %.3f dsyev %.3f dsyevj %.3f ms -> %s\n",
+                         n, t[0], t[1], t[2], kEigName[pick]);
+          it = eig_choice().emplace(cls, pick).first;
+        }
+        pick = it->second;
+      }
       WH_HIP(hipMemcpyAsync(A, a.data(), sizeof(double) * a.size(), hipMemcpyHostToDevice, s));
-      WH_REQUIRE(rocsolver_dsyevd(solver().h, rocblas_evect_original, rocblas_fill_upper, n, A, n, D, E, info) ==
-                     rocblas_status_success, "rocsolver_dsyevd failed");
+      run_syev(h, pick, n, A, D, E, info, resid, nsw);
       std::vector<double> w(n);
       rocblas_int hinfo = 0;
       WH_HIP(hipMemcpyAsync(a.data(), A, sizeof(double) * a.size(), hipMemcpyDeviceToHost, s));
       WH_HIP(hipMemcpyAsync(w.data(), D, sizeof(double) * n, hipMemcpyDeviceToHost, s));
       WH_HIP(hipMemcpyAsync(&hinfo, info, sizeof(rocblas_int), hipMemcpyDeviceToHost, s));
       WH_HIP(hipStreamSynchronize(s));
-      WH_REQUIRE(hinfo == 0, "dsyevd did not converge");
+      WH_REQUIRE(hinfo == 0, std::string(kEigName[pick]) + " did not converge");
       for (int j = 0; j < nvec; ++j) {
         ew[j] = w[j];
         std::memcpy(&vr[static_cast<size_t>(j) * nvecx], &a[static_cast<size_t>(j) * n], sizeof(double) * n);
