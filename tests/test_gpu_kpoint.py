@@ -1,8 +1,10 @@
 """The q-point path on the GPU against the oracle (``oracle/kpoint.py``),
 through the C ABI: the Sternheimer operator and CG at each k point, chi0(q)
 (lite and tight) and apply_operator at each q point, and the Davidson at each
-q point.  System: the H2 molecular crystal with s projectors on a 2x1x1 and a
-2x2x1 k grid (q = b1/2 folds k = 0 onto k - q = -b1/2 with g0 = -b1).  The
+q point.  System: the H2 molecular crystal with s projectors on a 2x1x1, a
+2x2x1 and a 3x1x1 k grid (q = b1/2 folds k = 0 onto k - q = -b1/2 with
+g0 = -b1; on the 3x1x1 grid k and q are not their own inverses, so the
+states are genuinely complex).  The
 runs use a system of their own, destroyed at the end.  Each case prints its
 name before it runs."""
 import numpy as np
@@ -15,7 +17,7 @@ def _rel(a, b):
     return float(np.abs(a - b).max() / np.abs(b).max())
 
 
-@pytest.fixture(scope="module", params=[(2, 1, 1), (2, 2, 1)], ids=["k211", "k221"])
+@pytest.fixture(scope="module", params=[(2, 1, 1), (2, 2, 1), (3, 1, 1)], ids=["k211", "k221", "k311"])
 def ksys(gpu, request):
     nk = request.param
     print(f"case kpoint {nk}: ground state", flush=True)

@@ -8,7 +8,8 @@ DFT), so agreement of the supercell's Gamma chi0 with the primitive cell's
 chi0(q) pins the whole q machinery: the k - q folding with its phase
 exp(-i g0.r) (k = 0, q = b1/2 folds with g0 = -b1), the full-sphere complex
 products, the k-point projectors, the energies at k - q, the weights 2 wk /
-Omega and the G = 0 rule.  The lite path (tr2 < 0) is not compared: its
+Omega and the G = 0 rule; on a 3x1x1 grid the same holds at q = b1/3 and
+2 b1/3, where the Bloch states are genuinely complex.  The lite path (tr2 < 0) is not compared: its
 preconditioner uses eprec of the states at k (dfpt_module.f90:211), which at
 Gamma is a half-sphere sum (Tools/set_eprec.f90), so only converged responses
 agree.
@@ -167,3 +168,38 @@ def test_randomize_q():
         assert np.all(a[:, n:] == 0)
         # distinct vectors
         assert abs(np.vdot(a[0], a[1])) < 0.9 * np.linalg.norm(a[0]) * np.linalg.norm(a[1])
+
+
+@pytest.mark.parametrize("iq", [1, 2])
+def test_chi0_q_complex_points_equal_supercell_gamma(iq):
+    """k and q points that are not their own inverses (the 3x1x1 grid,
+    q = b1/3 and 2 b1/3): Bloch states are genuinely complex and q != -q, so
+    a misplaced conjugate or a k + q in place of k - q would show.  A complex
+    perturbation f on the q sphere is the sector q of the real supercell
+    perturbation f e^{i(q+G)r} + c.c.; its chi0(q) f is the sector q of the
+    supercell response."""
+    prim = make_h2_crystal(8.0)
+    sup = make_h2_crystal(8.0, nrep=3)
+    kgs = make_k_ground_state(prim, (3, 1, 1), nbnd=4)
+    gs = make_ground_state(sup, lambda g: (lambda psi: h_psi(g, psi)), nbnd=8, tol=1e-12)
+    qb = kgs.qpts[iq].basis
+    lut = {tuple(x): i for i, x in enumerate(gs.basis.mill)}
+    rng = np.random.default_rng(3 + iq)
+    m = 2
+    fq = np.zeros((m, kgs.npwqx), complex)
+    fq[:, :qb.npw] = (rng.normal(size=(m, qb.npw)) + 1j * rng.normal(size=(m, qb.npw))) / (1 + qb.g2kin)
+    dvs = np.zeros((m, gs.npwx), complex)
+    own = {}
+    for ig, G in enumerate(qb.mill):
+        Gs = (3 * G[0] + iq, G[1], G[2])
+        if Gs in lut:
+            dvs[:, lut[Gs]] += fq[:, ig]
+            own[ig] = lut[Gs]
+        else:
+            dvs[:, lut[tuple(-np.array(Gs))]] += np.conj(fq[:, ig])
+    assert len(own) > 10
+    dn_s = dfpt(gs, dvs, 1e-22)
+    dn_q = K.dfpt_q(kgs, iq, fq, 1e-22)
+    ig = np.array(list(own.keys()))
+    js = np.array(list(own.values()))
+    assert np.abs(dn_s[:, js] - dn_q[:, ig]).max() < 1e-9 * np.abs(dn_s).max()
