@@ -92,6 +92,8 @@ def main():
                          "1 GiB read, 1 GiB written) to scale HBM byte counters of a profiler run")
     ap.add_argument("--no-davidson", action="store_true",
                     help="skip the Wstat Davidson wall-time runs (6 H2, 32 PDEP; Si64, 256 PDEP)")
+    ap.add_argument("--no-lanczos", action="store_true",
+                    help="skip the Wfreq Lanczos measurement (128 chains of 30 steps on the si64 ground state)")
     ap.add_argument("--no-qpoint", action="store_true",
                     help="skip the q-point chi0 measurement (H2 molecular crystal on a 3x3x3 k grid)")
     ap.add_argument("--gs-cache", default=None,
@@ -233,6 +235,12 @@ def main():
         if cpu is not None:
             cpu["davidson_6h2"] = _davidson_h2_cpu(h2, dav)
 
+    # ---- Wfreq's Lanczos chains on the same ground state, FP64, with the oracle beside it
+    lz = None
+    if not args.no_lanczos and args.workload == "si64":
+        wstat.set_ground_state(gs, sqvc=sq)
+        lz = _lanczos_measure(wstat, torch, args, gs, with_cpu=not args.no_cpu_baseline)
+
     # ---- chi0 at a q point (k-point ground state), FP64, with the oracle beside it
     qpt = None
     if not args.no_qpoint:
@@ -268,8 +276,59 @@ def main():
         "davidson": dav,
         "davidson_si64": dav64,
         "qpoint": qpt,
+        "lanczos": lz,
     }
     print(json.dumps(rec), flush=True)
+
+
+def _lanczos_measure(wstat, torch, args, gs, with_cpu=True, nrhs=128, nlsteps=30, ncpu=2):
+    """Wfreq's chi(i omega) Lanczos chains (solve_deflated_lanczos_w_full_ortho,
+    n_lanczos = 30 as WEST's default) for 128 right-hand sides on the
+    registered ground state, all occupied bands deflated: smooth seeded
+    vectors projected on the conduction space, the shape of the
+    P_c [psi_v dv_i] of solve_wfreq.  A unit is one Lanczos step of one chain
+    (an application of H, P_c, the recurrence and the re-orthogonalisation
+    against every earlier vector of the chain).  The first ``ncpu`` chains
+    are also run by the FP64 oracle on the host cores."""
+    from oracle.hamiltonian import apply_alpha_pc
+    from oracle.lanczos import solve_deflated_lanczos_w_full_ortho as lanczos_ref
+    from west_amd import wfreq
+    b = gs.basis
+    rng = np.random.default_rng(19)
+    v = np.zeros((nrhs, gs.npwx), complex)
+    v[:, :b.npw] = (rng.normal(size=(nrhs, b.npw)) + 1j * rng.normal(size=(nrhs, b.npw))) / (1.0 + b.g2kin)[None, :]
+    v[:, 0] = v[:, 0].real
+    rhs = apply_alpha_pc(gs, v, 1.0)
+    wfreq.solve_deflated_lanczos_w_full_ortho(gs.nbndval, rhs[:8], 4, want_basis=False)   # warm-up
+    torch.cuda.synchronize()
+    wstat.profile(True)
+    reps = 3
+    t = time.perf_counter()
+    for _ in range(reps):
+        a, bt, _, bn = wfreq.solve_deflated_lanczos_w_full_ortho(gs.nbndval, rhs, nlsteps, want_basis=False)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / reps
+    kern = {n: round(wstat.profile_query(n)[0] / reps, 3) for n in ("plane_vloc", "zinv", "zfwd", "gemm_bra",
+                                                                     "gemm_ket")}
+    wstat.profile(False)
+    rec = {"workload": f"Wfreq Lanczos on the si64 ground state: {nrhs} chains x {nlsteps} steps, "
+                       f"{gs.nbndval} bands deflated, npw={b.npw}",
+           "value": round(nrhs * nlsteps / dt, 1), "unit": "chain steps/s", "wall_s": round(dt, 4),
+           "kernels_ms": kern, "timed_runs": reps}
+    if with_cpu:
+        import oracle.fft_at_gamma as ofa
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        ofa.WORKERS = threads
+        tc = time.perf_counter()
+        ar, btr, _, bnr = lanczos_ref(gs, gs.nbndval, rhs[:ncpu], nlsteps)
+        tc = time.perf_counter() - tc
+        rec["alpha_max_rel_err_vs_oracle"] = float(np.abs(a[:ncpu] - ar).max() / np.abs(ar).max())
+        rec["beta_max_rel_err_vs_oracle"] = float(np.abs(bt[:ncpu] - btr).max() / np.abs(btr).max())
+        rec["cpu_baseline"] = {"value": round(ncpu * nlsteps / tc, 3), "unit": "chain steps/s", "cores": threads,
+                               "kind": "port", "sample": f"chains 1-{ncpu} of the batch, {nlsteps} steps, FP64 oracle, "
+                                                         f"{tc:.1f}s"}
+    log(f"[bench] lanczos: {rec['value']} chain steps/s ({dt:.3f}s for {nrhs}x{nlsteps})")
+    return rec
 
 
 def _qpoint_measure(wstat, torch, args, with_cpu=True, nk=(3, 3, 3), npert=16, ecut=25.0):
