@@ -15,7 +15,7 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 BUILD = HERE / "_build"
 LIB = HERE / "libwesthip.so"
-SOURCES = ["wh_fft.hip", "wh_fft_tpl.hip", "wh_fft_reg.hip", "wh_fft_reg64.hip", "wh_gemm.hip", "wh_gemm_bf16.hip", "wh_vec.hip", "wh_api.hip",
+SOURCES = ["wh_fft.hip", "wh_fft_tpl.hip", "wh_fft_reg.hip", "wh_fft_reg64.hip", "wh_fft_q4.hip", "wh_gemm.hip", "wh_gemm_bf16.hip", "wh_vec.hip", "wh_api.hip",
            "wh_davidson.hip", "wh_noalloc.hip", "wh_lanczos.hip", "wh_kpoint.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -29,7 +29,8 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
 # per-file additions: the register-line FFT kernels rely on arithmetic with
 # known zeros folding away (signed zeros, NaN and Inf never arise there)
 EXTRA = {"wh_fft_reg.hip": ["-fno-signed-zeros", "-ffinite-math-only"],
-         "wh_fft_reg64.hip": ["-fno-signed-zeros", "-ffinite-math-only"]}
+         "wh_fft_reg64.hip": ["-fno-signed-zeros", "-ffinite-math-only"],
+         "wh_fft_q4.hip": ["-fno-signed-zeros", "-ffinite-math-only"]}
 
 
 def _compile(src: str) -> Path:

@@ -19,6 +19,7 @@
 
 #include "wh_fft.h"
 #include "wh_fft_reg.h"
+#include "wh_fft_q4.h"
 #include "wh_fft_tpl.h"
 
 namespace wh {
@@ -723,15 +724,19 @@ void launch_plane_vloc(const StickPlan& p, const FftItem* d_items, int nitems, c
     // first of which also serves the untimed small launches (6-H2 Davidson,
     // 60^3: 235 ms against 240 ms with the LDS-pass kernel; 24 clusters,
     // concurrent: 1.35 s against 1.42 s)
+    // and then the quad-lane kernels
     const int max_occ = p.tpl->max_occ;
     const int nreg = plane_reg_variants(P, false);
+    const int nq4 = plane_q4_variants(P);
     plane_occ_launch(
-        p.vloc_occ, max_occ + nreg, nitems,
+        p.vloc_occ, max_occ + nreg + nq4, nitems,
         [&](int v) {
           if (v <= max_occ)
             p.tpl->vloc(P, d_items, nitems, sin, sout, vpot, ldpot, scale, v, s);
-          else
+          else if (v <= max_occ + nreg)
             plane_reg_vloc(v - max_occ - 1, P, d_items, nitems, sin, sout, vpot, ldpot, scale, s);
+          else
+            plane_q4_vloc(v - max_occ - nreg - 1, P, d_items, nitems, sin, sout, vpot, ldpot, scale, s);
         },
         s, nreg > 0 ? max_occ + 1 : 1);
     WH_HIP(hipGetLastError());

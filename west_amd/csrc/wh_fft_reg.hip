@@ -29,51 +29,6 @@ namespace wh {
 
 namespace {
 
-template <int... I, class F>
-__device__ __forceinline__ void sfor_seq(std::integer_sequence<int, I...>, F&& f) {
-  (f(std::integral_constant<int, I>{}), ...);
-}
-// f(integral_constant<int, i>) for i = 0 .. N-1, unrolled at compile time
-template <int N, class F>
-__device__ __forceinline__ void sfor(F&& f) {
-  sfor_seq(std::make_integer_sequence<int, N>{}, static_cast<F&&>(f));
-}
-
-// In-register transform of v[0..N): the Stockham passes of Plan<N, REV> with
-// every index and twiddle exp(DIR 2 pi i j k / (P R)) a compile-time constant.
-template <typename T, int N, int DIR, bool REV = false, int S = 0>
-__device__ __forceinline__ void fft_line(cplx<T>* v) {
-  using PL = Plan<N, REV>;
-  if constexpr (S < PL::nst) {
-    constexpr int R = PL::radix(S);
-    constexpr int P = PL::stride(S);
-    constexpr int NBL = N / R;
-    cplx<T> w[N];
-    sfor<NBL>([&](auto I) {
-      constexpr int i = decltype(I)::value;
-      constexpr int k = i % P;
-      cplx<T> b[R];
-      sfor<R>([&](auto J) {
-        constexpr int j = decltype(J)::value;
-        if constexpr (k != 0 && j != 0) {
-          constexpr T c = T(cos2pi(j * k, P * R));
-          constexpr T s = T(DIR * sin2pi(j * k, P * R));
-          b[j] = twiddle_mul(v[i + j * NBL], c, s);
-        } else {
-          b[j] = v[i + j * NBL];
-        }
-      });
-      dft<T, R, DIR>(b);
-      sfor<R>([&](auto J) {
-        constexpr int j = decltype(J)::value;
-        w[(i - k) * R + k + j * P] = b[j];
-      });
-    });
-    sfor<N>([&](auto I) { v[decltype(I)::value] = w[decltype(I)::value]; });
-    fft_line<T, N, DIR, REV, S + 1>(v);
-  }
-}
-
 // compile-time bound on imax the a1 pruning assumes (the smooth grid of a
 // norm-conserving calculation has N >= 4 imax + 1)
 template <int N>
