@@ -115,7 +115,10 @@ Solver& solver() {
 // (dsyevj, tolerance at machine precision).  On matrices of a few hundred
 // rows they differ several-fold in time; the first call in each size class
 // (n rounded up to 32) times all three on copies of its matrix and keeps the
-// fastest, and every call then runs that one.
+// fastest, and every call then runs that one.  Above kEigTimedMax rows
+// dsyevd runs untimed: QR iteration alone takes seconds there (3.5 s at
+// n = 1024), more than the timing could ever win back.
+constexpr int kEigTimedMax = 256;
 std::mutex g_eig_mutex;
 std::map<int, int>& eig_choice() {
   static std::map<int, int> m;
@@ -413,6 +416,7 @@ extern "C" int wh_davidson_diago(const wh_davidson_params* prm, double* ev_out, 
         std::lock_guard<std::mutex> lk(g_eig_mutex);
         const int cls = (n + 31) / 32;
         auto it = eig_choice().find(cls);
+        if (it == eig_choice().end() && n > kEigTimedMax) it = eig_choice().emplace(cls, 0).first;
         if (it == eig_choice().end()) {
           // time the three on copies (each run once untimed first: its workspace)
           float t[3] = {0.f, 0.f, 0.f};
