@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 # Measured HBM bytes per transform item (band pair) of the plane kernels on the
 # si64 workload: rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a streaming kernel
 # of known size (profiles/plane_vloc_si64_hbm_traffic.md).
-TRAFFIC_PER_ITEM = {"si64": {"plane_vloc": 1024257 + 974592, "plane_vloc32": 491706 + 487296}}
+TRAFFIC_PER_ITEM = {"si64": {"plane_vloc": 1028065 + 973824, "plane_vloc32": 491706 + 487296}}
 FP64_PEAK_TFLOPS = 78.6
 
 WORKLOADS = {

@@ -89,161 +89,9 @@ __device__ __forceinline__ void quad_twiddle(z64* v, const z64* tw) {
 
 __device__ __forceinline__ int rev2(int l) { return ((l & 1) << 1) | (l >> 1); }
 
-// swap of a double between the two halves of the wave (X = 32: lanes l and
-// l ^ 32 exchange the value of a in the upper half with b in the lower half)
-// or between neighbouring rows of 16 lanes (X = 16); afterwards every lane
-// holds in a the value of the pair's lower lane and in b that of its upper
-// lane (lower / upper: lane bit log2 X clear / set)
-template <int X>
-__device__ __forceinline__ void xswap(double& a, double& b) {
-  const long long ia = __double_as_longlong(a), ib = __double_as_longlong(b);
-  const unsigned alo = static_cast<unsigned>(ia), ahi = static_cast<unsigned>(ia >> 32);
-  const unsigned blo = static_cast<unsigned>(ib), bhi = static_cast<unsigned>(ib >> 32);
-  unsigned rlo0, rlo1, rhi0, rhi1;
-  if constexpr (X == 32) {
-    const auto lo = __builtin_amdgcn_permlane32_swap(alo, blo, false, false);
-    const auto hi = __builtin_amdgcn_permlane32_swap(ahi, bhi, false, false);
-    rlo0 = lo[0]; rlo1 = lo[1]; rhi0 = hi[0]; rhi1 = hi[1];
-  } else {
-    const auto lo = __builtin_amdgcn_permlane16_swap(alo, blo, false, false);
-    const auto hi = __builtin_amdgcn_permlane16_swap(ahi, bhi, false, false);
-    rlo0 = lo[0]; rlo1 = lo[1]; rhi0 = hi[0]; rhi1 = hi[1];
-  }
-  a = __longlong_as_double((static_cast<long long>(rhi0) << 32) | rlo0);
-  b = __longlong_as_double((static_cast<long long>(rhi1) << 32) | rlo1);
-}
-
-template <int X>
-__device__ __forceinline__ void xswap(z64& a, z64& b) {
-  xswap<X>(a.x, b.x);
-  xswap<X>(a.y, b.y);
-}
-
-// Where the four lanes of a line sit and how they combine.
-// Quad layout (ROWL false): lanes 4 g + l (l = 0..3), the length-4 DFT by
-// butterflies with DPP-exchanged partners; the transformed point k1 + M rev(l)
-// is in v[k1].
-// Row layout (ROWL true): lanes 16 w + g + 16 l of wave w (l = b0 + 2 b1 from
-// lane bits 4 and 5); each stage of the length-4 DFT first swaps registers
-// between the lanes of a pair (permlane swaps: one instruction moves a dword
-// each way), which turns the lane bit into a register index, and then does
-// the butterfly in registers, where the factor i is a free exchange of real
-// and imaginary parts.  With H = M / 2 and Q = H / 2:
-//   bit 5 (b1): registers (e, e + H) pair, e < H; afterwards register c H + j
-//     holds the c-th output (c = 0, 1) of element j + H b1;
-//   bit 4 (b0): registers (c H + j, c H + j + Q) pair, j < Q; afterwards the
-//     two hold the points of element j + Q b0 + H b1 with k2 = c and c + 2;
-//     for odd H the registers 2Q and H + 2Q pair with each other and hold
-//     element 2Q + H b1 with k2 = b0 + 2 c' (c' = 0 for register 2Q, 1 for
-//     H + 2Q), the factor i selected per lane;
-// the transformed point x = k1 + M k2 of register r is pos<r>().
-template <int M, bool ROWL>
-struct Lane4 {
-  static constexpr int H = M / 2;
-  static constexpr int Q = H / 2;
-  static constexpr bool ODD = (H % 2) != 0;
-  static_assert(!ROWL || M % 2 == 0, "row layout needs an even M");
-  int l;      // this lane's quarter of the line (0..3)
-  int off;    // position offset of the paired registers
-  int off2;   // position offset of the odd-H registers (row layout)
-
-  __device__ explicit Lane4(int tid) {
-    if constexpr (ROWL) {
-      l = (tid >> 4) & 3;
-      const int b0 = l & 1, b1 = l >> 1;
-      off = Q * b0 + H * b1;
-      off2 = H * b1 + M * b0;
-    } else {
-      l = tid & 3;
-      off = M * rev2(l);
-      off2 = off;
-    }
-  }
-  static __device__ int line(int tid) { return ROWL ? (tid >> 6) * 16 + (tid & 15) : tid >> 2; }
-
-  template <int R>
-  static constexpr int pos_const() {
-    if constexpr (!ROWL) {
-      return R;
-    } else {
-      constexpr int c = R / H, j = R % H;
-      if constexpr (j < 2 * Q) return j % Q + M * (c + 2 * (j >= Q ? 1 : 0));
-      else return 2 * Q + 2 * M * c;
-    }
-  }
-  template <int R>
-  __device__ int pos() const {
-    if constexpr (ROWL && ODD && R % H == 2 * Q) return pos_const<R>() + off2;
-    else return pos_const<R>() + off;
-  }
-
-  // length-4 DFT across the line's lanes: inverse (exp(+)) from lane l's
-  // register k1 to the points pos<r>; forward (exp(-)) back
-  __device__ void inv(z64* v) const {
-    if constexpr (!ROWL) {
-      quad_dft4<M, true>(v, l);
-    } else {
-      sfor<H>([&](auto E) {
-        constexpr int e = decltype(E)::value;
-        xswap<32>(v[e], v[e + H]);
-        const z64 t = v[e];
-        v[e] = t + v[e + H];
-        v[e + H] = t - v[e + H];
-      });
-      sfor<2 * Q>([&](auto E) {
-        constexpr int c = decltype(E)::value / Q, j = decltype(E)::value % Q;
-        constexpr int a = c * H + j, b = a + Q;
-        xswap<16>(v[a], v[b]);
-        const z64 t = c == 0 ? v[b] : mk(-v[b].y, v[b].x);
-        const z64 u = v[a];
-        v[a] = u + t;
-        v[b] = u - t;
-      });
-      if constexpr (ODD) {
-        constexpr int a = 2 * Q, b = H + 2 * Q;
-        xswap<16>(v[a], v[b]);
-        const bool rot = (l & 1) != 0;
-        const z64 t = mk(rot ? -v[b].y : v[b].x, rot ? v[b].x : v[b].y);
-        const z64 u = v[a];
-        v[a] = u + t;
-        v[b] = u - t;
-      }
-    }
-  }
-  __device__ void fwd(z64* v) const {
-    if constexpr (!ROWL) {
-      quad_dft4<M, false>(v, l);
-    } else {
-      sfor<2 * Q>([&](auto E) {
-        constexpr int c = decltype(E)::value / Q, j = decltype(E)::value % Q;
-        constexpr int a = c * H + j, b = a + Q;
-        const z64 d = v[a] - v[b];
-        v[a] = v[a] + v[b];
-        v[b] = c == 0 ? d : mk(d.y, -d.x);
-        xswap<16>(v[a], v[b]);
-      });
-      if constexpr (ODD) {
-        constexpr int a = 2 * Q, b = H + 2 * Q;
-        const z64 d = v[a] - v[b];
-        v[a] = v[a] + v[b];
-        const bool rot = (l & 1) != 0;
-        v[b] = mk(rot ? d.y : d.x, rot ? -d.x : d.y);
-        xswap<16>(v[a], v[b]);
-      }
-      sfor<H>([&](auto E) {
-        constexpr int e = decltype(E)::value;
-        const z64 d = v[e] - v[e + H];
-        v[e] = v[e] + v[e + H];
-        v[e + H] = d;
-        xswap<32>(v[e], v[e + H]);
-      });
-    }
-  }
-};
-
 // LDS rows of PZ planes: NR = 2 IMX + 1 rows (row IMX + i for a1 frequency i),
 // stride RS = N + 1 (odd: row-parallel accesses spread over the banks)
-template <int N, int PZ, int NTQ, int WPE, bool ROWL>
+template <int N, int PZ, int NTQ, int WPE>
 __global__ __launch_bounds__(NTQ) __attribute__((amdgpu_waves_per_eu(WPE))) void k_plane_vloc_q4(StickPlanDev P, const FftItem* items, const z64* sin,
                                                       z64* sout, const double* vpot, long ldpot, double scale) {
   constexpr int M = N / 4;
@@ -263,11 +111,11 @@ __global__ __launch_bounds__(NTQ) __attribute__((amdgpu_waves_per_eu(WPE))) void
   if (item.a < 0) return;   // inactive item (uniform per workgroup)
   const long sl = static_cast<long>(N) * nct;
   const int tid = threadIdx.x;
-  const Lane4<N / 4, ROWL> ln(tid);
-  const int l = ln.l;
-  const int quad = Lane4<N / 4, ROWL>::line(tid);
+  const int l = tid & 3;
+  const int quad = tid >> 2;
   constexpr int NQ = NTQ / 4;
-  static_assert(!ROWL || NTQ % 64 == 0, "row layout needs whole waves");
+  // after the inverse length-4 DFT register k holds the point k + off
+  const int off = M * rev2(l);
 
   // this lane's twiddle factors after the planes
   z64* tw = pl + PZ * PE;
@@ -304,8 +152,8 @@ __global__ __launch_bounds__(NTQ) __attribute__((amdgpu_waves_per_eu(WPE))) void
     });
     fft_line<double, M, +1>(v);
     quad_twiddle<M, +1>(v, twl);
-    ln.inv(v);
-    sfor<M>([&](auto Ki) { row[ln.template pos<decltype(Ki)::value>()] = v[decltype(Ki)::value]; });
+    quad_dft4<M, true>(v, l);
+    sfor<M>([&](auto Ki) { row[decltype(Ki)::value + off] = v[decltype(Ki)::value]; });
   }
   __syncthreads();
 
@@ -334,17 +182,17 @@ __global__ __launch_bounds__(NTQ) __attribute__((amdgpu_waves_per_eu(WPE))) void
       });
       fft_line<double, M, +1>(v);
       quad_twiddle<M, +1>(v, twl);
-      ln.inv(v);
-      // register k holds the point x = pos<k>; V(r) is loaded here, not
+      quad_dft4<M, true>(v, l);
+      // register k holds the point x = k + off; V(r) is loaded here, not
       // ahead of the inverse transform (its 2 M registers would be live
       // through it)
       __builtin_amdgcn_sched_barrier(0);
       const double* vr = V0 + (static_cast<long>(z0 + p) * N + y) * N;
       sfor<M>([&](auto Ki) {
         constexpr int k = decltype(Ki)::value;
-        v[k] = cscale(v[k], vr[ln.template pos<k>()] * sc);
+        v[k] = cscale(v[k], vr[k + off] * sc);
       });
-      ln.fwd(v);
+      quad_dft4<M, false>(v, l);
       quad_twiddle<M, -1>(v, twl);
       fft_line<double, M, -1>(v);
       // lane l holds frequencies i = l + 4 m; only |i| <= IMX are rows
@@ -370,8 +218,8 @@ __global__ __launch_bounds__(NTQ) __attribute__((amdgpu_waves_per_eu(WPE))) void
     const int p = q / nrows;
     z64* row = pl + p * PE + (r0 + q - p * nrows) * RS;
     z64 v[M];
-    sfor<M>([&](auto Ki) { v[decltype(Ki)::value] = row[ln.template pos<decltype(Ki)::value>()]; });
-    ln.fwd(v);
+    sfor<M>([&](auto Ki) { v[decltype(Ki)::value] = row[decltype(Ki)::value + off]; });
+    quad_dft4<M, false>(v, l);
     quad_twiddle<M, -1>(v, twl);
     fft_line<double, M, -1>(v);
     // every lane of the line has read its points of the row (one wave: the
@@ -401,7 +249,7 @@ size_t q4_smem(int n, int pz) {
 
 constexpr size_t kQ4MaxSmem = 160u * 1024u;
 
-template <int N, int PZ, int NTQ, int WPE, bool ROWL>
+template <int N, int PZ, int NTQ, int WPE>
 void launch_q4(const StickPlanDev& P, const FftItem* it, int n, const z64* si, z64* so, const double* v, long ldv,
                double scale, hipStream_t s) {
   static std::mutex mu;
@@ -409,18 +257,18 @@ void launch_q4(const StickPlanDev& P, const FftItem* it, int n, const z64* si, z
   {
     std::lock_guard<std::mutex> lk(mu);
     if (!set) {
-      WH_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_plane_vloc_q4<N, PZ, NTQ, WPE, ROWL>),
+      WH_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_plane_vloc_q4<N, PZ, NTQ, WPE>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kQ4MaxSmem)));
       set = true;
     }
   }
   const size_t sm = q4_smem(N, PZ);
-  k_plane_vloc_q4<N, PZ, NTQ, WPE, ROWL><<<dim3(ceil_div(N, PZ), n), NTQ, sm, s>>>(P, it, si, so, v, ldv, scale);
+  k_plane_vloc_q4<N, PZ, NTQ, WPE><<<dim3(ceil_div(N, PZ), n), NTQ, sm, s>>>(P, it, si, so, v, ldv, scale);
 }
 
-// 3 planes per workgroup: 768 threads in the quad layout, 1024 in the row
-// layout (4 waves per SIMD, 128 VGPRs)
-constexpr int kQ4Variants = 2;
+// 3 planes (24 workgroups) per item, 768 threads: one a1 column per quad and
+// plane in the first pass
+constexpr int kQ4Variants = 1;
 
 }  // namespace
 
@@ -435,10 +283,7 @@ void plane_q4_vloc(int v, const StickPlanDev& P, const FftItem* items, int n, co
                    const double* vpot, long ldpot, double scale, hipStream_t s) {
   WH_REQUIRE(v >= 0 && v < plane_q4_variants(P), "plane_q4_vloc: no such kernel for this grid");
   WH_REQUIRE(static_cast<const void*>(sin) != static_cast<const void*>(sout), "plane_q4_vloc: input and output alias");
-  switch (v) {
-    case 0: return launch_q4<72, 3, 768, 3, false>(P, items, n, sin, sout, vpot, ldpot, scale, s);
-    default: return launch_q4<72, 3, 1024, 4, true>(P, items, n, sin, sout, vpot, ldpot, scale, s);
-  }
+  launch_q4<72, 3, 768, 3>(P, items, n, sin, sout, vpot, ldpot, scale, s);
 }
 
 }  // namespace wh
