@@ -41,6 +41,7 @@ struct StickPlanDev {
   int imax;       // max |i| over columns
   int nrows;      // 2*imax + 1
   int kext;       // max |k| over the sphere
+  int jext;       // max |j| over the stored columns
   int zblk_pts;   // most points in kZregCols consecutive stored columns
   const int* col_off;   // (ncs+1)
   const int* col_kmin;  // (ncs)

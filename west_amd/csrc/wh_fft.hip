@@ -652,6 +652,8 @@ void StickPlan::build(int nr1, int nr2, int nr3, int npw, const int* nl, const i
   d.npw = npw; d.ncs = ncs; d.nct = nct; d.imax = imax; d.nrows = nrows;
   d.kext = 0;
   for (int s = 0; s < npw; ++s) d.kext = std::max(d.kext, std::abs(pt_k[s]));
+  d.jext = 0;
+  for (int c = 0; c < ncs; ++c) d.jext = std::max(d.jext, std::abs(col_j[c]));
   d.zblk_pts = 0;
   for (int c0 = 0; c0 < ncs; c0 += kZregCols)
     d.zblk_pts = std::max(d.zblk_pts, col_off[std::min(c0 + kZregCols, ncs)] - col_off[c0]);

@@ -275,7 +275,8 @@ constexpr int kQ4Variants = 1;
 int plane_q4_variants(const StickPlanDev& P) {
   if (P.n1 != P.n2 || P.n1 != P.n3) return 0;
   if (P.n1 != 72) return 0;
-  if (P.imax > (P.n1 + 3) / 4 || q4_smem(P.n1, 3) > kQ4MaxSmem) return 0;
+  // the a1 and a2 transforms take |i| and |j| beyond N/4 as known zeros
+  if (P.imax > (P.n1 + 3) / 4 || P.jext > (P.n2 + 3) / 4 || q4_smem(P.n1, 3) > kQ4MaxSmem) return 0;
   return kQ4Variants;
 }
 

@@ -9,8 +9,9 @@
 // transposes between the a2 and the a1 lines.  A workgroup takes PZ planes
 // of one item so that its PZ * N a1-lines fill whole wavefronts.
 //
-// The a1 transforms know at compile time which of their inputs are zero
-// (|i| > IMX, the sphere bound) and which outputs are dropped; this file is
+// The transforms know at compile time which of their inputs are zero
+// (|i| > IMX along a1, |j| > IMX along a2: the sphere bound) and which
+// outputs are dropped; this file is
 // compiled with -fno-signed-zeros -ffinite-math-only so that the arithmetic
 // on those known zeros folds away.  Neither flag changes any finite result.
 //
@@ -86,7 +87,12 @@ __global__ __launch_bounds__((nt_reg<N, PZ>())) __attribute__((amdgpu_waves_per_
     const int p = t / nrows;
     cplx<T>* row = pl + p * PE + (r0 + t - p * nrows) * RS;
     cplx<T> v[N];
-    sfor<N>([&](auto Y) { v[decltype(Y)::value] = row[decltype(Y)::value]; });
+    // |j| > IMX: known zeros (no stored column there)
+    sfor<N>([&](auto Y) {
+      constexpr int y = decltype(Y)::value;
+      if constexpr (y > IMX && y < N - IMX) v[y] = mk<T>(T(0), T(0));
+      else v[y] = row[y];
+    });
     fft_line<T, N, +1>(v);
     sfor<N>([&](auto Y) { row[decltype(Y)::value] = v[decltype(Y)::value]; });
   }
@@ -141,7 +147,11 @@ __global__ __launch_bounds__((nt_reg<N, PZ>())) __attribute__((amdgpu_waves_per_
     cplx<T> v[N];
     sfor<N>([&](auto Y) { v[decltype(Y)::value] = row[decltype(Y)::value]; });
     fft_line<T, N, -1>(v);
-    sfor<N>([&](auto Y) { row[decltype(Y)::value] = v[decltype(Y)::value]; });
+    // only |j| <= IMX reach a stick (the others are dropped)
+    sfor<N>([&](auto Y) {
+      constexpr int y = decltype(Y)::value;
+      if constexpr (!(y > IMX && y < N - IMX)) row[y] = v[y];
+    });
   }
   __syncthreads();
 
@@ -493,7 +503,8 @@ void zfwd_reg(const StickPlanDev& P, const FftItem* items, int n, const z32* sti
 
 int plane_reg_variants(const StickPlanDev& P, bool fp32) {
   if (P.n1 != P.n2 || P.n1 != P.n3 || !compiled(P.n1, fp32)) return 0;
-  if (P.imax > (P.n1 + 3) / 4) return 0;
+  // the a1 and a2 transforms take |i| and |j| beyond N/4 as known zeros
+  if (P.imax > (P.n1 + 3) / 4 || P.jext > (P.n2 + 3) / 4) return 0;
   const size_t elem = fp32 ? sizeof(z32) : sizeof(z64);
   int nv = 0;
   for (int v = 0; v < kNumPz; ++v)
