@@ -49,8 +49,9 @@ def test_operator_workload_grids(gpu, nrep, ecut, plane_variant, monkeypatch):
     (WH_PLANE_OCC: 1-3 the LDS-pass kernel sized for that many workgroups per
     CU, where compiled; above, the register-line kernels with 2, 3, 4 planes
     per workgroup where they are built (FP64: grids up to 64) and their LDS
-    fits, and on the 72^3 grid the quad-lane kernel (4); a number beyond the
-    last runs the last); None leaves the choice to the library."""
+    fits, and on the 72^3 grid the quad-lane kernels (4: rows bounded by
+    |i| <= 18; 5: by 16, where the sphere allows); a number beyond the last
+    runs the last); None leaves the choice to the library."""
     if plane_variant is not None:
         monkeypatch.setenv("WH_PLANE_OCC", str(plane_variant))
     from oracle.hamiltonian import apply_sternheimerop

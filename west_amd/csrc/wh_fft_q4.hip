@@ -91,11 +91,13 @@ __device__ __forceinline__ int rev2(int l) { return ((l & 1) << 1) | (l >> 1); }
 
 // LDS rows of PZ planes: NR = 2 IMX + 1 rows (row IMX + i for a1 frequency i),
 // stride RS = N + 1 (odd: row-parallel accesses spread over the banks)
-template <int N, int PZ, int NTQ, int WPE>
+// IMX: compile-time bound on the sphere's |i| and |j| (at most N/4; the
+// rows beyond the actual imax stay zero)
+template <int N, int IMX, int PZ, int NTQ, int WPE>
 __global__ __launch_bounds__(NTQ) __attribute__((amdgpu_waves_per_eu(WPE))) void k_plane_vloc_q4(StickPlanDev P, const FftItem* items, const z64* sin,
                                                       z64* sout, const double* vpot, long ldpot, double scale) {
   constexpr int M = N / 4;
-  constexpr int IMX = q4_imx<N>();
+  static_assert(IMX <= q4_imx<N>(), "the pruned lines need |i|, |j| <= N/4");
   constexpr int NR = 2 * IMX + 1;
   constexpr int RS = N + 1;
   constexpr int PE = NR * RS;
@@ -242,33 +244,36 @@ __global__ __launch_bounds__(NTQ) __attribute__((amdgpu_waves_per_eu(WPE))) void
 }
 
 // LDS bytes: the planes' rows and the twiddle table
-size_t q4_smem(int n, int pz) {
-  const int imx = (n + 3) / 4;
+size_t q4_smem(int n, int imx, int pz) {
   return sizeof(z64) * (static_cast<size_t>(pz) * (2 * imx + 1) * (n + 1) + n);
 }
 
 constexpr size_t kQ4MaxSmem = 160u * 1024u;
 
-template <int N, int PZ, int NTQ, int WPE>
+template <int N, int IMX, int PZ, int NTQ, int WPE>
 void launch_q4(const StickPlanDev& P, const FftItem* it, int n, const z64* si, z64* so, const double* v, long ldv,
                double scale, hipStream_t s) {
   static std::mutex mu;
   static bool set = false;
+  const size_t sm = q4_smem(N, IMX, PZ);
   {
     std::lock_guard<std::mutex> lk(mu);
     if (!set) {
-      WH_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_plane_vloc_q4<N, PZ, NTQ, WPE>),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kQ4MaxSmem)));
+      WH_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_plane_vloc_q4<N, IMX, PZ, NTQ, WPE>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(sm)));
       set = true;
     }
   }
-  const size_t sm = q4_smem(N, PZ);
-  k_plane_vloc_q4<N, PZ, NTQ, WPE><<<dim3(ceil_div(N, PZ), n), NTQ, sm, s>>>(P, it, si, so, v, ldv, scale);
+  k_plane_vloc_q4<N, IMX, PZ, NTQ, WPE><<<dim3(ceil_div(N, PZ), n), NTQ, sm, s>>>(P, it, si, so, v, ldv, scale);
 }
 
-// 3 planes (24 workgroups) per item, 768 threads: one a1 column per quad and
-// plane in the first pass
-constexpr int kQ4Variants = 1;
+// variants (sphere bound, planes per workgroup, threads): 0: (18, 3, 768),
+// one workgroup of 130 KB per CU, one a1 column per quad and plane in the
+// first pass; 1: (16, 3, 768), 4 fewer LDS rows per plane and more zeros
+// pruned, where the sphere spans |i|, |j| <= 16 (the Si64 cell).  Two
+// planes of 16-bounded rows (78 KB: two workgroups of 384 threads per CU)
+// measured 2.38 ms against 1.85 ms per 1,024 items and are not offered.
+bool q4_tight(const StickPlanDev& P) { return P.imax <= 16 && P.jext <= 16; }
 
 }  // namespace
 
@@ -276,15 +281,16 @@ int plane_q4_variants(const StickPlanDev& P) {
   if (P.n1 != P.n2 || P.n1 != P.n3) return 0;
   if (P.n1 != 72) return 0;
   // the a1 and a2 transforms take |i| and |j| beyond N/4 as known zeros
-  if (P.imax > (P.n1 + 3) / 4 || P.jext > (P.n2 + 3) / 4 || q4_smem(P.n1, 3) > kQ4MaxSmem) return 0;
-  return kQ4Variants;
+  if (P.imax > (P.n1 + 3) / 4 || P.jext > (P.n2 + 3) / 4 || q4_smem(P.n1, 18, 3) > kQ4MaxSmem) return 0;
+  return q4_tight(P) ? 2 : 1;
 }
 
 void plane_q4_vloc(int v, const StickPlanDev& P, const FftItem* items, int n, const z64* sin, z64* sout,
                    const double* vpot, long ldpot, double scale, hipStream_t s) {
   WH_REQUIRE(v >= 0 && v < plane_q4_variants(P), "plane_q4_vloc: no such kernel for this grid");
   WH_REQUIRE(static_cast<const void*>(sin) != static_cast<const void*>(sout), "plane_q4_vloc: input and output alias");
-  launch_q4<72, 3, 768, 3>(P, items, n, sin, sout, vpot, ldpot, scale, s);
+  if (v == 0) launch_q4<72, 18, 3, 768, 3>(P, items, n, sin, sout, vpot, ldpot, scale, s);
+  else launch_q4<72, 16, 3, 768, 3>(P, items, n, sin, sout, vpot, ldpot, scale, s);
 }
 
 }  // namespace wh
