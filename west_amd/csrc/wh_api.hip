@@ -100,7 +100,8 @@ void require_ready(bool need_states) {
 int Context::items_capacity() const {
   const long per = 2L * sticks_bytes();
   long cap = static_cast<long>(sticks_budget) / std::max<long>(per, 1);
-  return static_cast<int>(std::max<long>(1, std::min<long>(cap, 1L << 16)));
+  // items are the grid's y dimension in the transform launches: at most 65535
+  return static_cast<int>(std::max<long>(1, std::min<long>(cap, 65535L)));
 }
 
 // ---------------------------------------------------------------------------
