@@ -40,7 +40,7 @@ def test_sternheimer_operator(gpu, key, m, request):
     assert _rel(out, ref) < 1e-12
 
 
-@pytest.mark.parametrize("plane_variant", [None, 1, 2, 4, 5, 6])
+@pytest.mark.parametrize("plane_variant", [None, 1, 2, 4, 5, 6, 9])
 @pytest.mark.parametrize("nrep,ecut", [(1, 25.0), (1, 70.0), (2, 25.0)])
 def test_operator_workload_grids(gpu, nrep, ecut, plane_variant, monkeypatch):
     """The operator on the workload grids (36^3, 60^3 and 72^3; the plane kernel
@@ -49,9 +49,10 @@ def test_operator_workload_grids(gpu, nrep, ecut, plane_variant, monkeypatch):
     (WH_PLANE_OCC: 1-3 the LDS-pass kernel sized for that many workgroups per
     CU, where compiled; above, the register-line kernels with 2, 3, 4 planes
     per workgroup where they are built (FP64: grids up to 64) and their LDS
-    fits, and on the 72^3 grid the quad-lane kernels (4: rows bounded by
-    |i| <= 18; 5: by 16, where the sphere allows); a number beyond the last
-    runs the last); None leaves the choice to the library."""
+    fits, and last the quad-lane kernels (72^3: 4 with rows bounded by
+    |i| <= 18, 5 by 16 where the sphere allows; 60^3: after the
+    register-line kernels); a number beyond the last runs the last, so 9 is
+    the last variant of each grid); None leaves the choice to the library."""
     if plane_variant is not None:
         monkeypatch.setenv("WH_PLANE_OCC", str(plane_variant))
     from oracle.hamiltonian import apply_sternheimerop

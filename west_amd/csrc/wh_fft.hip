@@ -722,14 +722,16 @@ void launch_plane_vloc(const StickPlan& p, const FftItem* d_items, int nitems, c
   if (p.tpl) {
     WH_REQUIRE(static_cast<const void*>(sin) != static_cast<const void*>(sout), "plane_vloc: input and output alias");
     // variants 1..max_occ: LDS-pass kernel sized for that many workgroups per
-    // CU; above: register-line kernels (2, 3, 4 planes per workgroup), the
-    // first of which also serves the untimed small launches (6-H2 Davidson,
-    // 60^3: 235 ms against 240 ms with the LDS-pass kernel; 24 clusters,
-    // concurrent: 1.35 s against 1.42 s)
-    // and then the quad-lane kernels
+    // CU; above: register-line kernels (2, 3, 4 planes per workgroup), then
+    // the quad-lane kernels.  The untimed small launches take the first
+    // quad-lane kernel where the grid has one (6-H2 Davidson, 60^3: 0.226 s
+    // against 0.244 s with the register-line kernel), else the first
+    // register-line kernel (235 ms against 240 ms with the LDS-pass kernel;
+    // 24 clusters, concurrent: 1.35 s against 1.42 s)
     const int max_occ = p.tpl->max_occ;
     const int nreg = plane_reg_variants(P, false);
     const int nq4 = plane_q4_variants(P);
+    const int untimed = nq4 > 0 ? max_occ + nreg + 1 : (nreg > 0 ? max_occ + 1 : 1);
     plane_occ_launch(
         p.vloc_occ, max_occ + nreg + nq4, nitems,
         [&](int v) {
@@ -740,7 +742,7 @@ void launch_plane_vloc(const StickPlan& p, const FftItem* d_items, int nitems, c
           else
             plane_q4_vloc(v - max_occ - nreg - 1, P, d_items, nitems, sin, sout, vpot, ldpot, scale, s);
         },
-        s, nreg > 0 ? max_occ + 1 : 1);
+        s, untimed);
     WH_HIP(hipGetLastError());
     return;
   }
