@@ -279,10 +279,10 @@ bool q4_tight(const StickPlanDev& P) { return P.imax <= 16 && P.jext <= 16; }
 
 int plane_q4_variants(const StickPlanDev& P) {
   if (P.n1 != P.n2 || P.n1 != P.n3) return 0;
-  if (P.n1 != 72 && P.n1 != 60) return 0;
+  if (P.n1 != 72 && P.n1 != 60 && P.n1 != 100) return 0;
   // the a1 and a2 transforms take |i| and |j| beyond N/4 as known zeros
   const int imx = (P.n1 + 3) / 4;
-  if (P.imax > imx || P.jext > imx || q4_smem(P.n1, imx, 3) > kQ4MaxSmem) return 0;
+  if (P.imax > imx || P.jext > imx || q4_smem(P.n1, imx, P.n1 == 100 ? 1 : 3) > kQ4MaxSmem) return 0;
   return (P.n1 == 72 && q4_tight(P)) ? 2 : 1;
 }
 
@@ -291,6 +291,7 @@ void plane_q4_vloc(int v, const StickPlanDev& P, const FftItem* items, int n, co
   WH_REQUIRE(v >= 0 && v < plane_q4_variants(P), "plane_q4_vloc: no such kernel for this grid");
   WH_REQUIRE(static_cast<const void*>(sin) != static_cast<const void*>(sout), "plane_q4_vloc: input and output alias");
   if (P.n1 == 60) launch_q4<60, 15, 3, 768, 3>(P, items, n, sin, sout, vpot, ldpot, scale, s);
+  else if (P.n1 == 100) launch_q4<100, 25, 1, 448, 2>(P, items, n, sin, sout, vpot, ldpot, scale, s);
   else if (v == 0) launch_q4<72, 18, 3, 768, 3>(P, items, n, sin, sout, vpot, ldpot, scale, s);
   else launch_q4<72, 16, 3, 768, 3>(P, items, n, sin, sout, vpot, ldpot, scale, s);
 }
