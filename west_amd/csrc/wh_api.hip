@@ -59,12 +59,14 @@ static Context* new_context(int device) {
   c->device = device;
   WH_HIP(hipSetDevice(device));
   WH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  WH_HIP(hipEventCreateWithFlags(&c->ev_caller, hipEventDisableTiming));
   return c;
 }
 
 static void free_context(Context* cp) {
   Context& c = *cp;
   (void)hipStreamSynchronize(c.stream);
+  if (c.ev_caller) (void)hipEventDestroy(c.ev_caller);
   c.prof.flush();
   for (auto e : c.prof.pool) (void)hipEventDestroy(e);
   c.plan.release();
@@ -153,13 +155,19 @@ static bool on_device(const void* p) {
 
 bool on_device_mem(const void* p) { return on_device(p); }
 
+void wait_for_caller(Context& c) {
+  WH_HIP(hipEventRecord(c.ev_caller, nullptr));
+  WH_HIP(hipStreamWaitEvent(c.stream, c.ev_caller, 0));
+}
+
 // device view of an input array (copied if it is host memory)
 template <typename T>
 static const T* dev_in(const T* p, size_t count, DevBuf& tmp) {
   if (on_device(p)) {
-    // the caller may have produced the array on any stream (the library's own
-    // stream does not synchronise with the null stream): wait for the device
-    WH_HIP(hipDeviceSynchronize());
+    // the library's stream does not synchronise with the null stream by
+    // itself: order it after the caller's null-stream work (a device array
+    // produced on another stream must be complete when the call is made)
+    wait_for_caller(ctx());
     return p;
   }
   T* d = tmp.as<T>(count);

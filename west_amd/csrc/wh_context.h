@@ -89,6 +89,7 @@ struct KPath;   // the q/k-point path's state (wh_kpoint.hip)
 struct Context {
   int device = -1;
   hipStream_t stream = nullptr;
+  hipEvent_t ev_caller = nullptr;   // orders the stream after the caller's null-stream work
   // grid / sphere
   bool have_grid = false;
   int nr1 = 0, nr2 = 0, nr3 = 0, nnr = 0, npw = 0, npwx = 0, gstart = 2;
@@ -167,6 +168,10 @@ struct Context {
 };
 
 Context& ctx();
+// Make the current system's stream wait for the work the caller queued on the
+// null (default) stream, e.g. the kernels that produced a device array it
+// passes in; neither the host nor other streams wait.
+void wait_for_caller(Context& c);
 // the q/k-point path (wh_kpoint.hip): state release, and the C-ABI hot-path
 // entries when a k grid is registered in the current system
 void free_kpath(KPath* k);

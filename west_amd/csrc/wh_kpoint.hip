@@ -382,7 +382,7 @@ double now_s() {
 template <typename T>
 const T* dev_in(const T* p, size_t count, DevBuf& tmp) {
   if (on_device_mem(p)) {
-    WH_HIP(hipDeviceSynchronize());
+    wait_for_caller(ctx());
     return p;
   }
   T* d = tmp.as<T>(count);

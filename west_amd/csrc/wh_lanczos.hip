@@ -154,7 +154,7 @@ extern "C" int wh_solve_deflated_lanczos_w_full_ortho(int nbnd_to_deflate, int n
       const size_t cin = static_cast<size_t>(nrhs) * c.npwx;
       const z64* bin = nullptr;
       if (on_device_mem(b)) {
-        WH_HIP(hipDeviceSynchronize());
+        wait_for_caller(c);
         bin = reinterpret_cast<const z64*>(b);
       } else {
         z64* t = c.w_in.as<z64>(cin);
@@ -243,7 +243,7 @@ extern "C" int wh_glbrak_gamma(int na, const double* a, int nb, const double* b,
       const size_t cin = static_cast<size_t>(n) * c.npwx;
       const z64* in = nullptr;
       if (on_device_mem(p)) {
-        WH_HIP(hipDeviceSynchronize());
+        wait_for_caller(c);
         in = reinterpret_cast<const z64*>(p);
       } else {
         z64* t = tmp.as<z64>(cin);
