@@ -414,6 +414,9 @@ def _qpoint_measure(wstat, torch, args, with_cpu=True, nk=(3, 3, 3), npert=16, e
 # alone, free of the Davidson convergence error.
 DAV_DEFAULT = {"tr2_dfpt": 1e-12}
 DAV_CONVERGED = {"tr2_dfpt": 1e-20, "trev_pdep": 1e-9, "trev_pdep_rel": 1e-9, "n_pdep_maxiter": 200}
+# the loosest of the settings tried whose Si64 PDEP eigenvalues stay within 1e-6 (relative) of the
+# converged FP64 ones in both precisions (trev 1e-7 or tr2_dfpt 1e-16 with trev 1e-8: 4e-6 to 7e-6)
+DAV_1E6 = {"tr2_dfpt": 1e-18, "trev_pdep": 1e-8, "trev_pdep_rel": 1e-8, "n_pdep_maxiter": 100}
 
 
 def _max_rel(a, b):
@@ -518,14 +521,16 @@ def _davidson_si64(wstat, torch, args, system, gs, npdep=256):
 
 def _davidson_si64_measure(wstat, torch, args, npdep=256):
     """Wstat Davidson for 256 PDEP of the 64-atom Si cell (n_pdep_times 4,
-    nvecx 1024), FP64 and mixed, with the default and the converged settings;
-    eigenvalue errors against the converged FP64 run."""
+    nvecx 1024), FP64 and mixed, with the default, the converged and the
+    1e-6 settings (the metric's eigenvalue accuracy); eigenvalue errors
+    against the converged FP64 run."""
     res, evs = {}, {}
     for mode in ("fp64", args.mixed_mode):
         wstat.set_precision(mode)
         key = "fp64" if mode == "fp64" else "mixed"
         res[key] = {}
-        for name, setting in (("default", DAV_DEFAULT), ("converged_setting", dict(DAV_CONVERGED, n_pdep_maxiter=100))):
+        for name, setting in (("default", DAV_DEFAULT), ("converged_setting", dict(DAV_CONVERGED, n_pdep_maxiter=100)),
+                              ("setting_1e-6", DAV_1E6)):
             log(f"[bench] si64 davidson {mode} {name} ...")
             torch.cuda.synchronize()
             c0 = wstat.counters()
@@ -543,6 +548,8 @@ def _davidson_si64_measure(wstat, torch, args, npdep=256):
     ref = evs["fp64", "converged_setting"]
     for key in ("fp64", "mixed"):
         res[key]["default"]["ev_max_rel_err_vs_converged_fp64"] = _max_rel(evs[key, "default"], ref)
+        res[key]["setting_1e-6"]["ev_max_rel_err_vs_converged_fp64"] = _max_rel(evs[key, "setting_1e-6"], ref)
+        res[key]["setting_1e-6"]["settings"] = DAV_1E6
     mixed_err = _max_rel(evs["mixed", "converged_setting"], ref)
     return {"fp64": res["fp64"],
             "mixed_precision": dict(res["mixed"], mode=args.mixed_mode,
