@@ -16,7 +16,7 @@ CSRC = HERE / "csrc"
 BUILD = HERE / "_build"
 LIB = HERE / "libwesthip.so"
 SOURCES = ["wh_fft.hip", "wh_fft_tpl.hip", "wh_fft_reg.hip", "wh_fft_reg64.hip", "wh_fft_q4.hip", "wh_gemm.hip", "wh_gemm_bf16.hip", "wh_vec.hip", "wh_api.hip",
-           "wh_davidson.hip", "wh_noalloc.hip", "wh_lanczos.hip", "wh_kpoint.hip"]
+           "wh_davidson.hip", "wh_noalloc.hip", "wh_lanczos.hip", "wh_kpoint.hip", "wh_kfft_lds.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("WH_OFFLOAD_ARCH", "gfx950")

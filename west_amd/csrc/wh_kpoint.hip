@@ -25,6 +25,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -423,7 +424,15 @@ static void stern_k(KPath& K, int ik, const z64* x, z64* t, const double* d_e, i
   const long ld = K.npwx;
   const long nnr = K.fft.nnr();
   const int gc = grid_cols(K);
-  for (int c0 = 0; c0 < m; c0 += gc) {
+  // small grids: the whole local term in one launch with the grid in LDS
+  // (WH_KLDS=0: the full-grid transforms through HBM)
+  static const bool klds_env = [] {
+    const char* ev = std::getenv("WH_KLDS");
+    return !(ev && ev[0] == '0');
+  }();
+  const bool lds = klds_env && !c.l_kinetic_only && kvloc_lds_available(K.fft.n1, K.fft.n2, K.fft.n3);
+  if (lds) WH_TIMED("kfft", kvloc_lds(K.fft.n1, K.fft.n2, K.fft.n3, m, P.npw, ld, P.nl, x, K.vrs, P.g2kin, d_e, t, s));
+  for (int c0 = 0; c0 < (lds ? 0 : m); c0 += gc) {
     const int mc = std::min(gc, m - c0);
     const z64* xc = x + c0 * ld;
     z64* tc = t + c0 * ld;
