@@ -369,11 +369,16 @@ def _qpoint_measure(wstat, torch, args, with_cpu=True, nk=(3, 3, 3), npert=16, e
             wstat.dfpt(npert, dvg, args.tr2, iq=iq + 1, dng=dng)
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t) / steps
-        kern = {n: wstat.profile_query(n) for n in ("kfft", "gemm_bra", "gemm_ket")}
+        kern = {n: wstat.profile_query(n) for n in ("kfft", "kvloc_lds", "gemm_bra", "gemm_ket")}
         op_cols = (wstat.counters()[0] - cols0) / steps
         wstat.profile(False)
         pairs = npert * nb * nks
-        fft_cols = npert + nks * (nb + 2 * npert * nb + npert) + 2 * op_cols
+        # grid columns through the full-grid transforms in HBM (three passes,
+        # read + write each); the operator's own columns take the LDS-resident
+        # kernel when the grid allows it (timed as kvloc_lds) and are then not
+        # among them
+        lds_op = kern["kvloc_lds"][1] > 0
+        fft_cols = npert + nks * (nb + 2 * npert * nb + npert) + (0 if lds_op else 2 * op_cols)
         fft_bytes = fft_cols * 6 * 16 * nnr
         kfft_ms = kern["kfft"][0] / steps
         rec = {"workload": f"H2 molecular crystal (2 H2, 6.5x6.0x5.5 bohr triclinic, s projectors), {ecut:g} Ry, "

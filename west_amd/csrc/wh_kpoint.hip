@@ -431,7 +431,7 @@ static void stern_k(KPath& K, int ik, const z64* x, z64* t, const double* d_e, i
     return !(ev && ev[0] == '0');
   }();
   const bool lds = klds_env && !c.l_kinetic_only && kvloc_lds_available(K.fft.n1, K.fft.n2, K.fft.n3);
-  if (lds) WH_TIMED("kfft", kvloc_lds(K.fft.n1, K.fft.n2, K.fft.n3, m, P.npw, ld, P.nl, x, K.vrs, P.g2kin, d_e, t, s));
+  if (lds) WH_TIMED("kvloc_lds", kvloc_lds(K.fft.n1, K.fft.n2, K.fft.n3, m, P.npw, ld, P.nl, x, K.vrs, P.g2kin, d_e, t, s));
   for (int c0 = 0; c0 < (lds ? 0 : m); c0 += gc) {
     const int mc = std::min(gc, m - c0);
     const z64* xc = x + c0 * ld;
