@@ -159,10 +159,11 @@ struct GridFft {
 void launch_grid_fft(const GridFft& g, z64* data, int batch, int dir, hipStream_t s);
 // t[b] = (1/N_r) nl-gather(F^-1 V(r) F nl-scatter(x[b])) + (g2 - e[b]) x[b] for m bands (rows of
 // length ld, npw points, zero beyond) in one launch with the grid in LDS, where every grid
-// dimension has a compiled line transform and the grid fits the LDS (wh_kfft_lds.hip)
+// dimension has a compiled line transform and the grid fits the LDS (wh_kfft_lds.hip);
+// d_m (device int, may be null): only the first min(m, *d_m) bands
 bool kvloc_lds_available(int n1, int n2, int n3);
 void kvloc_lds(int n1, int n2, int n3, int m, int npw, long ld, const int* nl, const z64* x, const double* vrs,
-               const double* g2, const double* e, z64* t, hipStream_t s);
+               const double* g2, const double* e, z64* t, hipStream_t s, const int* d_m = nullptr);
 
 // kernel launchers (double precision)
 void launch_zinv(const StickPlan& p, const FftItem* d_items, int nitems, const z64* psi, long ldpsi,

@@ -84,11 +84,13 @@ __device__ __forceinline__ int kl_at(int i1, int i2, int i3, int p1, int n2) { r
 __global__ __launch_bounds__(KL_NT) void k_kvloc_lds(int n1, int n2, int n3, int p1, int npw, long ld,
                                                    const int* __restrict__ nl, const z64* __restrict__ x,
                                                    const double* __restrict__ vrs, const double* __restrict__ g2,
-                                                   const double* __restrict__ e, z64* __restrict__ t) {
+                                                   const double* __restrict__ e, z64* __restrict__ t,
+                                                   const int* __restrict__ d_m) {
   extern __shared__ char smem_raw[];
   z64* grid = reinterpret_cast<z64*>(smem_raw);
   const int tid = threadIdx.x;
   const int b = blockIdx.x;
+  if (d_m && b >= *d_m) return;   // a band the device switched off (whole workgroup)
   const int np = p1 * n2 * n3;
   const z64* xb = x + b * ld;
   for (int i = tid; i < np; i += KL_NT) grid[i] = mk(0.0, 0.0);
@@ -147,7 +149,7 @@ bool kvloc_lds_available(int n1, int n2, int n3) {
 }
 
 void kvloc_lds(int n1, int n2, int n3, int m, int npw, long ld, const int* nl, const z64* x, const double* vrs,
-               const double* g2, const double* e, z64* t, hipStream_t s) {
+               const double* g2, const double* e, z64* t, hipStream_t s, const int* d_m) {
   WH_REQUIRE(kvloc_lds_available(n1, n2, n3), "kvloc_lds: grid not supported");
   if (m <= 0) return;
   static const bool attr_set = [] {   // once per process (thread-safe static initialisation)
@@ -156,7 +158,7 @@ void kvloc_lds(int n1, int n2, int n3, int m, int npw, long ld, const int* nl, c
     return true;
   }();
   (void)attr_set;
-  k_kvloc_lds<<<m, KL_NT, kl_smem(n1, n2, n3), s>>>(n1, n2, n3, n1 | 1, npw, ld, nl, x, vrs, g2, e, t);
+  k_kvloc_lds<<<m, KL_NT, kl_smem(n1, n2, n3), s>>>(n1, n2, n3, n1 | 1, npw, ld, nl, x, vrs, g2, e, t, d_m);
   WH_HIP(hipGetLastError());
 }
 

@@ -33,6 +33,7 @@
 #include "wh_context.h"
 #include "wh_fft.h"
 #include "wh_gemm.h"
+#include "wh_vec.h"
 
 namespace wh {
 
@@ -74,7 +75,7 @@ struct KPath {
   std::vector<void*> allocs;
   DevBuf g_a, g_b, g_c, g_aux;                       // grids
   DevBuf w_x, w_t, w_g, w_h, w_hold, w_bc, w_tc;     // CG vectors
-  DevBuf w_c2, w_l2i, w_rho, w_dots, w_beta, w_eu, w_ecol, w_epcol;
+  DevBuf w_c2, w_l2i, w_rho, w_rhoold, w_conv, w_dots, w_beta, w_eu, w_ecol, w_epcol;
   DevBuf w_dv, w_dn, w_in, w_out, w_in2;
   DevBuf dav[8];
 };
@@ -84,8 +85,8 @@ void free_kpath(KPath* k) {
   for (void* p : k->allocs) (void)hipFree(p);
   k->fft.release();
   for (DevBuf* b : {&k->g_a, &k->g_b, &k->g_c, &k->g_aux, &k->w_x, &k->w_t, &k->w_g, &k->w_h, &k->w_hold, &k->w_bc,
-                    &k->w_tc, &k->w_c2, &k->w_l2i, &k->w_rho, &k->w_dots, &k->w_beta, &k->w_eu, &k->w_ecol,
-                    &k->w_epcol, &k->w_dv, &k->w_dn, &k->w_in, &k->w_out, &k->w_in2})
+                    &k->w_tc, &k->w_c2, &k->w_l2i, &k->w_rho, &k->w_rhoold, &k->w_conv, &k->w_dots, &k->w_beta,
+                    &k->w_eu, &k->w_ecol, &k->w_epcol, &k->w_dv, &k->w_dn, &k->w_in, &k->w_out, &k->w_in2})
     b->release();
   for (DevBuf& b : k->dav) b.release();
   delete k;
@@ -239,12 +240,17 @@ __global__ void k_precond(int m, int npw, long ld, const double* __restrict__ g2
 
 // ---- CG at a k point (linsolve_sternheimer_m_wfcts.f90, non-Gamma products)
 
-// h = g / max(1, g2kin/eprec) on the listed columns, rho = Re <h|g> (150-160, 183-205)
-__global__ __launch_bounds__(NT) void k_cg_prec_rho(const int* __restrict__ l2i, int npw, long ld,
-                                                    const double* __restrict__ g2, const double* __restrict__ ep,
-                                                    const z64* __restrict__ g, z64* __restrict__ h,
-                                                    double* __restrict__ rho) {
+// The iteration's kernels are launched for a capacity of listed columns and
+// skip those at or beyond the device's active count *nc (a count set by
+// cg_select on the device; nc null: all launched columns).
+
+// h = g / max(1, g2kin/eprec) on the listed columns, rho[band] = Re <h|g> (150-160, 183-205)
+__global__ __launch_bounds__(NT) void k_cg_prec_rho(const int* __restrict__ l2i, const int* __restrict__ nc,
+                                                    int npw, long ld, const double* __restrict__ g2,
+                                                    const double* __restrict__ ep, const z64* __restrict__ g,
+                                                    z64* __restrict__ h, double* __restrict__ rho) {
   __shared__ double red[NT / 64];
+  if (nc && static_cast<int>(blockIdx.x) >= *nc) return;   // whole workgroup
   const int col = l2i[blockIdx.x];
   const z64* gc = g + col * ld;
   z64* hc = h + col * ld;
@@ -260,14 +266,14 @@ __global__ __launch_bounds__(NT) void k_cg_prec_rho(const int* __restrict__ l2i,
     }
   }
   s = block_sum(s, red);
-  if (threadIdx.x == 0) rho[blockIdx.x] = s;
+  if (threadIdx.x == 0) rho[col] = s;
 }
 
 // h = -h + beta hold on the listed columns; bc[lbnd] = h   (226-240)
-__global__ void k_cg_dir(int nc, const int* __restrict__ l2i, long ld, const double* __restrict__ beta,
-                         const double* __restrict__ e, z64* __restrict__ h, const z64* __restrict__ hold,
-                         z64* __restrict__ bc, double* __restrict__ eu) {
-  const long n = static_cast<long>(nc) * ld;
+__global__ void k_cg_dir(int cap, const int* __restrict__ nc, const int* __restrict__ l2i, long ld,
+                         const double* __restrict__ beta, const double* __restrict__ e, z64* __restrict__ h,
+                         const z64* __restrict__ hold, z64* __restrict__ bc, double* __restrict__ eu) {
+  const long n = static_cast<long>(min(cap, *nc)) * ld;
   for (long t = blockIdx.x * static_cast<long>(NT) + threadIdx.x; t < n; t += static_cast<long>(gridDim.x) * NT) {
     const int lb = static_cast<int>(t / ld);
     const long ig = t - static_cast<long>(lb) * ld;
@@ -280,11 +286,12 @@ __global__ void k_cg_dir(int nc, const int* __restrict__ l2i, long ld, const dou
 }
 
 // a = Re <h|g>, c = Re <h|t>   (277-305)
-__global__ __launch_bounds__(NT) void k_cg_dots(const int* __restrict__ l2i, int npw, long ld,
-                                                const z64* __restrict__ h, const z64* __restrict__ g,
+__global__ __launch_bounds__(NT) void k_cg_dots(const int* __restrict__ l2i, const int* __restrict__ nc, int npw,
+                                                long ld, const z64* __restrict__ h, const z64* __restrict__ g,
                                                 const z64* __restrict__ tc, double* __restrict__ ac) {
   __shared__ double red[NT / 64];
   const int lb = blockIdx.x;
+  if (lb >= *nc) return;   // whole workgroup
   const int col = l2i[lb];
   const z64* hc = h + col * ld;
   const z64* gc = g + col * ld;
@@ -303,10 +310,10 @@ __global__ __launch_bounds__(NT) void k_cg_dots(const int* __restrict__ l2i, int
 }
 
 // x -= a/c h ; g -= a/c t ; hold = h   (313-321)
-__global__ void k_cg_upd(int nc, const int* __restrict__ l2i, long ld, const double* __restrict__ ac,
-                         z64* __restrict__ x, z64* __restrict__ g, const z64* __restrict__ h,
-                         const z64* __restrict__ tc, z64* __restrict__ hold) {
-  const long n = static_cast<long>(nc) * ld;
+__global__ void k_cg_upd(int cap, const int* __restrict__ nc, const int* __restrict__ l2i, long ld,
+                         const double* __restrict__ ac, z64* __restrict__ x, z64* __restrict__ g,
+                         const z64* __restrict__ h, const z64* __restrict__ tc, z64* __restrict__ hold) {
+  const long n = static_cast<long>(min(cap, *nc)) * ld;
   for (long t = blockIdx.x * static_cast<long>(NT) + threadIdx.x; t < n; t += static_cast<long>(gridDim.x) * NT) {
     const int lb = static_cast<int>(t / ld);
     const long ig = t - static_cast<long>(lb) * ld;
@@ -417,7 +424,10 @@ void set_qscale2(KPath& K, double alpha) {
 // t = H_k x - e x + alpha P_v x  (apply_sternheimerop_to_m_wfcs.f90:49-76 at k:
 // QE h_psi_ = |k+G|^2 + vloc_psi_k + calbec/add_vuspsi, then alpha P_v)
 // ---------------------------------------------------------------------------
-static void stern_k(KPath& K, int ik, const z64* x, z64* t, const double* d_e, int m, double alpha) {
+// d_nc (device int, may be null): only the first min(m, *d_nc) columns are
+// used (a CG launch sized for a capacity); the caller counts the applications.
+static void stern_k(KPath& K, int ik, const z64* x, z64* t, const double* d_e, int m, double alpha,
+                    const int* d_nc = nullptr) {
   Context& c = ctx();
   hipStream_t s = c.stream;
   const KPointDev& P = K.k[ik];
@@ -431,7 +441,9 @@ static void stern_k(KPath& K, int ik, const z64* x, z64* t, const double* d_e, i
     return !(ev && ev[0] == '0');
   }();
   const bool lds = klds_env && !c.l_kinetic_only && kvloc_lds_available(K.fft.n1, K.fft.n2, K.fft.n3);
-  if (lds) WH_TIMED("kvloc_lds", kvloc_lds(K.fft.n1, K.fft.n2, K.fft.n3, m, P.npw, ld, P.nl, x, K.vrs, P.g2kin, d_e, t, s));
+  if (lds)
+    WH_TIMED("kvloc_lds", kvloc_lds(K.fft.n1, K.fft.n2, K.fft.n3, m, P.npw, ld, P.nl, x, K.vrs, P.g2kin, d_e, t, s,
+                                    d_nc));
   for (int c0 = 0; c0 < (lds ? 0 : m); c0 += gc) {
     const int mc = std::min(gc, m - c0);
     const z64* xc = x + c0 * ld;
@@ -456,11 +468,11 @@ static void stern_k(KPath& K, int ik, const z64* x, z64* t, const double* d_e, i
     double* c2 = K.w_c2.as<double>(static_cast<size_t>(2) * nq * m);
     const double* q2 = reinterpret_cast<const double*>(P.q2);
     WH_TIMED("gemm_bra", dgemm_bra(2 * nq, m, 2 * P.npw, 1.0, q2, 2 * ld, reinterpret_cast<const double*>(x), 2 * ld,
-                                   0.0, K.qscale2, c2, 2 * nq, s));
+                                   0.0, K.qscale2, c2, 2 * nq, s, d_nc));
     WH_TIMED("gemm_ket", dgemm(false, 2 * P.npw, m, 2 * nq, 1.0, q2, 2 * ld, c2, 2 * nq, 1.0,
-                               reinterpret_cast<double*>(t), 2 * ld, s));
+                               reinterpret_cast<double*>(t), 2 * ld, s, d_nc));
   }
-  c.nstern += m;
+  if (!d_nc) c.nstern += m;
 }
 
 // f <- -f + evc (evc^H f)   (apply_alpha_pc_to_m_wfcs with alpha = -1, dfpt_module.f90:346)
@@ -480,6 +492,14 @@ static void minus_pc_k(KPath& K, int ik, z64* f, int m) {
 
 // PCG for (H_k - e_i + alpha_pv P_v) x_i = b_i, i < m; b is destroyed.
 // Columns are perturbation-major, `group` bands per perturbation (call count).
+// The loop (linsolve_sternheimer_m_wfcts.f90:146-330) runs on the device:
+// cg_select does the convergence test, the compaction of the active list and
+// beta at the start of each iteration, and the iteration's kernels are
+// launched for a capacity (the active count the host last read, rounded up
+// to a power of two) and skip the columns beyond the device's count.  The
+// host reads the active count of the iteration two back (finished while the
+// next one is queued) and stops once it is zero; iterations launched after
+// the last band converged do nothing.
 static int linsolve_k(KPath& K, int ik, int m, z64* b, z64* x, const double* d_e, const double* d_ep, double tr2,
                       std::vector<int>* ierr_col, int group) {
   Context& c = ctx();
@@ -493,52 +513,72 @@ static int linsolve_k(KPath& K, int ik, int m, z64* b, z64* x, const double* d_e
   z64* bc = K.w_bc.as<z64>(vec);
   z64* tc = K.w_tc.as<z64>(vec);
   int* d_l2i = K.w_l2i.as<int>(m);
-  double* d_rho = K.w_rho.as<double>(m);
+  double* d_rhob = K.w_rho.as<double>(m);       // rho by band
+  double* d_rhoold = K.w_rhoold.as<double>(m);
   double* d_ac = K.w_dots.as<double>(2 * static_cast<size_t>(m));
   double* d_beta = K.w_beta.as<double>(m);
   double* d_eu = K.w_eu.as<double>(m);
+  int* d_conv = K.w_conv.as<int>(static_cast<size_t>(m) + 4);
+  int* d_nc = d_conv + m;       // active count of the current iteration
+  int* d_rec = d_conv + m + 1;  // (active count, perturbations among them)
+  const int itmax = c.n_dfpt_maxiter;
+  int* rec = c.h_cgrec.as<int>(2 * static_cast<size_t>(itmax + 1));
   WH_HIP(hipMemsetAsync(hold, 0, sizeof(z64) * vec, s));
-  // step 1 (127-138): g = A x - b
+  WH_HIP(hipMemsetAsync(d_conv, 0, sizeof(int) * (static_cast<size_t>(m) + 4), s));
+  // step 1 (127-138): g = A x - b, and h = M^-1 g, rho of every band
   stern_k(K, ik, x, g, d_e, m, K.alpha_pv);
   count_calls(m, nullptr, group);
   k_sub<<<blocks(static_cast<long>(vec)), NT, 0, s>>>(static_cast<long>(vec), b, g);
-  std::vector<int> l2i(m);
-  for (int i = 0; i < m; ++i) l2i[i] = i;
-  std::vector<double> rho(m, 0.0), rhoold(m, 0.0), rc(m), beta(m);
-  std::vector<char> conv(m, 0);
-  int nc = m;
-  for (int it = 1; it <= c.n_dfpt_maxiter; ++it) {
-    ++c.cg_iters;
-    WH_HIP(hipMemcpyAsync(d_l2i, l2i.data(), sizeof(int) * nc, hipMemcpyHostToDevice, s));
-    k_cg_prec_rho<<<nc, NT, 0, s>>>(d_l2i, P.npw, ld, P.g2kin, d_ep, g, h, d_rho);
-    WH_HIP(hipGetLastError());
-    WH_HIP(hipMemcpyAsync(rc.data(), d_rho, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
-    WH_HIP(hipStreamSynchronize(s));
-    // un-compact and test (208-218)
-    for (int lb = 0; lb < nc; ++lb) {
-      rho[l2i[lb]] = rc[lb];
-      if (rc[lb] < tr2) conv[l2i[lb]] = 1;
+  iota(d_l2i, m, s);
+  k_cg_prec_rho<<<m, NT, 0, s>>>(d_l2i, nullptr, P.npw, ld, P.g2kin, d_ep, g, h, d_rhob);
+  WH_HIP(hipGetLastError());
+  hipEvent_t ev[3];
+  for (hipEvent_t& e : ev) WH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  int last = 0;   // iterations launched
+  int cap = m;
+  for (int it = 1; it <= itmax; ++it) {
+    if (it >= 3) {
+      WH_HIP(hipEventSynchronize(ev[(it - 2) % 3]));
+      const int nlag = rec[2 * (it - 2)];
+      if (nlag == 0) break;
+      int p2 = 1;
+      while (p2 < nlag) p2 <<= 1;
+      cap = std::min(m, p2);
     }
-    int n2 = 0;
-    for (int i = 0; i < m; ++i)
-      if (!conv[i]) l2i[n2++] = i;
-    nc = n2;
-    if (nc == 0) break;
-    for (int lb = 0; lb < nc; ++lb) beta[lb] = it != 1 ? rho[l2i[lb]] / rhoold[l2i[lb]] : 0.0;
-    WH_HIP(hipMemcpyAsync(d_l2i, l2i.data(), sizeof(int) * nc, hipMemcpyHostToDevice, s));
-    WH_HIP(hipMemcpyAsync(d_beta, beta.data(), sizeof(double) * nc, hipMemcpyHostToDevice, s));
-    k_cg_dir<<<blocks(static_cast<long>(nc) * ld), NT, 0, s>>>(nc, d_l2i, ld, d_beta, d_e, h, hold, bc, d_eu);
-    stern_k(K, ik, bc, tc, d_eu, nc, K.alpha_pv);
-    count_calls(nc, l2i.data(), group);
-    k_cg_dots<<<nc, NT, 0, s>>>(d_l2i, P.npw, ld, h, g, tc, d_ac);
-    k_cg_upd<<<blocks(static_cast<long>(nc) * ld), NT, 0, s>>>(nc, d_l2i, ld, d_ac, x, g, h, tc, hold);
+    // convergence test, compaction, beta (208-224)
+    cg_select(m, tr2, it == 1 ? 1 : 0, group, d_rhob, d_rhoold, d_conv, d_e, d_l2i, d_beta, d_eu, d_nc, d_rec, s);
+    k_cg_dir<<<blocks(static_cast<long>(cap) * ld), NT, 0, s>>>(cap, d_nc, d_l2i, ld, d_beta, d_e, h, hold, bc, d_eu);
+    stern_k(K, ik, bc, tc, d_eu, cap, K.alpha_pv, d_nc);
+    k_cg_dots<<<cap, NT, 0, s>>>(d_l2i, d_nc, P.npw, ld, h, g, tc, d_ac);
+    k_cg_upd<<<blocks(static_cast<long>(cap) * ld), NT, 0, s>>>(cap, d_nc, d_l2i, ld, d_ac, x, g, h, tc, hold);
+    // the next iteration's preconditioned residual and rho on this active list
+    k_cg_prec_rho<<<cap, NT, 0, s>>>(d_l2i, d_nc, P.npw, ld, P.g2kin, d_ep, g, h, d_rhob);
     WH_HIP(hipGetLastError());
-    for (int lb = 0; lb < nc; ++lb) rhoold[l2i[lb]] = rho[l2i[lb]];
+    WH_HIP(hipMemcpyAsync(rec + 2 * it, d_rec, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    WH_HIP(hipEventRecord(ev[it % 3], s));
+    last = it;
   }
-  if (ierr_col) {
-    ierr_col->assign(m, 0);
-    for (int i = 0; i < m; ++i) (*ierr_col)[i] = conv[i] ? 0 : 1;
+  WH_HIP(hipStreamSynchronize(s));
+  for (hipEvent_t e : ev) WH_HIP(hipEventDestroy(e));
+  // counters: an iteration counts when it had active bands
+  for (int it = 1; it <= last; ++it) {
+    const int n = rec[2 * it];
+    if (n > 0) {
+      ++c.cg_iters;
+      c.nstern += n;
+      c.nstern_calls += rec[2 * it + 1];
+    }
   }
+  // ierr: the bands still active when the iterations ran out (332)
+  std::vector<int> conv(m);
+  WH_HIP(hipMemcpy(conv.data(), d_conv, sizeof(int) * m, hipMemcpyDeviceToHost));
+  int nc = 0;
+  if (ierr_col) ierr_col->assign(m, 0);
+  for (int i = 0; i < m; ++i)
+    if (!conv[i]) {
+      ++nc;
+      if (ierr_col) (*ierr_col)[i] = 1;
+    }
   return nc;
 }
 
