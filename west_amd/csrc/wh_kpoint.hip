@@ -44,6 +44,8 @@ struct KPointDev {
   z64* q2 = nullptr;          // (npwx, 2 nq): evc, i evc, vkb, i vkb
   std::vector<double> et;     // nbndval
   std::vector<double> eprec;  // nbndval: 1.35 sum |evc|^2 |k+G|^2 (Tools/set_eprec.f90:40-56)
+  double* d_et = nullptr;     // the same two on the device
+  double* d_eprec = nullptr;
   bool set = false;
 };
 
@@ -75,9 +77,10 @@ struct KPath {
   std::vector<void*> allocs;
   DevBuf g_a, g_b, g_c, g_aux;                       // grids
   DevBuf w_x, w_t, w_g, w_h, w_hold, w_bc, w_tc;     // CG vectors
-  DevBuf w_c2, w_l2i, w_rho, w_rhoold, w_conv, w_dots, w_beta, w_eu, w_ecol, w_epcol;
+  DevBuf w_c2, w_l2i, w_rho, w_rhoold, w_conv, w_nonconv, w_beta, w_eu, w_ecol, w_epcol;
   DevBuf w_dv, w_dn, w_in, w_out, w_in2;
   DevBuf dav[8];
+  hipEvent_t cg_ev[3] = {nullptr, nullptr, nullptr};   // lagged reads of the CG loop
 };
 
 void free_kpath(KPath* k) {
@@ -85,10 +88,12 @@ void free_kpath(KPath* k) {
   for (void* p : k->allocs) (void)hipFree(p);
   k->fft.release();
   for (DevBuf* b : {&k->g_a, &k->g_b, &k->g_c, &k->g_aux, &k->w_x, &k->w_t, &k->w_g, &k->w_h, &k->w_hold, &k->w_bc,
-                    &k->w_tc, &k->w_c2, &k->w_l2i, &k->w_rho, &k->w_rhoold, &k->w_conv, &k->w_dots, &k->w_beta,
+                    &k->w_tc, &k->w_c2, &k->w_l2i, &k->w_rho, &k->w_rhoold, &k->w_conv, &k->w_nonconv, &k->w_beta,
                     &k->w_eu, &k->w_ecol, &k->w_epcol, &k->w_dv, &k->w_dn, &k->w_in, &k->w_out, &k->w_in2})
     b->release();
   for (DevBuf& b : k->dav) b.release();
+  for (hipEvent_t e : k->cg_ev)
+    if (e) (void)hipEventDestroy(e);
   delete k;
 }
 
@@ -285,16 +290,23 @@ __global__ void k_cg_dir(int cap, const int* __restrict__ nc, const int* __restr
   }
 }
 
-// a = Re <h|g>, c = Re <h|t>   (277-305)
-__global__ __launch_bounds__(NT) void k_cg_dots(const int* __restrict__ l2i, const int* __restrict__ nc, int npw,
-                                                long ld, const z64* __restrict__ h, const z64* __restrict__ g,
-                                                const z64* __restrict__ tc, double* __restrict__ ac) {
+// One workgroup per listed column: a = Re <h|g>, c = Re <h|t> (277-305);
+// x -= a/c h, g -= a/c t, hold = h (313-321); then the next iteration's
+// h = g / max(1, g2kin/eprec) and rho[band] = Re <h|g> (150-205).
+__global__ __launch_bounds__(NT) void k_cg_step(const int* __restrict__ l2i, const int* __restrict__ nc, int npw,
+                                                long ld, const double* __restrict__ g2,
+                                                const double* __restrict__ ep, z64* __restrict__ x,
+                                                z64* __restrict__ g, z64* __restrict__ h,
+                                                const z64* __restrict__ tc, z64* __restrict__ hold,
+                                                double* __restrict__ rho) {
   __shared__ double red[NT / 64];
   const int lb = blockIdx.x;
   if (lb >= *nc) return;   // whole workgroup
   const int col = l2i[lb];
-  const z64* hc = h + col * ld;
-  const z64* gc = g + col * ld;
+  z64* xc = x + col * ld;
+  z64* gc = g + col * ld;
+  z64* hc = h + col * ld;
+  z64* ho = hold + col * ld;
   const z64* tt = tc + lb * ld;
   double sa = 0.0, sc = 0.0;
   for (int ig = threadIdx.x; ig < npw; ig += NT) {
@@ -303,25 +315,46 @@ __global__ __launch_bounds__(NT) void k_cg_dots(const int* __restrict__ l2i, con
   }
   sa = block_sum(sa, red);
   sc = block_sum(sc, red);
-  if (threadIdx.x == 0) {
-    ac[2 * lb] = sa;
-    ac[2 * lb + 1] = sc;
+  const double st = sa / sc;
+  const double e = ep[col];
+  double sr = 0.0;
+  for (int ig = threadIdx.x; ig < ld; ig += NT) {
+    const z64 hv = hc[ig];
+    const z64 gn = gc[ig] - cscale(tt[ig], st);
+    xc[ig] = xc[ig] - cscale(hv, st);
+    gc[ig] = gn;
+    ho[ig] = hv;
+    if (ig < npw) {
+      const z64 hn = cscale(gn, 1.0 / fmax(1.0, g2[ig] / e));
+      hc[ig] = hn;
+      sr += hn.x * gn.x + hn.y * gn.y;
+    } else {
+      hc[ig] = mk(0.0, 0.0);
+    }
   }
+  sr = block_sum(sr, red);
+  if (threadIdx.x == 0) rho[col] = sr;
 }
 
-// x -= a/c h ; g -= a/c t ; hold = h   (313-321)
-__global__ void k_cg_upd(int cap, const int* __restrict__ nc, const int* __restrict__ l2i, long ld,
-                         const double* __restrict__ ac, z64* __restrict__ x, z64* __restrict__ g,
-                         const z64* __restrict__ h, const z64* __restrict__ tc, z64* __restrict__ hold) {
-  const long n = static_cast<long>(min(cap, *nc)) * ld;
-  for (long t = blockIdx.x * static_cast<long>(NT) + threadIdx.x; t < n; t += static_cast<long>(gridDim.x) * NT) {
-    const int lb = static_cast<int>(t / ld);
-    const long ig = t - static_cast<long>(lb) * ld;
-    const long o = static_cast<long>(l2i[lb]) * ld + ig;
-    const double st = ac[2 * lb] / ac[2 * lb + 1];
-    x[o] = x[o] - cscale(h[o], st);
-    g[o] = g[o] - cscale(tc[t], st);
-    hold[o] = h[o];
+// nonconv[j / group] += !conv[j]  (group <= 0: one total)
+__global__ void k_count_nonconv(int m, int group, const int* __restrict__ conv, int* __restrict__ nonconv) {
+  const int j = blockIdx.x * NT + threadIdx.x;
+  if (j < m && !conv[j]) atomicAdd(nonconv + (group > 0 ? j / group : 0), 1);
+}
+
+// a[j * ld] = 0, j < m
+__global__ void k_zero_first(int m, long ld, z64* __restrict__ a) {
+  const int j = blockIdx.x * NT + threadIdx.x;
+  if (j < m) a[j * ld] = mk(0.0, 0.0);
+}
+
+// per-column energies of `cols` columns, nb bands per perturbation
+__global__ void k_band_cols(int cols, int nb, const double* __restrict__ et, const double* __restrict__ ep,
+                            double* __restrict__ ec, double* __restrict__ epc) {
+  const int j = blockIdx.x * NT + threadIdx.x;
+  if (j < cols) {
+    ec[j] = et[j % nb];
+    epc[j] = ep[j % nb];
   }
 }
 
@@ -499,9 +532,14 @@ static void minus_pc_k(KPath& K, int ik, z64* f, int m) {
 // to a power of two) and skip the columns beyond the device's count.  The
 // host reads the active count of the iteration two back (finished while the
 // next one is queued) and stops once it is zero; iterations launched after
-// the last band converged do nothing.
+// the last band converged do nothing.  Nothing waits for the end: the
+// unconverged bands are added on the device to nonconv[band / group]
+// (group <= 0: nonconv[0]), each iteration's (active count, perturbations)
+// goes to rec[2 it] (pinned, 2 (n_dfpt_maxiter + 1) ints), and the caller,
+// once the stream is done, passes rec and the returned number of iterations
+// launched to cg_k_count.
 static int linsolve_k(KPath& K, int ik, int m, z64* b, z64* x, const double* d_e, const double* d_ep, double tr2,
-                      std::vector<int>* ierr_col, int group) {
+                      int group, int* rec, int* nonconv) {
   Context& c = ctx();
   hipStream_t s = c.stream;
   const KPointDev& P = K.k[ik];
@@ -515,14 +553,14 @@ static int linsolve_k(KPath& K, int ik, int m, z64* b, z64* x, const double* d_e
   int* d_l2i = K.w_l2i.as<int>(m);
   double* d_rhob = K.w_rho.as<double>(m);       // rho by band
   double* d_rhoold = K.w_rhoold.as<double>(m);
-  double* d_ac = K.w_dots.as<double>(2 * static_cast<size_t>(m));
   double* d_beta = K.w_beta.as<double>(m);
   double* d_eu = K.w_eu.as<double>(m);
   int* d_conv = K.w_conv.as<int>(static_cast<size_t>(m) + 4);
   int* d_nc = d_conv + m;       // active count of the current iteration
   int* d_rec = d_conv + m + 1;  // (active count, perturbations among them)
+  if (!K.cg_ev[0])
+    for (hipEvent_t& e : K.cg_ev) WH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   const int itmax = c.n_dfpt_maxiter;
-  int* rec = c.h_cgrec.as<int>(2 * static_cast<size_t>(itmax + 1));
   WH_HIP(hipMemsetAsync(hold, 0, sizeof(z64) * vec, s));
   WH_HIP(hipMemsetAsync(d_conv, 0, sizeof(int) * (static_cast<size_t>(m) + 4), s));
   // step 1 (127-138): g = A x - b, and h = M^-1 g, rho of every band
@@ -532,13 +570,11 @@ static int linsolve_k(KPath& K, int ik, int m, z64* b, z64* x, const double* d_e
   iota(d_l2i, m, s);
   k_cg_prec_rho<<<m, NT, 0, s>>>(d_l2i, nullptr, P.npw, ld, P.g2kin, d_ep, g, h, d_rhob);
   WH_HIP(hipGetLastError());
-  hipEvent_t ev[3];
-  for (hipEvent_t& e : ev) WH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   int last = 0;   // iterations launched
   int cap = m;
   for (int it = 1; it <= itmax; ++it) {
     if (it >= 3) {
-      WH_HIP(hipEventSynchronize(ev[(it - 2) % 3]));
+      WH_HIP(hipEventSynchronize(K.cg_ev[(it - 2) % 3]));
       const int nlag = rec[2 * (it - 2)];
       if (nlag == 0) break;
       int p2 = 1;
@@ -549,18 +585,21 @@ static int linsolve_k(KPath& K, int ik, int m, z64* b, z64* x, const double* d_e
     cg_select(m, tr2, it == 1 ? 1 : 0, group, d_rhob, d_rhoold, d_conv, d_e, d_l2i, d_beta, d_eu, d_nc, d_rec, s);
     k_cg_dir<<<blocks(static_cast<long>(cap) * ld), NT, 0, s>>>(cap, d_nc, d_l2i, ld, d_beta, d_e, h, hold, bc, d_eu);
     stern_k(K, ik, bc, tc, d_eu, cap, K.alpha_pv, d_nc);
-    k_cg_dots<<<cap, NT, 0, s>>>(d_l2i, d_nc, P.npw, ld, h, g, tc, d_ac);
-    k_cg_upd<<<blocks(static_cast<long>(cap) * ld), NT, 0, s>>>(cap, d_nc, d_l2i, ld, d_ac, x, g, h, tc, hold);
-    // the next iteration's preconditioned residual and rho on this active list
-    k_cg_prec_rho<<<cap, NT, 0, s>>>(d_l2i, d_nc, P.npw, ld, P.g2kin, d_ep, g, h, d_rhob);
+    k_cg_step<<<cap, NT, 0, s>>>(d_l2i, d_nc, P.npw, ld, P.g2kin, d_ep, x, g, h, tc, hold, d_rhob);
     WH_HIP(hipGetLastError());
     WH_HIP(hipMemcpyAsync(rec + 2 * it, d_rec, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
-    WH_HIP(hipEventRecord(ev[it % 3], s));
+    WH_HIP(hipEventRecord(K.cg_ev[it % 3], s));
     last = it;
   }
-  WH_HIP(hipStreamSynchronize(s));
-  for (hipEvent_t e : ev) WH_HIP(hipEventDestroy(e));
-  // counters: an iteration counts when it had active bands
+  // the bands still active when the iterations ran out (332)
+  k_count_nonconv<<<blocks(m), NT, 0, s>>>(m, group, d_conv, nonconv);
+  WH_HIP(hipGetLastError());
+  return last;
+}
+
+// counters of a finished linsolve_k: an iteration counts when it had active bands
+static void cg_k_count(const int* rec, int last) {
+  Context& c = ctx();
   for (int it = 1; it <= last; ++it) {
     const int n = rec[2 * it];
     if (n > 0) {
@@ -569,17 +608,6 @@ static int linsolve_k(KPath& K, int ik, int m, z64* b, z64* x, const double* d_e
       c.nstern_calls += rec[2 * it + 1];
     }
   }
-  // ierr: the bands still active when the iterations ran out (332)
-  std::vector<int> conv(m);
-  WH_HIP(hipMemcpy(conv.data(), d_conv, sizeof(int) * m, hipMemcpyDeviceToHost));
-  int nc = 0;
-  if (ierr_col) ierr_col->assign(m, 0);
-  for (int i = 0; i < m; ++i)
-    if (!conv[i]) {
-      ++nc;
-      if (ierr_col) (*ierr_col)[i] = 1;
-    }
-  return nc;
 }
 
 // dng (npwqx, m) = chi0(q) dvg for q point iq (0-based)  (dfpt_module.f90:163-505)
@@ -595,15 +623,21 @@ static int dfpt_q(KPath& K, int iq, int m, const z64* dvg, z64* dng, double tr2,
   const int nb = K.nbndval;
   WH_HIP(hipMemsetAsync(dng, 0, sizeof(z64) * m * ldq, s));
   if (ierr_pert) ierr_pert->assign(m, 0);
-  int nonconv = 0;
   if (m == 0 || nb == 0) return 0;
+  // perturbations per chunk: their band columns' grids within the budget
+  const int mp = std::max(1, std::min(m, grid_cols(K) / std::max(nb, 1)));
+  // per solve: its CG records (pinned) and iterations launched, read at the end
+  const int nsolve = K.nks * ((m + mp - 1) / mp);
+  const size_t rec_len = 2 * (static_cast<size_t>(c.n_dfpt_maxiter) + 1);
+  int* recs = c.h_cgrec.as<int>(rec_len * nsolve);
+  std::vector<int> launched;
+  int* d_nonconv = K.w_nonconv.as<int>(m);   // unconverged bands per perturbation
+  WH_HIP(hipMemsetAsync(d_nonconv, 0, sizeof(int) * m, s));
   // the perturbations in real space (237)
   z64* aux = K.g_aux.as<z64>(static_cast<size_t>(m) * nnr);
   zero_grids(aux, m * nnr, s);
   k_scatter<<<blocks(static_cast<long>(m) * Q.npwq), NT, 0, s>>>(m, Q.npwq, Q.nl, dvg, ldq, aux, nnr);
   WH_TIMED("kfft", launch_grid_fft(K.fft, aux, m, +1, s));
-  // perturbations per chunk: their band columns' grids within the budget
-  const int mp = std::max(1, std::min(m, grid_cols(K) / std::max(nb, 1)));
   for (int ik = 0; ik < K.nks; ++ik) {
     const KPointDev& P = K.k[ik];
     const int ikq = Q.ikq[ik];
@@ -620,16 +654,9 @@ static int dfpt_q(KPath& K, int iq, int m, const z64* dvg, z64* dng, double tr2,
       const int mc = std::min(mp, m - p0);
       const int cols = mc * nb;
       // per column: e at k - q (222), eprec from the states at k (211)
-      std::vector<double> ec(cols), epc(cols);
-      for (int p = 0; p < mc; ++p)
-        for (int v = 0; v < nb; ++v) {
-          ec[p * nb + v] = PQ.et[v];
-          epc[p * nb + v] = P.eprec[v];
-        }
       double* d_e = K.w_ecol.as<double>(cols);
       double* d_ep = K.w_epcol.as<double>(cols);
-      WH_HIP(hipMemcpyAsync(d_e, ec.data(), sizeof(double) * cols, hipMemcpyHostToDevice, s));
-      WH_HIP(hipMemcpyAsync(d_ep, epc.data(), sizeof(double) * cols, hipMemcpyHostToDevice, s));
+      k_band_cols<<<blocks(cols), NT, 0, s>>>(cols, nb, PQ.d_et, P.d_eprec, d_e, d_ep);
       // dvpsi = fwfft(psi_{k-q} phase dv) on the k sphere (295-302)
       z64* R = K.g_a.as<z64>(static_cast<size_t>(cols) * nnr);
       k_rhs<<<blocks(static_cast<long>(cols) * nnr), NT, 0, s>>>(nb, mc, nnr, psik, aux + p0 * nnr, R);
@@ -644,10 +671,8 @@ static int dfpt_q(KPath& K, int iq, int m, const z64* dvg, z64* dng, double tr2,
       k_precond<<<blocks(static_cast<long>(cols) * ld), NT, 0, s>>>(cols, P.npw, ld, P.g2kin, d_ep, dvpsi, dpsi);
       WH_HIP(hipGetLastError());
       if (tr2 >= 0.0) {                                                  // (350-361)
-        std::vector<int> ierr_col;
-        nonconv += linsolve_k(K, ik, cols, dvpsi, dpsi, d_e, d_ep, tr2, &ierr_col, nb);
-        if (ierr_pert)
-          for (int j = 0; j < cols; ++j) (*ierr_pert)[p0 + j / nb] += ierr_col[j];
+        int* rec = recs + rec_len * launched.size();
+        launched.push_back(linsolve_k(K, ik, cols, dvpsi, dpsi, d_e, d_ep, tr2, nb, rec, d_nonconv + p0));
       }
       // response density sum_v conj(psi_{k-q} phase) dpsi (433-441)
       z64* Rd = K.g_a.as<z64>(static_cast<size_t>(cols) * nnr);
@@ -665,9 +690,17 @@ static int dfpt_q(KPath& K, int iq, int m, const z64* dvg, z64* dng, double tr2,
     }
   }
   if (Q.is_gamma) {   // (505): the perturbation acts on the body only
-    std::vector<z64> zero(1, mk(0.0, 0.0));
-    for (int p = 0; p < m; ++p) WH_HIP(hipMemcpyAsync(dng + p * ldq, zero.data(), sizeof(z64), hipMemcpyHostToDevice, s));
-    WH_HIP(hipStreamSynchronize(s));
+    k_zero_first<<<blocks(m), NT, 0, s>>>(m, ldq, dng);
+    WH_HIP(hipGetLastError());
+  }
+  std::vector<int> nc_pert(m);
+  WH_HIP(hipMemcpyAsync(nc_pert.data(), d_nonconv, sizeof(int) * m, hipMemcpyDeviceToHost, s));
+  WH_HIP(hipStreamSynchronize(s));
+  for (size_t i = 0; i < launched.size(); ++i) cg_k_count(recs + rec_len * i, launched[i]);
+  int nonconv = 0;
+  for (int p = 0; p < m; ++p) {
+    nonconv += nc_pert[p];
+    if (ierr_pert) (*ierr_pert)[p] = nc_pert[p];
   }
   return nonconv;
 }
@@ -738,7 +771,14 @@ int kpath_linsolve(int nbndval, int m, double* b, double* x, const double* e, co
   double* dep = K.w_epcol.as<double>(m);
   WH_HIP(hipMemcpyAsync(de, e, sizeof(double) * m, hipMemcpyDefault, s));
   WH_HIP(hipMemcpyAsync(dep, eprec, sizeof(double) * m, hipMemcpyDefault, s));
-  const int nc = linsolve_k(K, ik, m, db, dx, de, dep, tr2, nullptr, 0);
+  int* d_nonconv = K.w_nonconv.as<int>(1);
+  WH_HIP(hipMemsetAsync(d_nonconv, 0, sizeof(int), s));
+  int* rec = c.h_cgrec.as<int>(2 * (static_cast<size_t>(c.n_dfpt_maxiter) + 1));
+  const int last = linsolve_k(K, ik, m, db, dx, de, dep, tr2, 0, rec, d_nonconv);
+  int nc = 0;
+  WH_HIP(hipMemcpyAsync(&nc, d_nonconv, sizeof(int), hipMemcpyDeviceToHost, s));
+  WH_HIP(hipStreamSynchronize(s));
+  cg_k_count(rec, last);
   WH_HIP(hipMemcpyAsync(x, dx, sizeof(z64) * cin, hipMemcpyDefault, s));
   finish();
   if (ierr) *ierr = nc;
@@ -908,7 +948,9 @@ int wh_set_kpoint(int ik, int npw, const int* nl, const double* g2kin, const dou
   WH_REQUIRE(K.nkb == 0 || vkb, "vkb is required when nkb > 0");
   WH_HIP(hipStreamSynchronize(ctx().stream));
   KPointDev& P = K.k[ik - 1];
-  for (void* p : {static_cast<void*>(P.nl), static_cast<void*>(P.g2kin), static_cast<void*>(P.q2)}) release(K, p);
+  for (void* p : {static_cast<void*>(P.nl), static_cast<void*>(P.g2kin), static_cast<void*>(P.q2),
+                  static_cast<void*>(P.d_et), static_cast<void*>(P.d_eprec)})
+    release(K, p);
   const long nnr = K.fft.nnr();
   std::vector<int> hnl(npw);
   std::vector<double> hg2(npw);
@@ -953,6 +995,8 @@ int wh_set_kpoint(int ik, int npw, const int* nl, const double* g2kin, const dou
   P.nl = upload(K, hnl.data(), hnl.size());
   P.g2kin = upload(K, hg2.data(), hg2.size());
   P.q2 = upload(K, q2.data(), q2.size());
+  P.d_et = upload(K, P.et.data(), P.et.size());
+  P.d_eprec = upload(K, P.eprec.data(), P.eprec.size());
   P.set = true;
   return 0;
   WH_K_END
