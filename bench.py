@@ -361,7 +361,8 @@ def _qpoint_measure(wstat, torch, args, with_cpu=True, nk=(3, 3, 3), npert=16, e
         dng = torch.zeros_like(dvg)
         wstat.dfpt(npert, dvg, args.tr2, iq=iq + 1, dng=dng)          # warm-up (kernel variants, workspaces)
         torch.cuda.synchronize()
-        wstat.profile(True)
+        # wall time without the per-call timing events (its launches are a few
+        # microseconds each), then the kernel times from a profiled pass
         cols0 = wstat.counters()[0]
         steps = 3
         t = time.perf_counter()
@@ -369,15 +370,19 @@ def _qpoint_measure(wstat, torch, args, with_cpu=True, nk=(3, 3, 3), npert=16, e
             wstat.dfpt(npert, dvg, args.tr2, iq=iq + 1, dng=dng)
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t) / steps
-        kern = {n: wstat.profile_query(n) for n in ("kfft", "kvloc_lds", "gemm_bra", "gemm_ket")}
         op_cols = (wstat.counters()[0] - cols0) / steps
+        wstat.profile(True)
+        for _ in range(steps):
+            wstat.dfpt(npert, dvg, args.tr2, iq=iq + 1, dng=dng)
+        torch.cuda.synchronize()
+        kern = {n: wstat.profile_query(n) for n in ("kfft", "kvloc_lds", "cg_iter_lds", "gemm_bra", "gemm_ket")}
         wstat.profile(False)
         pairs = npert * nb * nks
         # grid columns through the full-grid transforms in HBM (three passes,
         # read + write each); the operator's own columns take the LDS-resident
-        # kernel when the grid allows it (timed as kvloc_lds) and are then not
-        # among them
-        lds_op = kern["kvloc_lds"][1] > 0
+        # kernels when the grid allows it (the first application as kvloc_lds,
+        # the CG iterations as cg_iter_lds) and are then not among them
+        lds_op = kern["kvloc_lds"][1] > 0 or kern["cg_iter_lds"][1] > 0
         fft_cols = npert + nks * (nb + 2 * npert * nb + npert) + (0 if lds_op else 2 * op_cols)
         fft_bytes = fft_cols * 6 * 16 * nnr
         kfft_ms = kern["kfft"][0] / steps

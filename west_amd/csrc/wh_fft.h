@@ -164,6 +164,17 @@ void launch_grid_fft(const GridFft& g, z64* data, int batch, int dir, hipStream_
 bool kvloc_lds_available(int n1, int n2, int n3);
 void kvloc_lds(int n1, int n2, int n3, int m, int npw, long ld, const int* nl, const z64* x, const double* vrs,
                const double* g2, const double* e, z64* t, hipStream_t s, const int* d_m = nullptr);
+// One CG iteration of the k-point Sternheimer solve per listed column (lb <
+// *nc of cap launched; band l2i[lb]) in one workgroup with the grid in LDS:
+// direction h = -h + beta hold (bc = h), t = (H_k - eu + projectors) h with
+// the local term in LDS and the projector block q2 (real view, 2 ld rows,
+// nq2 <= 32 columns, row scales qs), the CG step on x, g, hold, and the next
+// preconditioned residual h and rho[band] (wh_kfft_lds.hip)
+bool cg_iter_lds_available(int n1, int n2, int n3, int nq2);
+void cg_iter_lds(int n1, int n2, int n3, int npw, long ld, const int* nl, const double* vrs, const double* g2,
+                 int nq2, const double* q2, const double* qs, int cap, const int* l2i, const int* nc,
+                 const double* beta, const double* eu, const double* ep, z64* h, z64* hold, z64* x, z64* g, z64* bc,
+                 z64* tc, double* rho, hipStream_t s);
 
 // kernel launchers (double precision)
 void launch_zinv(const StickPlan& p, const FftItem* d_items, int nitems, const z64* psi, long ldpsi,

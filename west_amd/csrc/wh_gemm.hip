@@ -250,12 +250,18 @@ __global__ __launch_bounds__(NT, 2) void k_dgemm(int M, int N, int K, int kchunk
 // Bra product with few rows (M <= MB, e.g. the bands of a small molecule with
 // no projectors): one workgroup per (column j, K chunk z) streams B's column
 // once and keeps the M running sums per thread (A's rows come through the
-// caches); partial sums P_z(i, j) at P + z * pstride + j * ldc + i.
+// caches); partial sums P_z(i, j) at P + z * pstride + j * ldc + i.  With
+// FIN (one K chunk) the workgroup writes the result itself, as k_reduce_g0
+// would: P(i, j) = s_i * (alpha * sum - g0 * A[i*lda] * B[j*ldb]).
 constexpr int NTS = 256;
-template <int MB>
+struct BraFin {
+  double alpha, g0;
+  const double* s;
+};
+template <int MB, bool FIN>
 __global__ __launch_bounds__(NTS) void k_bra_skinny(int M, int K, int kchunk, const double* __restrict__ A, long lda,
                                                    const double* __restrict__ B, long ldb, double* __restrict__ P,
-                                                   long ldc, long pstride, const int* __restrict__ d_n) {
+                                                   long ldc, long pstride, const int* __restrict__ d_n, BraFin fin) {
   __shared__ double red[MB][NTS / 64];
   const int j = blockIdx.x;
   if (d_n && j >= *d_n) return;
@@ -287,6 +293,11 @@ __global__ __launch_bounds__(NTS) void k_bra_skinny(int M, int K, int kchunk, co
     double v = 0.0;
 #pragma unroll
     for (int q = 0; q < NTS / 64; ++q) v += red[threadIdx.x][q];
+    if (FIN) {
+      const double acc = 0.0 + v;
+      v = fin.alpha * acc - fin.g0 * A[static_cast<long>(threadIdx.x) * lda] * Bj[0];
+      if (fin.s) v *= fin.s[threadIdx.x];
+    }
     P[static_cast<long>(z) * pstride + static_cast<long>(j) * ldc + threadIdx.x] = v;
   }
 }
@@ -517,11 +528,21 @@ static void dgemm_bra_skinny(int M, int N, int K, double alpha, const double* A,
   int split = 1, kchunk = 1;
   skinny_split(N, K, split, kchunk);
   const long pstride = static_cast<long>(ldc) * N;
-  if (!P) P = partials().get(static_cast<size_t>(split) * pstride, s);
+  const BraFin fin{alpha, g0, scale};
   dim3 grid(N, split);
-  if (M <= 8) k_bra_skinny<8><<<grid, NTS, 0, s>>>(M, K, kchunk, A, lda, B, ldb, P, ldc, pstride, d_n);
-  else if (M <= 16) k_bra_skinny<16><<<grid, NTS, 0, s>>>(M, K, kchunk, A, lda, B, ldb, P, ldc, pstride, d_n);
-  else k_bra_skinny<32><<<grid, NTS, 0, s>>>(M, K, kchunk, A, lda, B, ldb, P, ldc, pstride, d_n);
+  if (split == 1) {   // one K chunk: the workgroups write C themselves
+    if (M <= 8) k_bra_skinny<8, true><<<grid, NTS, 0, s>>>(M, K, kchunk, A, lda, B, ldb, C, ldc, pstride, d_n, fin);
+    else if (M <= 16)
+      k_bra_skinny<16, true><<<grid, NTS, 0, s>>>(M, K, kchunk, A, lda, B, ldb, C, ldc, pstride, d_n, fin);
+    else k_bra_skinny<32, true><<<grid, NTS, 0, s>>>(M, K, kchunk, A, lda, B, ldb, C, ldc, pstride, d_n, fin);
+    WH_HIP(hipGetLastError());
+    return;
+  }
+  if (!P) P = partials().get(static_cast<size_t>(split) * pstride, s);
+  if (M <= 8) k_bra_skinny<8, false><<<grid, NTS, 0, s>>>(M, K, kchunk, A, lda, B, ldb, P, ldc, pstride, d_n, fin);
+  else if (M <= 16)
+    k_bra_skinny<16, false><<<grid, NTS, 0, s>>>(M, K, kchunk, A, lda, B, ldb, P, ldc, pstride, d_n, fin);
+  else k_bra_skinny<32, false><<<grid, NTS, 0, s>>>(M, K, kchunk, A, lda, B, ldb, P, ldc, pstride, d_n, fin);
   WH_HIP(hipGetLastError());
   const long n = static_cast<long>(M) * N;
   k_reduce_g0<<<ceil_div(n, 256), 256, 0, s>>>(M, N, split, P, pstride, alpha, g0, A, lda, B, ldb, scale, C, ldc,

@@ -81,6 +81,33 @@ __device__ void kl_pass_vloc(int n, z64* p, int stride, const double* vr, long v
 // LDS index of grid point (i1, i2, i3): pitch p1 (odd) along a1
 __device__ __forceinline__ int kl_at(int i1, int i2, int i3, int p1, int n2) { return i1 + p1 * (i2 + n2 * i3); }
 
+// The local term on the LDS grid: inverse passes along a1 and a2, the a3
+// pass with V(r) between its inverse and forward transforms, forward passes
+// along a2 and a1 (the caller's barrier before, this function's after).
+__device__ __forceinline__ void kl_local(z64* grid, int n1, int n2, int n3, int p1, const double* __restrict__ vrs) {
+  const int tid = threadIdx.x;
+  for (int l = tid; l < n2 * n3; l += KL_NT) kl_pass<+1>(n1, grid + p1 * l, 1);
+  __syncthreads();
+  for (int l = tid; l < n1 * n3; l += KL_NT) {
+    const int i1 = l % n1, i3 = l / n1;
+    kl_pass<+1>(n2, grid + kl_at(i1, 0, i3, p1, n2), p1);
+  }
+  __syncthreads();
+  for (int l = tid; l < n1 * n2; l += KL_NT) {
+    const int i1 = l % n1, i2 = l / n1;
+    kl_pass_vloc(n3, grid + kl_at(i1, i2, 0, p1, n2), p1 * n2, vrs + i1 + static_cast<long>(n1) * i2,
+                 static_cast<long>(n1) * n2);
+  }
+  __syncthreads();
+  for (int l = tid; l < n1 * n3; l += KL_NT) {
+    const int i1 = l % n1, i3 = l / n1;
+    kl_pass<-1>(n2, grid + kl_at(i1, 0, i3, p1, n2), p1);
+  }
+  __syncthreads();
+  for (int l = tid; l < n2 * n3; l += KL_NT) kl_pass<-1>(n1, grid + p1 * l, 1);
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(KL_NT) void k_kvloc_lds(int n1, int n2, int n3, int p1, int npw, long ld,
                                                    const int* __restrict__ nl, const z64* __restrict__ x,
                                                    const double* __restrict__ vrs, const double* __restrict__ g2,
@@ -101,29 +128,7 @@ __global__ __launch_bounds__(KL_NT) void k_kvloc_lds(int n1, int n2, int n3, int
     grid[i1 + p1 * (r / n1)] = xb[ig];
   }
   __syncthreads();
-  // inverse along a1 and a2
-  for (int l = tid; l < n2 * n3; l += KL_NT) kl_pass<+1>(n1, grid + p1 * l, 1);
-  __syncthreads();
-  for (int l = tid; l < n1 * n3; l += KL_NT) {
-    const int i1 = l % n1, i3 = l / n1;
-    kl_pass<+1>(n2, grid + kl_at(i1, 0, i3, p1, n2), p1);
-  }
-  __syncthreads();
-  // a3: inverse, V(r), forward
-  for (int l = tid; l < n1 * n2; l += KL_NT) {
-    const int i1 = l % n1, i2 = l / n1;
-    kl_pass_vloc(n3, grid + kl_at(i1, i2, 0, p1, n2), p1 * n2, vrs + i1 + static_cast<long>(n1) * i2,
-                 static_cast<long>(n1) * n2);
-  }
-  __syncthreads();
-  // forward along a2 and a1
-  for (int l = tid; l < n1 * n3; l += KL_NT) {
-    const int i1 = l % n1, i3 = l / n1;
-    kl_pass<-1>(n2, grid + kl_at(i1, 0, i3, p1, n2), p1);
-  }
-  __syncthreads();
-  for (int l = tid; l < n2 * n3; l += KL_NT) kl_pass<-1>(n1, grid + p1 * l, 1);
-  __syncthreads();
+  kl_local(grid, n1, n2, n3, p1, vrs);
   const double sc = 1.0 / (static_cast<double>(n1) * n2 * n3);
   const double eb = e[b];
   z64* tb = t + b * ld;
@@ -138,7 +143,156 @@ __global__ __launch_bounds__(KL_NT) void k_kvloc_lds(int n1, int n2, int n3, int
   }
 }
 
+// sum over the workgroup of NV values per thread (NV <= KL_NQ2); every
+// thread gets the sums in out[] (LDS)
+constexpr int KL_NQ2 = 32;   // projector rows (real view) the fused CG iteration takes
+__device__ __forceinline__ void kl_block_sums(double* v, int nv, double (*red)[KL_NT / 64], double* out) {
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < KL_NQ2; ++i) {
+    if (i < nv) {
+      double x = v[i];
+      for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+      if (lane == 0) red[i][w] = x;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < nv) {
+    double x = 0.0;
+#pragma unroll
+    for (int q = 0; q < KL_NT / 64; ++q) x += red[threadIdx.x][q];
+    out[threadIdx.x] = x;
+  }
+  __syncthreads();
+}
+
+// One CG iteration of the Sternheimer solve at a k point for listed column
+// lb < *nc (band col = l2i[lb]), one workgroup per column, the grid in LDS:
+//   h = -h + beta hold, bc = h                                (226-240)
+//   t = (H_k - e + alpha P_v) h: local term on the LDS grid, |k+G|^2 - e,
+//       and t += Q2 (s * (Q2^T h)) in the real view of the projectors  (251)
+//   a = Re <h|g>, c = Re <h|t>; x -= a/c h, g -= a/c t, hold = h  (253-321)
+//   h = g / max(1, g2kin/eprec), rho[col] = Re <h|g> (the next 150-205)
+// q2: real view of the projector block (2 ld rows, nq2 columns), qs: its
+// row scales.  Each thread keeps the plane waves ig = tid + k KL_NT in every
+// phase, so the values it writes to bc and tc it reads back itself.
+struct KlCgArgs {
+  int n1, n2, n3, p1, npw;
+  long ld;
+  const int* nl;
+  const double* vrs;
+  const double* g2;
+  int nq2;
+  const double* q2;
+  const double* qs;
+  const int* l2i;
+  const int* nc;
+  const double* beta;
+  const double* eu;
+  const double* ep;
+  z64* h;
+  z64* hold;
+  z64* x;
+  z64* g;
+  z64* bc;
+  z64* tc;
+  double* rho;
+};
+
+__global__ __launch_bounds__(KL_NT) void k_cg_iter_lds(KlCgArgs a) {
+  extern __shared__ char smem_raw[];
+  __shared__ double red[KL_NQ2][KL_NT / 64];
+  __shared__ double sums[KL_NQ2];
+  __shared__ double c2s[KL_NQ2];
+  z64* grid = reinterpret_cast<z64*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int lb = blockIdx.x;
+  if (lb >= *a.nc) return;   // whole workgroup
+  const int col = a.l2i[lb];
+  const long ld = a.ld;
+  const int npw = a.npw;
+  const int nq2 = a.nq2;
+  const long ldr = 2 * ld;   // real view
+  z64* hc = a.h + col * ld;
+  z64* ho = a.hold + col * ld;
+  z64* xc = a.x + col * ld;
+  z64* gc = a.g + col * ld;
+  z64* bb = a.bc + lb * ld;
+  z64* tt = a.tc + lb * ld;
+  const int np = a.p1 * a.n2 * a.n3;
+  for (int i = tid; i < np; i += KL_NT) grid[i] = mk(0.0, 0.0);
+  __syncthreads();
+  // new direction, scattered into the grid; bra sums of the projectors
+  const double bt = a.beta[lb];
+  double acc[KL_NQ2];
+#pragma unroll
+  for (int i = 0; i < KL_NQ2; ++i) acc[i] = 0.0;
+  for (int ig = tid; ig < ld; ig += KL_NT) {
+    const z64 v = cscale(hc[ig], -1.0) + cscale(ho[ig], bt);
+    hc[ig] = v;
+    bb[ig] = v;
+    if (ig < npw) {
+      const int r = a.nl[ig];
+      grid[r % a.n1 + a.p1 * (r / a.n1)] = v;
+#pragma unroll
+      for (int i = 0; i < KL_NQ2; ++i)
+        if (i < nq2) acc[i] += a.q2[i * ldr + 2 * ig] * v.x + a.q2[i * ldr + 2 * ig + 1] * v.y;
+    }
+  }
+  kl_block_sums(acc, nq2, red, sums);
+  if (tid < nq2) c2s[tid] = a.qs[tid] * sums[tid];
+  // (kl_block_sums ends with a barrier: the grid is complete)
+  kl_local(grid, a.n1, a.n2, a.n3, a.p1, a.vrs);
+  // t: local + kinetic + projectors; dots of the new direction
+  const double sc = 1.0 / (static_cast<double>(a.n1) * a.n2 * a.n3);
+  const double eb = a.eu[lb];
+  double dots[2] = {0.0, 0.0};
+  for (int ig = tid; ig < ld; ig += KL_NT) {
+    if (ig >= npw) {
+      tt[ig] = mk(0.0, 0.0);
+      continue;
+    }
+    const int r = a.nl[ig];
+    const z64 v = bb[ig];
+    z64 t = cscale(grid[r % a.n1 + a.p1 * (r / a.n1)], sc) + cscale(v, a.g2[ig] - eb);
+    double kr = 0.0, ki = 0.0;
+#pragma unroll
+    for (int i = 0; i < KL_NQ2; ++i)
+      if (i < nq2) {
+        kr += a.q2[i * ldr + 2 * ig] * c2s[i];
+        ki += a.q2[i * ldr + 2 * ig + 1] * c2s[i];
+      }
+    t = t + mk(kr, ki);
+    tt[ig] = t;
+    dots[0] += v.x * gc[ig].x + v.y * gc[ig].y;
+    dots[1] += v.x * t.x + v.y * t.y;
+  }
+  kl_block_sums(dots, 2, red, sums);
+  const double st = sums[0] / sums[1];
+  const double e = a.ep[col];
+  double rr[1] = {0.0};
+  for (int ig = tid; ig < ld; ig += KL_NT) {
+    const z64 v = bb[ig];
+    const z64 gn = gc[ig] - cscale(tt[ig], st);
+    xc[ig] = xc[ig] - cscale(v, st);
+    gc[ig] = gn;
+    ho[ig] = v;
+    if (ig < npw) {
+      const z64 hn = cscale(gn, 1.0 / fmax(1.0, a.g2[ig] / e));
+      hc[ig] = hn;
+      rr[0] += hn.x * gn.x + hn.y * gn.y;
+    } else {
+      hc[ig] = mk(0.0, 0.0);
+    }
+  }
+  kl_block_sums(rr, 1, red, sums);
+  if (tid == 0) a.rho[col] = sums[0];
+}
+
 constexpr size_t kKlMaxSmem = 160u * 1024u;
+// LDS the fused CG iteration uses besides the grid (its reduction arrays)
+constexpr size_t kKlCgStatic = sizeof(double) * (KL_NQ2 * (KL_NT / 64) + 2 * KL_NQ2);
 
 size_t kl_smem(int n1, int n2, int n3) { return sizeof(z64) * static_cast<size_t>(n1 | 1) * n2 * n3; }
 
@@ -146,6 +300,31 @@ size_t kl_smem(int n1, int n2, int n3) { return sizeof(z64) * static_cast<size_t
 
 bool kvloc_lds_available(int n1, int n2, int n3) {
   return kl_size(n1) && kl_size(n2) && kl_size(n3) && kl_smem(n1, n2, n3) <= kKlMaxSmem;
+}
+
+bool cg_iter_lds_available(int n1, int n2, int n3, int nq2) {
+  return kl_size(n1) && kl_size(n2) && kl_size(n3) && nq2 >= 0 && nq2 <= KL_NQ2 &&
+         kl_smem(n1, n2, n3) + kKlCgStatic <= kKlMaxSmem;
+}
+
+void cg_iter_lds(int n1, int n2, int n3, int npw, long ld, const int* nl, const double* vrs, const double* g2,
+                 int nq2, const double* q2, const double* qs, int cap, const int* l2i, const int* nc,
+                 const double* beta, const double* eu, const double* ep, z64* h, z64* hold, z64* x, z64* g, z64* bc,
+                 z64* tc, double* rho, hipStream_t s) {
+  WH_REQUIRE(cg_iter_lds_available(n1, n2, n3, nq2), "cg_iter_lds: grid or projector count not supported");
+  if (cap <= 0) return;
+  const size_t dyn = kl_smem(n1, n2, n3);
+  static const bool attr_set = [] {   // once per process (thread-safe static initialisation)
+    WH_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_cg_iter_lds),
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               static_cast<int>(kKlMaxSmem - kKlCgStatic)));
+    return true;
+  }();
+  (void)attr_set;
+  const KlCgArgs a{n1, n2, n3, n1 | 1, npw, ld, nl, vrs, g2, nq2, q2, qs, l2i, nc, beta, eu, ep,
+                   h, hold, x, g, bc, tc, rho};
+  k_cg_iter_lds<<<cap, KL_NT, dyn, s>>>(a);
+  WH_HIP(hipGetLastError());
 }
 
 void kvloc_lds(int n1, int n2, int n3, int m, int npw, long ld, const int* nl, const z64* x, const double* vrs,

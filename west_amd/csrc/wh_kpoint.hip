@@ -80,8 +80,14 @@ struct KPath {
   DevBuf w_c2, w_l2i, w_rho, w_rhoold, w_conv, w_nonconv, w_beta, w_eu, w_ecol, w_epcol;
   DevBuf w_dv, w_dn, w_in, w_out, w_in2;
   DevBuf dav[8];
-  hipEvent_t cg_ev[3] = {nullptr, nullptr, nullptr};   // lagged reads of the CG loop
+  hipEvent_t cg_ev[4] = {nullptr, nullptr, nullptr, nullptr};   // lagged reads of the CG loop
 };
+
+// The k-point CG reads an iteration's active count this many iterations late:
+// its iterations are short (tens of microseconds), so the queue keeps two
+// of them ahead of a host that wakes from an event wait.
+constexpr int kCgLag = 3;
+static_assert(sizeof(KPath::cg_ev) / sizeof(hipEvent_t) == kCgLag + 1, "one event per iteration in flight");
 
 void free_kpath(KPath* k) {
   if (!k) return;
@@ -457,6 +463,16 @@ void set_qscale2(KPath& K, double alpha) {
 // t = H_k x - e x + alpha P_v x  (apply_sternheimerop_to_m_wfcs.f90:49-76 at k:
 // QE h_psi_ = |k+G|^2 + vloc_psi_k + calbec/add_vuspsi, then alpha P_v)
 // ---------------------------------------------------------------------------
+// Small grids keep the local term in LDS (WH_KLDS=0: the full-grid
+// transforms through HBM)
+static bool klds_enabled() {
+  static const bool on = [] {
+    const char* ev = std::getenv("WH_KLDS");
+    return !(ev && ev[0] == '0');
+  }();
+  return on;
+}
+
 // d_nc (device int, may be null): only the first min(m, *d_nc) columns are
 // used (a CG launch sized for a capacity); the caller counts the applications.
 static void stern_k(KPath& K, int ik, const z64* x, z64* t, const double* d_e, int m, double alpha,
@@ -468,12 +484,7 @@ static void stern_k(KPath& K, int ik, const z64* x, z64* t, const double* d_e, i
   const long nnr = K.fft.nnr();
   const int gc = grid_cols(K);
   // small grids: the whole local term in one launch with the grid in LDS
-  // (WH_KLDS=0: the full-grid transforms through HBM)
-  static const bool klds_env = [] {
-    const char* ev = std::getenv("WH_KLDS");
-    return !(ev && ev[0] == '0');
-  }();
-  const bool lds = klds_env && !c.l_kinetic_only && kvloc_lds_available(K.fft.n1, K.fft.n2, K.fft.n3);
+  const bool lds = klds_enabled() && !c.l_kinetic_only && kvloc_lds_available(K.fft.n1, K.fft.n2, K.fft.n3);
   if (lds)
     WH_TIMED("kvloc_lds", kvloc_lds(K.fft.n1, K.fft.n2, K.fft.n3, m, P.npw, ld, P.nl, x, K.vrs, P.g2kin, d_e, t, s,
                                     d_nc));
@@ -530,9 +541,9 @@ static void minus_pc_k(KPath& K, int ik, z64* f, int m) {
 // beta at the start of each iteration, and the iteration's kernels are
 // launched for a capacity (the active count the host last read, rounded up
 // to a power of two) and skip the columns beyond the device's count.  The
-// host reads the active count of the iteration two back (finished while the
-// next one is queued) and stops once it is zero; iterations launched after
-// the last band converged do nothing.  Nothing waits for the end: the
+// host reads the active count of the iteration kCgLag back (finished while
+// the later ones are queued) and stops once it is zero; iterations launched
+// after the last band converged do nothing.  Nothing waits for the end: the
 // unconverged bands are added on the device to nonconv[band / group]
 // (group <= 0: nonconv[0]), each iteration's (active count, perturbations)
 // goes to rec[2 it] (pinned, 2 (n_dfpt_maxiter + 1) ints), and the caller,
@@ -570,12 +581,18 @@ static int linsolve_k(KPath& K, int ik, int m, z64* b, z64* x, const double* d_e
   iota(d_l2i, m, s);
   k_cg_prec_rho<<<m, NT, 0, s>>>(d_l2i, nullptr, P.npw, ld, P.g2kin, d_ep, g, h, d_rhob);
   WH_HIP(hipGetLastError());
+  // small grids with few projector rows: the iteration after cg_select is
+  // one workgroup per column (cg_iter_lds) instead of five launches
+  const int nq2 = 2 * (K.nbndval + K.nkb);
+  const bool fused = klds_enabled() && !c.l_kinetic_only &&
+                     cg_iter_lds_available(K.fft.n1, K.fft.n2, K.fft.n3, nq2);
+  if (fused) set_qscale2(K, K.alpha_pv);
   int last = 0;   // iterations launched
   int cap = m;
   for (int it = 1; it <= itmax; ++it) {
-    if (it >= 3) {
-      WH_HIP(hipEventSynchronize(K.cg_ev[(it - 2) % 3]));
-      const int nlag = rec[2 * (it - 2)];
+    if (it > kCgLag) {
+      WH_HIP(hipEventSynchronize(K.cg_ev[(it - kCgLag) % (kCgLag + 1)]));
+      const int nlag = rec[2 * (it - kCgLag)];
       if (nlag == 0) break;
       int p2 = 1;
       while (p2 < nlag) p2 <<= 1;
@@ -583,12 +600,19 @@ static int linsolve_k(KPath& K, int ik, int m, z64* b, z64* x, const double* d_e
     }
     // convergence test, compaction, beta (208-224)
     cg_select(m, tr2, it == 1 ? 1 : 0, group, d_rhob, d_rhoold, d_conv, d_e, d_l2i, d_beta, d_eu, d_nc, d_rec, s);
-    k_cg_dir<<<blocks(static_cast<long>(cap) * ld), NT, 0, s>>>(cap, d_nc, d_l2i, ld, d_beta, d_e, h, hold, bc, d_eu);
-    stern_k(K, ik, bc, tc, d_eu, cap, K.alpha_pv, d_nc);
-    k_cg_step<<<cap, NT, 0, s>>>(d_l2i, d_nc, P.npw, ld, P.g2kin, d_ep, x, g, h, tc, hold, d_rhob);
-    WH_HIP(hipGetLastError());
+    if (fused) {
+      WH_TIMED("cg_iter_lds", cg_iter_lds(K.fft.n1, K.fft.n2, K.fft.n3, P.npw, ld, P.nl, K.vrs, P.g2kin, nq2,
+                                          reinterpret_cast<const double*>(P.q2), K.qscale2, cap, d_l2i, d_nc,
+                                          d_beta, d_eu, d_ep, h, hold, x, g, bc, tc, d_rhob, s));
+    } else {
+      k_cg_dir<<<blocks(static_cast<long>(cap) * ld), NT, 0, s>>>(cap, d_nc, d_l2i, ld, d_beta, d_e, h, hold, bc,
+                                                                  d_eu);
+      stern_k(K, ik, bc, tc, d_eu, cap, K.alpha_pv, d_nc);
+      k_cg_step<<<cap, NT, 0, s>>>(d_l2i, d_nc, P.npw, ld, P.g2kin, d_ep, x, g, h, tc, hold, d_rhob);
+      WH_HIP(hipGetLastError());
+    }
     WH_HIP(hipMemcpyAsync(rec + 2 * it, d_rec, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
-    WH_HIP(hipEventRecord(K.cg_ev[it % 3], s));
+    WH_HIP(hipEventRecord(K.cg_ev[it % (kCgLag + 1)], s));
     last = it;
   }
   // the bands still active when the iterations ran out (332)
