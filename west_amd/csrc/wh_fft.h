@@ -170,11 +170,20 @@ void kvloc_lds(int n1, int n2, int n3, int m, int npw, long ld, const int* nl, c
 // the local term in LDS and the projector block q2 (real view, 2 ld rows,
 // nq2 <= 32 columns, row scales qs), the CG step on x, g, hold, and the next
 // preconditioned residual h and rho[band] (wh_kfft_lds.hip)
+// Columns of several k points in one launch: ktab (device, may be null) holds
+// each k point's sphere and projectors, and band column col belongs to k point
+// col / cols_per_k (ktab null: npw, nl, g2 and q2 for every column).
+struct KlKpoint {
+  int npw;
+  const int* nl;
+  const double* g2;
+  const double* q2;
+};
 bool cg_iter_lds_available(int n1, int n2, int n3, int nq2);
 void cg_iter_lds(int n1, int n2, int n3, int npw, long ld, const int* nl, const double* vrs, const double* g2,
                  int nq2, const double* q2, const double* qs, int cap, const int* l2i, const int* nc,
                  const double* beta, const double* eu, const double* ep, z64* h, z64* hold, z64* x, z64* g, z64* bc,
-                 z64* tc, double* rho, hipStream_t s);
+                 z64* tc, double* rho, hipStream_t s, const KlKpoint* ktab = nullptr, int cols_per_k = 1);
 
 // kernel launchers (double precision)
 void launch_zinv(const StickPlan& p, const FftItem* d_items, int nitems, const z64* psi, long ldpsi,

@@ -198,6 +198,8 @@ struct KlCgArgs {
   z64* bc;
   z64* tc;
   double* rho;
+  const KlKpoint* ktab;
+  int cols_per_k;
 };
 
 __global__ __launch_bounds__(KL_NT) void k_cg_iter_lds(KlCgArgs a) {
@@ -211,7 +213,17 @@ __global__ __launch_bounds__(KL_NT) void k_cg_iter_lds(KlCgArgs a) {
   if (lb >= *a.nc) return;   // whole workgroup
   const int col = a.l2i[lb];
   const long ld = a.ld;
-  const int npw = a.npw;
+  int npw = a.npw;
+  const int* nl = a.nl;
+  const double* g2 = a.g2;
+  const double* q2 = a.q2;
+  if (a.ktab) {   // the column's k point
+    const KlKpoint kp = a.ktab[col / a.cols_per_k];
+    npw = kp.npw;
+    nl = kp.nl;
+    g2 = kp.g2;
+    q2 = kp.q2;
+  }
   const int nq2 = a.nq2;
   const long ldr = 2 * ld;   // real view
   z64* hc = a.h + col * ld;
@@ -233,11 +245,11 @@ __global__ __launch_bounds__(KL_NT) void k_cg_iter_lds(KlCgArgs a) {
     hc[ig] = v;
     bb[ig] = v;
     if (ig < npw) {
-      const int r = a.nl[ig];
+      const int r = nl[ig];
       grid[r % a.n1 + a.p1 * (r / a.n1)] = v;
 #pragma unroll
       for (int i = 0; i < KL_NQ2; ++i)
-        if (i < nq2) acc[i] += a.q2[i * ldr + 2 * ig] * v.x + a.q2[i * ldr + 2 * ig + 1] * v.y;
+        if (i < nq2) acc[i] += q2[i * ldr + 2 * ig] * v.x + q2[i * ldr + 2 * ig + 1] * v.y;
     }
   }
   kl_block_sums(acc, nq2, red, sums);
@@ -253,15 +265,15 @@ __global__ __launch_bounds__(KL_NT) void k_cg_iter_lds(KlCgArgs a) {
       tt[ig] = mk(0.0, 0.0);
       continue;
     }
-    const int r = a.nl[ig];
+    const int r = nl[ig];
     const z64 v = bb[ig];
-    z64 t = cscale(grid[r % a.n1 + a.p1 * (r / a.n1)], sc) + cscale(v, a.g2[ig] - eb);
+    z64 t = cscale(grid[r % a.n1 + a.p1 * (r / a.n1)], sc) + cscale(v, g2[ig] - eb);
     double kr = 0.0, ki = 0.0;
 #pragma unroll
     for (int i = 0; i < KL_NQ2; ++i)
       if (i < nq2) {
-        kr += a.q2[i * ldr + 2 * ig] * c2s[i];
-        ki += a.q2[i * ldr + 2 * ig + 1] * c2s[i];
+        kr += q2[i * ldr + 2 * ig] * c2s[i];
+        ki += q2[i * ldr + 2 * ig + 1] * c2s[i];
       }
     t = t + mk(kr, ki);
     tt[ig] = t;
@@ -279,7 +291,7 @@ __global__ __launch_bounds__(KL_NT) void k_cg_iter_lds(KlCgArgs a) {
     gc[ig] = gn;
     ho[ig] = v;
     if (ig < npw) {
-      const z64 hn = cscale(gn, 1.0 / fmax(1.0, a.g2[ig] / e));
+      const z64 hn = cscale(gn, 1.0 / fmax(1.0, g2[ig] / e));
       hc[ig] = hn;
       rr[0] += hn.x * gn.x + hn.y * gn.y;
     } else {
@@ -310,8 +322,9 @@ bool cg_iter_lds_available(int n1, int n2, int n3, int nq2) {
 void cg_iter_lds(int n1, int n2, int n3, int npw, long ld, const int* nl, const double* vrs, const double* g2,
                  int nq2, const double* q2, const double* qs, int cap, const int* l2i, const int* nc,
                  const double* beta, const double* eu, const double* ep, z64* h, z64* hold, z64* x, z64* g, z64* bc,
-                 z64* tc, double* rho, hipStream_t s) {
+                 z64* tc, double* rho, hipStream_t s, const KlKpoint* ktab, int cols_per_k) {
   WH_REQUIRE(cg_iter_lds_available(n1, n2, n3, nq2), "cg_iter_lds: grid or projector count not supported");
+  WH_REQUIRE(!ktab || cols_per_k > 0, "cg_iter_lds: cols_per_k must be > 0");
   if (cap <= 0) return;
   const size_t dyn = kl_smem(n1, n2, n3);
   static const bool attr_set = [] {   // once per process (thread-safe static initialisation)
@@ -322,7 +335,7 @@ void cg_iter_lds(int n1, int n2, int n3, int npw, long ld, const int* nl, const 
   }();
   (void)attr_set;
   const KlCgArgs a{n1, n2, n3, n1 | 1, npw, ld, nl, vrs, g2, nq2, q2, qs, l2i, nc, beta, eu, ep,
-                   h, hold, x, g, bc, tc, rho};
+                   h, hold, x, g, bc, tc, rho, ktab, cols_per_k};
   k_cg_iter_lds<<<cap, KL_NT, dyn, s>>>(a);
   WH_HIP(hipGetLastError());
 }

@@ -81,6 +81,7 @@ struct KPath {
   DevBuf w_dv, w_dn, w_in, w_out, w_in2;
   DevBuf dav[8];
   hipEvent_t cg_ev[4] = {nullptr, nullptr, nullptr, nullptr};   // lagged reads of the CG loop
+  KlKpoint* ktab = nullptr;   // device: each k point's sphere and projectors (batched CG)
 };
 
 // The k-point CG reads an iteration's active count this many iterations late:
@@ -342,10 +343,10 @@ __global__ __launch_bounds__(NT) void k_cg_step(const int* __restrict__ l2i, con
   if (threadIdx.x == 0) rho[col] = sr;
 }
 
-// nonconv[j / group] += !conv[j]  (group <= 0: one total)
-__global__ void k_count_nonconv(int m, int group, const int* __restrict__ conv, int* __restrict__ nonconv) {
+// nonconv[(j / group) % nper] += !conv[j]  (group <= 0: one total)
+__global__ void k_count_nonconv(int m, int group, int nper, const int* __restrict__ conv, int* __restrict__ nonconv) {
   const int j = blockIdx.x * NT + threadIdx.x;
-  if (j < m && !conv[j]) atomicAdd(nonconv + (group > 0 ? j / group : 0), 1);
+  if (j < m && !conv[j]) atomicAdd(nonconv + (group > 0 ? (j / group) % nper : 0), 1);
 }
 
 // a[j * ld] = 0, j < m
@@ -544,16 +545,21 @@ static void minus_pc_k(KPath& K, int ik, z64* f, int m) {
 // host reads the active count of the iteration kCgLag back (finished while
 // the later ones are queued) and stops once it is zero; iterations launched
 // after the last band converged do nothing.  Nothing waits for the end: the
-// unconverged bands are added on the device to nonconv[band / group]
+// unconverged bands are added on the device to nonconv[(band / group) % nper]
 // (group <= 0: nonconv[0]), each iteration's (active count, perturbations)
 // goes to rec[2 it] (pinned, 2 (n_dfpt_maxiter + 1) ints), and the caller,
 // once the stream is done, passes rec and the returned number of iterations
 // launched to cg_k_count.
-static int linsolve_k(KPath& K, int ik, int m, z64* b, z64* x, const double* d_e, const double* d_ep, double tr2,
-                      int group, int* rec, int* nonconv) {
+// nkb = 1: the cols columns belong to k point ik.  nkb = nks: columns
+// [k cols, (k + 1) cols) belong to k point k, all in one solve (the fused
+// iteration only, kbatch_ok).
+static int linsolve_k(KPath& K, int ik, int nkb, int cols, z64* b, z64* x, const double* d_e, const double* d_ep,
+                      double tr2, int group, int nper, int* rec, int* nonconv) {
   Context& c = ctx();
   hipStream_t s = c.stream;
-  const KPointDev& P = K.k[ik];
+  const int m = nkb * cols;
+  WH_REQUIRE(nkb == 1 || nkb == K.nks, "linsolve_k: one k point or all of them");
+  const KPointDev& P = K.k[nkb == 1 ? ik : 0];
   const long ld = K.npwx;
   const size_t vec = static_cast<size_t>(m) * ld;
   z64* g = K.w_g.as<z64>(vec);
@@ -575,17 +581,25 @@ static int linsolve_k(KPath& K, int ik, int m, z64* b, z64* x, const double* d_e
   WH_HIP(hipMemsetAsync(hold, 0, sizeof(z64) * vec, s));
   WH_HIP(hipMemsetAsync(d_conv, 0, sizeof(int) * (static_cast<size_t>(m) + 4), s));
   // step 1 (127-138): g = A x - b, and h = M^-1 g, rho of every band
-  stern_k(K, ik, x, g, d_e, m, K.alpha_pv);
+  iota(d_l2i, m, s);
+  for (int kb = 0; kb < nkb; ++kb) {
+    const int kk = nkb == 1 ? ik : kb;
+    const long o = static_cast<long>(kb) * cols;
+    stern_k(K, kk, x + o * ld, g + o * ld, d_e + o, cols, K.alpha_pv);
+  }
   count_calls(m, nullptr, group);
   k_sub<<<blocks(static_cast<long>(vec)), NT, 0, s>>>(static_cast<long>(vec), b, g);
-  iota(d_l2i, m, s);
-  k_cg_prec_rho<<<m, NT, 0, s>>>(d_l2i, nullptr, P.npw, ld, P.g2kin, d_ep, g, h, d_rhob);
+  for (int kb = 0; kb < nkb; ++kb) {
+    const KPointDev& Pk = K.k[nkb == 1 ? ik : kb];
+    k_cg_prec_rho<<<cols, NT, 0, s>>>(d_l2i + kb * cols, nullptr, Pk.npw, ld, Pk.g2kin, d_ep, g, h, d_rhob);
+  }
   WH_HIP(hipGetLastError());
   // small grids with few projector rows: the iteration after cg_select is
   // one workgroup per column (cg_iter_lds) instead of five launches
   const int nq2 = 2 * (K.nbndval + K.nkb);
   const bool fused = klds_enabled() && !c.l_kinetic_only &&
                      cg_iter_lds_available(K.fft.n1, K.fft.n2, K.fft.n3, nq2);
+  WH_REQUIRE(fused || nkb == 1, "linsolve_k: several k points need the fused iteration");
   if (fused) set_qscale2(K, K.alpha_pv);
   int last = 0;   // iterations launched
   int cap = m;
@@ -603,7 +617,8 @@ static int linsolve_k(KPath& K, int ik, int m, z64* b, z64* x, const double* d_e
     if (fused) {
       WH_TIMED("cg_iter_lds", cg_iter_lds(K.fft.n1, K.fft.n2, K.fft.n3, P.npw, ld, P.nl, K.vrs, P.g2kin, nq2,
                                           reinterpret_cast<const double*>(P.q2), K.qscale2, cap, d_l2i, d_nc,
-                                          d_beta, d_eu, d_ep, h, hold, x, g, bc, tc, d_rhob, s));
+                                          d_beta, d_eu, d_ep, h, hold, x, g, bc, tc, d_rhob, s,
+                                          nkb > 1 ? K.ktab : nullptr, cols));
     } else {
       k_cg_dir<<<blocks(static_cast<long>(cap) * ld), NT, 0, s>>>(cap, d_nc, d_l2i, ld, d_beta, d_e, h, hold, bc,
                                                                   d_eu);
@@ -616,7 +631,7 @@ static int linsolve_k(KPath& K, int ik, int m, z64* b, z64* x, const double* d_e
     last = it;
   }
   // the bands still active when the iterations ran out (332)
-  k_count_nonconv<<<blocks(m), NT, 0, s>>>(m, group, d_conv, nonconv);
+  k_count_nonconv<<<blocks(m), NT, 0, s>>>(m, group, nper, d_conv, nonconv);
   WH_HIP(hipGetLastError());
   return last;
 }
@@ -632,6 +647,17 @@ static void cg_k_count(const int* rec, int last) {
       c.nstern_calls += rec[2 * it + 1];
     }
   }
+}
+
+// Whether the CG of every k point's cols columns can run as one solve: the
+// fused iteration applies (small grid, at most 32 projector rows in the real
+// view) and the solve's vectors (seven of nks cols columns) fit the grid budget.
+static bool kbatch_ok(const KPath& K, int cols) {
+  const Context& c = ctx();
+  const long all = static_cast<long>(K.nks) * cols;
+  return K.nks > 1 && klds_enabled() && !c.l_kinetic_only && all <= (1L << 24) &&
+         cg_iter_lds_available(K.fft.n1, K.fft.n2, K.fft.n3, 2 * (K.nbndval + K.nkb)) &&
+         7 * all * K.npwx * static_cast<long>(sizeof(z64)) <= static_cast<long>(K.grid_budget);
 }
 
 // dng (npwqx, m) = chi0(q) dvg for q point iq (0-based)  (dfpt_module.f90:163-505)
@@ -662,11 +688,9 @@ static int dfpt_q(KPath& K, int iq, int m, const z64* dvg, z64* dng, double tr2,
   zero_grids(aux, m * nnr, s);
   k_scatter<<<blocks(static_cast<long>(m) * Q.npwq), NT, 0, s>>>(m, Q.npwq, Q.nl, dvg, ldq, aux, nnr);
   WH_TIMED("kfft", launch_grid_fft(K.fft, aux, m, +1, s));
-  for (int ik = 0; ik < K.nks; ++ik) {
-    const KPointDev& P = K.k[ik];
-    const int ikq = Q.ikq[ik];
-    const KPointDev& PQ = K.k[ikq];
-    // states at k - q in real space with the phase exp(-i g0.r) (288, 295)
+  // states at k - q in real space with the phase exp(-i g0.r) (288, 295)
+  auto states_kmq = [&](int ik) {
+    const KPointDev& PQ = K.k[Q.ikq[ik]];
     z64* psik = K.g_b.as<z64>(static_cast<size_t>(nb) * nnr);
     zero_grids(psik, nb * nnr, s);
     k_scatter<<<blocks(static_cast<long>(nb) * PQ.npw), NT, 0, s>>>(nb, PQ.npw, PQ.nl, PQ.q2, ld, psik, nnr);
@@ -674,43 +698,79 @@ static int dfpt_q(KPath& K, int iq, int m, const z64* dvg, z64* dng, double tr2,
     k_phase<<<blocks(nb * nnr), NT, 0, s>>>(nb, K.fft.n1, K.fft.n2, K.fft.n3, Q.g0[3 * ik], Q.g0[3 * ik + 1],
                                            Q.g0[3 * ik + 2], psik);
     WH_HIP(hipGetLastError());
-    for (int p0 = 0; p0 < m; p0 += mp) {
-      const int mc = std::min(mp, m - p0);
-      const int cols = mc * nb;
-      // per column: e at k - q (222), eprec from the states at k (211)
-      double* d_e = K.w_ecol.as<double>(cols);
-      double* d_ep = K.w_epcol.as<double>(cols);
-      k_band_cols<<<blocks(cols), NT, 0, s>>>(cols, nb, PQ.d_et, P.d_eprec, d_e, d_ep);
-      // dvpsi = fwfft(psi_{k-q} phase dv) on the k sphere (295-302)
-      z64* R = K.g_a.as<z64>(static_cast<size_t>(cols) * nnr);
-      k_rhs<<<blocks(static_cast<long>(cols) * nnr), NT, 0, s>>>(nb, mc, nnr, psik, aux + p0 * nnr, R);
-      WH_TIMED("kfft", launch_grid_fft(K.fft, R, cols, -1, s));
-      z64* dvpsi = K.w_dv.as<z64>(static_cast<size_t>(cols) * ld);
-      k_gather<<<blocks(static_cast<long>(cols) * ld), NT, 0, s>>>(cols, P.npw, ld, P.nl, R, nnr,
-                                                                   1.0 / static_cast<double>(nnr), nullptr, nullptr,
-                                                                   nullptr, 0, dvpsi);
-      WH_HIP(hipGetLastError());
-      minus_pc_k(K, ik, dvpsi, cols);                                    // (346)
-      z64* dpsi = K.w_x.as<z64>(static_cast<size_t>(cols) * ld);
-      k_precond<<<blocks(static_cast<long>(cols) * ld), NT, 0, s>>>(cols, P.npw, ld, P.g2kin, d_ep, dvpsi, dpsi);
-      WH_HIP(hipGetLastError());
-      if (tr2 >= 0.0) {                                                  // (350-361)
-        int* rec = recs + rec_len * launched.size();
-        launched.push_back(linsolve_k(K, ik, cols, dvpsi, dpsi, d_e, d_ep, tr2, nb, rec, d_nonconv + p0));
+    return psik;
+  };
+  // perturbations p0 .. p0 + mc - 1 at k point ik: per column e at k - q
+  // (222) and eprec from the states at k (211); dvpsi = -P_c fwfft(psi_{k-q}
+  // phase dv) on the k sphere (295-302, 346) and dpsi = M^-1 dvpsi (348)
+  auto rhs = [&](int ik, int p0, int mc, const z64* psik, z64* dvpsi, z64* dpsi, double* d_e, double* d_ep) {
+    const KPointDev& P = K.k[ik];
+    const KPointDev& PQ = K.k[Q.ikq[ik]];
+    const int cols = mc * nb;
+    k_band_cols<<<blocks(cols), NT, 0, s>>>(cols, nb, PQ.d_et, P.d_eprec, d_e, d_ep);
+    z64* R = K.g_a.as<z64>(static_cast<size_t>(cols) * nnr);
+    k_rhs<<<blocks(static_cast<long>(cols) * nnr), NT, 0, s>>>(nb, mc, nnr, psik, aux + p0 * nnr, R);
+    WH_TIMED("kfft", launch_grid_fft(K.fft, R, cols, -1, s));
+    k_gather<<<blocks(static_cast<long>(cols) * ld), NT, 0, s>>>(cols, P.npw, ld, P.nl, R, nnr,
+                                                                 1.0 / static_cast<double>(nnr), nullptr, nullptr,
+                                                                 nullptr, 0, dvpsi);
+    WH_HIP(hipGetLastError());
+    minus_pc_k(K, ik, dvpsi, cols);
+    k_precond<<<blocks(static_cast<long>(cols) * ld), NT, 0, s>>>(cols, P.npw, ld, P.g2kin, d_ep, dvpsi, dpsi);
+    WH_HIP(hipGetLastError());
+  };
+  // response density sum_v conj(psi_{k-q} phase) dpsi (433-441) and
+  // dng += 2 wk / Omega fwfft(aux) on the q sphere (480-487)
+  auto density = [&](int ik, int p0, int mc, const z64* psik, const z64* dpsi) {
+    const KPointDev& P = K.k[ik];
+    const int cols = mc * nb;
+    z64* Rd = K.g_a.as<z64>(static_cast<size_t>(cols) * nnr);
+    zero_grids(Rd, static_cast<long>(cols) * nnr, s);
+    k_scatter<<<blocks(static_cast<long>(cols) * P.npw), NT, 0, s>>>(cols, P.npw, P.nl, dpsi, ld, Rd, nnr);
+    WH_TIMED("kfft", launch_grid_fft(K.fft, Rd, cols, +1, s));
+    z64* D = K.g_c.as<z64>(static_cast<size_t>(mc) * nnr);
+    k_density<<<blocks(static_cast<long>(mc) * nnr), NT, 0, s>>>(nb, mc, nnr, psik, Rd, D);
+    WH_TIMED("kfft", launch_grid_fft(K.fft, D, mc, -1, s));
+    const double sc = 2.0 * K.wk[ik] / K.omega / static_cast<double>(nnr);
+    k_gather<<<blocks(static_cast<long>(mc) * Q.npwq), NT, 0, s>>>(mc, Q.npwq, ldq, Q.nl, D, nnr, sc, nullptr,
+                                                                   nullptr, nullptr, 1, dng + p0 * ldq);
+    WH_HIP(hipGetLastError());
+  };
+  if (tr2 >= 0.0 && mp == m && kbatch_ok(K, m * nb)) {
+    // every k point's columns in one CG (350-361): the right-hand sides of
+    // all k points, one solve whose iterations take them all at once, then
+    // the densities (the states at k - q are transformed again)
+    const int cols = m * nb;
+    const size_t all = static_cast<size_t>(K.nks) * cols;
+    z64* dvpsi = K.w_dv.as<z64>(all * ld);
+    z64* dpsi = K.w_x.as<z64>(all * ld);
+    double* d_e = K.w_ecol.as<double>(all);
+    double* d_ep = K.w_epcol.as<double>(all);
+    for (int ik = 0; ik < K.nks; ++ik) {
+      const size_t o = static_cast<size_t>(ik) * cols;
+      rhs(ik, 0, m, states_kmq(ik), dvpsi + o * ld, dpsi + o * ld, d_e + o, d_ep + o);
+    }
+    launched.push_back(linsolve_k(K, 0, K.nks, cols, dvpsi, dpsi, d_e, d_ep, tr2, nb, m, recs, d_nonconv));
+    for (int ik = 0; ik < K.nks; ++ik)
+      density(ik, 0, m, states_kmq(ik), dpsi + static_cast<size_t>(ik) * cols * ld);
+  } else {
+    for (int ik = 0; ik < K.nks; ++ik) {
+      const z64* psik = states_kmq(ik);
+      for (int p0 = 0; p0 < m; p0 += mp) {
+        const int mc = std::min(mp, m - p0);
+        const int cols = mc * nb;
+        double* d_e = K.w_ecol.as<double>(cols);
+        double* d_ep = K.w_epcol.as<double>(cols);
+        z64* dvpsi = K.w_dv.as<z64>(static_cast<size_t>(cols) * ld);
+        z64* dpsi = K.w_x.as<z64>(static_cast<size_t>(cols) * ld);
+        rhs(ik, p0, mc, psik, dvpsi, dpsi, d_e, d_ep);
+        if (tr2 >= 0.0) {                                                // (350-361)
+          int* rec = recs + rec_len * launched.size();
+          launched.push_back(linsolve_k(K, ik, 1, cols, dvpsi, dpsi, d_e, d_ep, tr2, nb, mc, rec,
+                                        d_nonconv + p0));
+        }
+        density(ik, p0, mc, psik, dpsi);
       }
-      // response density sum_v conj(psi_{k-q} phase) dpsi (433-441)
-      z64* Rd = K.g_a.as<z64>(static_cast<size_t>(cols) * nnr);
-      zero_grids(Rd, static_cast<long>(cols) * nnr, s);
-      k_scatter<<<blocks(static_cast<long>(cols) * P.npw), NT, 0, s>>>(cols, P.npw, P.nl, dpsi, ld, Rd, nnr);
-      WH_TIMED("kfft", launch_grid_fft(K.fft, Rd, cols, +1, s));
-      z64* D = K.g_c.as<z64>(static_cast<size_t>(mc) * nnr);
-      k_density<<<blocks(static_cast<long>(mc) * nnr), NT, 0, s>>>(nb, mc, nnr, psik, Rd, D);
-      WH_TIMED("kfft", launch_grid_fft(K.fft, D, mc, -1, s));
-      // dng += 2 wk / Omega fwfft(aux) on the q sphere (480-487)
-      const double sc = 2.0 * K.wk[ik] / K.omega / static_cast<double>(nnr);
-      k_gather<<<blocks(static_cast<long>(mc) * Q.npwq), NT, 0, s>>>(mc, Q.npwq, ldq, Q.nl, D, nnr, sc, nullptr,
-                                                                     nullptr, nullptr, 1, dng + p0 * ldq);
-      WH_HIP(hipGetLastError());
     }
   }
   if (Q.is_gamma) {   // (505): the perturbation acts on the body only
@@ -798,7 +858,7 @@ int kpath_linsolve(int nbndval, int m, double* b, double* x, const double* e, co
   int* d_nonconv = K.w_nonconv.as<int>(1);
   WH_HIP(hipMemsetAsync(d_nonconv, 0, sizeof(int), s));
   int* rec = c.h_cgrec.as<int>(2 * (static_cast<size_t>(c.n_dfpt_maxiter) + 1));
-  const int last = linsolve_k(K, ik, m, db, dx, de, dep, tr2, 0, rec, d_nonconv);
+  const int last = linsolve_k(K, ik, 1, m, db, dx, de, dep, tr2, 0, 1, rec, d_nonconv);
   int nc = 0;
   WH_HIP(hipMemcpyAsync(&nc, d_nonconv, sizeof(int), hipMemcpyDeviceToHost, s));
   WH_HIP(hipStreamSynchronize(s));
@@ -1021,6 +1081,12 @@ int wh_set_kpoint(int ik, int npw, const int* nl, const double* g2kin, const dou
   P.q2 = upload(K, q2.data(), q2.size());
   P.d_et = upload(K, P.et.data(), P.et.size());
   P.d_eprec = upload(K, P.eprec.data(), P.eprec.size());
+  if (!K.ktab) {
+    const std::vector<KlKpoint> none(K.nks, KlKpoint{0, nullptr, nullptr, nullptr});
+    K.ktab = upload(K, none.data(), none.size());
+  }
+  const KlKpoint kt{npw, P.nl, P.g2kin, reinterpret_cast<const double*>(P.q2)};
+  WH_HIP(hipMemcpy(K.ktab + (ik - 1), &kt, sizeof(kt), hipMemcpyHostToDevice));
   P.set = true;
   return 0;
   WH_K_END
