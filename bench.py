@@ -375,15 +375,20 @@ def _qpoint_measure(wstat, torch, args, with_cpu=True, nk=(3, 3, 3), npert=16, e
         for _ in range(steps):
             wstat.dfpt(npert, dvg, args.tr2, iq=iq + 1, dng=dng)
         torch.cuda.synchronize()
-        kern = {n: wstat.profile_query(n) for n in ("kfft", "kvloc_lds", "cg_iter_lds", "gemm_bra", "gemm_ket")}
+        kern = {n: wstat.profile_query(n) for n in ("kfft", "kgrid_lds", "kvloc_lds", "cg_iter_lds", "gemm_bra",
+                                                    "gemm_ket")}
         wstat.profile(False)
         pairs = npert * nb * nks
         # grid columns through the full-grid transforms in HBM (three passes,
-        # read + write each); the operator's own columns take the LDS-resident
-        # kernels when the grid allows it (the first application as kvloc_lds,
-        # the CG iterations as cg_iter_lds) and are then not among them
+        # read + write each): the perturbations, and unless every k point
+        # runs at once on LDS grids (kgrid_lds: states at k - q, right-hand
+        # sides, densities) those per k point; the operator's own columns take
+        # the LDS-resident kernels when the grid allows it (the first
+        # application as kvloc_lds, the CG iterations as cg_iter_lds) and are
+        # then not among them either
         lds_op = kern["kvloc_lds"][1] > 0 or kern["cg_iter_lds"][1] > 0
-        fft_cols = npert + nks * (nb + 2 * npert * nb + npert) + (0 if lds_op else 2 * op_cols)
+        lds_grids = kern["kgrid_lds"][1] > 0
+        fft_cols = npert + (0 if lds_grids else nks * (nb + 2 * npert * nb + npert)) + (0 if lds_op else 2 * op_cols)
         fft_bytes = fft_cols * 6 * 16 * nnr
         kfft_ms = kern["kfft"][0] / steps
         rec = {"workload": f"H2 molecular crystal (2 H2, 6.5x6.0x5.5 bohr triclinic, s projectors), {ecut:g} Ry, "

@@ -180,6 +180,22 @@ struct KlKpoint {
   const double* q2;
 };
 bool cg_iter_lds_available(int n1, int n2, int n3, int nq2);
+// The grids of dfpt at q on LDS grids for every k point in one launch each
+// (kvloc_lds_available grids; qtab: per k point its k - q index and g0):
+// states_kmq_lds: psik[k][v] = F^-1[evc_v at k - q] exp(-i g0.r) (nks nb grids);
+// rhs_lds: out[k m nb + p nb + v] = (1/N_r) F[psik[k][v] aux[p]] on the k sphere;
+// density_lds: dng[p] += sum_k sck[k] F[sum_v conj(psik[k][v]) F^-1[dpsi[k m nb + p nb + v]]]
+// on the q sphere (dsc: nks m grids of scratch, dk: nks m columns of ldq).
+struct KlQpoint {
+  int ikq, g1, g2, g3;
+};
+void states_kmq_lds(int n1, int n2, int n3, int nks, int nb, long ld, const KlKpoint* ktab, const KlQpoint* qtab,
+                    z64* psik, hipStream_t s);
+void rhs_lds(int n1, int n2, int n3, int nks, int nb, int m, long ld, const KlKpoint* ktab, const z64* psik,
+             const z64* aux, z64* out, hipStream_t s);
+void density_lds(int n1, int n2, int n3, int nks, int nb, int m, long ld, const KlKpoint* ktab, const z64* psik,
+                 const z64* dpsi, z64* dsc, int npwq, long ldq, const int* nlq, const double* sck, z64* dk, z64* dng,
+                 hipStream_t s);
 void cg_iter_lds(int n1, int n2, int n3, int npw, long ld, const int* nl, const double* vrs, const double* g2,
                  int nq2, const double* q2, const double* qs, int cap, const int* l2i, const int* nc,
                  const double* beta, const double* eu, const double* ep, z64* h, z64* hold, z64* x, z64* g, z64* bc,

@@ -108,6 +108,30 @@ __device__ __forceinline__ void kl_local(z64* grid, int n1, int n2, int n3, int 
   __syncthreads();
 }
 
+// 3D transform of the LDS grid (DIR +1 inverse, -1 forward, unnormalised),
+// one line per thread in each of the three passes (barrier after each)
+template <int DIR>
+__device__ __forceinline__ void kl_transform(z64* grid, int n1, int n2, int n3, int p1) {
+  const int tid = threadIdx.x;
+  for (int l = tid; l < n2 * n3; l += KL_NT) kl_pass<DIR>(n1, grid + p1 * l, 1);
+  __syncthreads();
+  for (int l = tid; l < n1 * n3; l += KL_NT) {
+    const int i1 = l % n1, i3 = l / n1;
+    kl_pass<DIR>(n2, grid + kl_at(i1, 0, i3, p1, n2), p1);
+  }
+  __syncthreads();
+  for (int l = tid; l < n1 * n2; l += KL_NT) {
+    const int i1 = l % n1, i2 = l / n1;
+    kl_pass<DIR>(n3, grid + kl_at(i1, i2, 0, p1, n2), p1 * n2);
+  }
+  __syncthreads();
+}
+
+// LDS index of the grid point with linear index r (r = i1 + n1 (i2 + n2 i3))
+__device__ __forceinline__ int kl_lin(long r, int n1, int p1) {
+  return static_cast<int>(r % n1) + p1 * static_cast<int>(r / n1);
+}
+
 __global__ __launch_bounds__(KL_NT) void k_kvloc_lds(int n1, int n2, int n3, int p1, int npw, long ld,
                                                    const int* __restrict__ nl, const z64* __restrict__ x,
                                                    const double* __restrict__ vrs, const double* __restrict__ g2,
@@ -302,6 +326,124 @@ __global__ __launch_bounds__(KL_NT) void k_cg_iter_lds(KlCgArgs a) {
   if (tid == 0) a.rho[col] = sums[0];
 }
 
+// States at k - q in real space with the phase exp(-i g0.r) for every k
+// point (dfpt_module.f90:288-295), workgroup kq * nb + v:
+//   psik[kq][v](r) = F^-1[evc_v at k - q](r) exp(-i g0.r)
+// The phase is taken from integer numerators (Tools/types_bz_grid.f90:71-113).
+__global__ __launch_bounds__(KL_NT) void k_states_lds(int n1, int n2, int n3, int p1, int nb, long ld,
+                                                    const KlKpoint* __restrict__ ktab,
+                                                    const KlQpoint* __restrict__ qtab, z64* __restrict__ psik) {
+  extern __shared__ char smem_raw[];
+  z64* grid = reinterpret_cast<z64*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int kq = blockIdx.x / nb;
+  const int v = blockIdx.x - kq * nb;
+  const KlQpoint qk = qtab[kq];
+  const KlKpoint pk = ktab[qk.ikq];
+  const z64* evc = reinterpret_cast<const z64*>(pk.q2) + static_cast<long>(v) * ld;
+  const int np = p1 * n2 * n3;
+  for (int i = tid; i < np; i += KL_NT) grid[i] = mk(0.0, 0.0);
+  __syncthreads();
+  for (int ig = tid; ig < pk.npw; ig += KL_NT) grid[kl_lin(pk.nl[ig], n1, p1)] = evc[ig];
+  __syncthreads();
+  kl_transform<+1>(grid, n1, n2, n3, p1);
+  const long nnr = static_cast<long>(n1) * n2 * n3;
+  z64* out = psik + static_cast<long>(blockIdx.x) * nnr;
+  for (long r = tid; r < nnr; r += KL_NT) {
+    const int i = static_cast<int>(r % n1);
+    const int j = static_cast<int>((r / n1) % n2);
+    const int l = static_cast<int>(r / (static_cast<long>(n1) * n2));
+    const long a1 = ((static_cast<long>(qk.g1) * i) % n1 + n1) % n1;
+    const long a2 = ((static_cast<long>(qk.g2) * j) % n2 + n2) % n2;
+    const long a3 = ((static_cast<long>(qk.g3) * l) % n3 + n3) % n3;
+    const double x = 2.0 * (static_cast<double>(a1) / n1 + static_cast<double>(a2) / n2 + static_cast<double>(a3) / n3);
+    double sn, cs;
+    sincospi(x, &sn, &cs);
+    out[r] = cmul(grid[kl_lin(r, n1, p1)], mk(cs, -sn));
+  }
+}
+
+// Right-hand sides on the k spheres (dfpt_module.f90:295-302), workgroup
+// kq * cols + c with c = p nb + v:
+//   out[kq cols + c][ig] = (1/N_r) F[psik[kq][v] aux[p]](nl_kq[ig]), 0 for npw <= ig < ld
+__global__ __launch_bounds__(KL_NT) void k_rhs_lds(int n1, int n2, int n3, int p1, int nb, int cols, long ld,
+                                                 const KlKpoint* __restrict__ ktab, const z64* __restrict__ psik,
+                                                 const z64* __restrict__ aux, z64* __restrict__ out) {
+  extern __shared__ char smem_raw[];
+  z64* grid = reinterpret_cast<z64*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int kq = blockIdx.x / cols;
+  const int c = blockIdx.x - kq * cols;
+  const int p = c / nb;
+  const int v = c - p * nb;
+  const long nnr = static_cast<long>(n1) * n2 * n3;
+  const z64* ps = psik + (static_cast<long>(kq) * nb + v) * nnr;
+  const z64* av = aux + static_cast<long>(p) * nnr;
+  for (long r = tid; r < nnr; r += KL_NT) grid[kl_lin(r, n1, p1)] = cmul(ps[r], av[r]);
+  __syncthreads();
+  kl_transform<-1>(grid, n1, n2, n3, p1);
+  const KlKpoint pk = ktab[kq];
+  const double sc = 1.0 / static_cast<double>(nnr);
+  z64* o = out + static_cast<long>(blockIdx.x) * ld;
+  for (int ig = tid; ig < ld; ig += KL_NT)
+    o[ig] = ig < pk.npw ? cscale(grid[kl_lin(pk.nl[ig], n1, p1)], sc) : mk(0.0, 0.0);
+}
+
+// Response density of perturbation p at k point kq on the q sphere
+// (dfpt_module.f90:433-441, 480-487), workgroup kq * m + p:
+//   D(r) = sum_v conj(psik[kq][v](r)) F^-1[scatter_kq(dpsi[kq cols + p nb + v])](r)
+//   out[kq m + p][igq] = sck[kq] F[D](nlq[igq]) for igq < npwq
+// D is kept in dsc (one grid per workgroup) between the bands.
+__global__ __launch_bounds__(KL_NT) void k_density_lds(int n1, int n2, int n3, int p1, int nb, int m, long ld,
+                                                     const KlKpoint* __restrict__ ktab, const z64* __restrict__ psik,
+                                                     const z64* __restrict__ dpsi, z64* __restrict__ dsc, int npwq,
+                                                     long ldq, const int* __restrict__ nlq,
+                                                     const double* __restrict__ sck, z64* __restrict__ out) {
+  extern __shared__ char smem_raw[];
+  z64* grid = reinterpret_cast<z64*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int kq = blockIdx.x / m;
+  const int p = blockIdx.x - kq * m;
+  const KlKpoint pk = ktab[kq];
+  const long nnr = static_cast<long>(n1) * n2 * n3;
+  const int np = p1 * n2 * n3;
+  z64* D = dsc + static_cast<long>(blockIdx.x) * nnr;
+  for (int v = 0; v < nb; ++v) {
+    for (int i = tid; i < np; i += KL_NT) grid[i] = mk(0.0, 0.0);
+    __syncthreads();
+    const z64* dp = dpsi + (static_cast<long>(kq) * m * nb + static_cast<long>(p) * nb + v) * ld;
+    for (int ig = tid; ig < pk.npw; ig += KL_NT) grid[kl_lin(pk.nl[ig], n1, p1)] = dp[ig];
+    __syncthreads();
+    kl_transform<+1>(grid, n1, n2, n3, p1);
+    const z64* ps = psik + (static_cast<long>(kq) * nb + v) * nnr;
+    for (long r = tid; r < nnr; r += KL_NT) {
+      const z64 t = cmul(conjg(ps[r]), grid[kl_lin(r, n1, p1)]);
+      D[r] = v == 0 ? mk(0.0, 0.0) + t : D[r] + t;
+    }
+    __syncthreads();
+  }
+  // each thread wrote the D points it now loads
+  for (long r = tid; r < nnr; r += KL_NT) grid[kl_lin(r, n1, p1)] = D[r];
+  __syncthreads();
+  kl_transform<-1>(grid, n1, n2, n3, p1);
+  const double sc = sck[kq];
+  z64* o = out + static_cast<long>(blockIdx.x) * ldq;
+  for (int ig = tid; ig < npwq; ig += KL_NT) o[ig] = cscale(grid[kl_lin(nlq[ig], n1, p1)], sc);
+}
+
+// dng[p][igq] += sum_kq dk[kq m + p][igq] (kq in order), igq < npwq
+__global__ void k_sum_k(int nks, int m, int npwq, long ldq, const z64* __restrict__ dk, z64* __restrict__ dng) {
+  const long n = static_cast<long>(m) * npwq;
+  for (long t = blockIdx.x * static_cast<long>(KL_NT) + threadIdx.x; t < n;
+       t += static_cast<long>(gridDim.x) * KL_NT) {
+    const int p = static_cast<int>(t / npwq);
+    const int ig = static_cast<int>(t - static_cast<long>(p) * npwq);
+    z64 acc = dng[p * ldq + ig];
+    for (int kq = 0; kq < nks; ++kq) acc = acc + dk[(static_cast<long>(kq) * m + p) * ldq + ig];
+    dng[p * ldq + ig] = acc;
+  }
+}
+
 constexpr size_t kKlMaxSmem = 160u * 1024u;
 // LDS the fused CG iteration uses besides the grid (its reduction arrays)
 constexpr size_t kKlCgStatic = sizeof(double) * (KL_NQ2 * (KL_NT / 64) + 2 * KL_NQ2);
@@ -317,6 +459,57 @@ bool kvloc_lds_available(int n1, int n2, int n3) {
 bool cg_iter_lds_available(int n1, int n2, int n3, int nq2) {
   return kl_size(n1) && kl_size(n2) && kl_size(n3) && nq2 >= 0 && nq2 <= KL_NQ2 &&
          kl_smem(n1, n2, n3) + kKlCgStatic <= kKlMaxSmem;
+}
+
+namespace {
+template <typename Kern>
+void kl_set_smem(Kern k) {
+  WH_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
+                             static_cast<int>(kKlMaxSmem)));
+}
+bool kl_q_attrs() {   // once per process (thread-safe static initialisation)
+  static const bool set = [] {
+    kl_set_smem(k_states_lds);
+    kl_set_smem(k_rhs_lds);
+    kl_set_smem(k_density_lds);
+    return true;
+  }();
+  return set;
+}
+}  // namespace
+
+void states_kmq_lds(int n1, int n2, int n3, int nks, int nb, long ld, const KlKpoint* ktab, const KlQpoint* qtab,
+                    z64* psik, hipStream_t s) {
+  WH_REQUIRE(kvloc_lds_available(n1, n2, n3), "states_kmq_lds: grid not supported");
+  if (nks <= 0 || nb <= 0) return;
+  (void)kl_q_attrs();
+  k_states_lds<<<nks * nb, KL_NT, kl_smem(n1, n2, n3), s>>>(n1, n2, n3, n1 | 1, nb, ld, ktab, qtab, psik);
+  WH_HIP(hipGetLastError());
+}
+
+void rhs_lds(int n1, int n2, int n3, int nks, int nb, int m, long ld, const KlKpoint* ktab, const z64* psik,
+             const z64* aux, z64* out, hipStream_t s) {
+  WH_REQUIRE(kvloc_lds_available(n1, n2, n3), "rhs_lds: grid not supported");
+  if (nks <= 0 || nb <= 0 || m <= 0) return;
+  (void)kl_q_attrs();
+  k_rhs_lds<<<nks * m * nb, KL_NT, kl_smem(n1, n2, n3), s>>>(n1, n2, n3, n1 | 1, nb, m * nb, ld, ktab, psik, aux,
+                                                             out);
+  WH_HIP(hipGetLastError());
+}
+
+void density_lds(int n1, int n2, int n3, int nks, int nb, int m, long ld, const KlKpoint* ktab, const z64* psik,
+                 const z64* dpsi, z64* dsc, int npwq, long ldq, const int* nlq, const double* sck, z64* dk, z64* dng,
+                 hipStream_t s) {
+  WH_REQUIRE(kvloc_lds_available(n1, n2, n3), "density_lds: grid not supported");
+  if (nks <= 0 || nb <= 0 || m <= 0) return;
+  (void)kl_q_attrs();
+  k_density_lds<<<nks * m, KL_NT, kl_smem(n1, n2, n3), s>>>(n1, n2, n3, n1 | 1, nb, m, ld, ktab, psik, dpsi, dsc,
+                                                            npwq, ldq, nlq, sck, dk);
+  WH_HIP(hipGetLastError());
+  const long n = static_cast<long>(m) * npwq;
+  const unsigned nblk = static_cast<unsigned>(std::min<long>((n + KL_NT - 1) / KL_NT, 1L << 16));
+  k_sum_k<<<nblk, KL_NT, 0, s>>>(nks, m, npwq, ldq, dk, dng);
+  WH_HIP(hipGetLastError());
 }
 
 void cg_iter_lds(int n1, int n2, int n3, int npw, long ld, const int* nl, const double* vrs, const double* g2,

@@ -58,6 +58,7 @@ struct QPointDev {
   bool is_gamma = false;      // q = 0: q + G = 0 is the first vector
   std::vector<int> ikq;       // 0-based index of k - q for each k
   std::vector<int> g0;        // (3, nks): k - q = k_ikq + g0, crystal integers
+  KlQpoint* d_kq = nullptr;   // the same two on the device, per k point
   bool set = false;
 };
 
@@ -82,6 +83,8 @@ struct KPath {
   DevBuf dav[8];
   hipEvent_t cg_ev[4] = {nullptr, nullptr, nullptr, nullptr};   // lagged reads of the CG loop
   KlKpoint* ktab = nullptr;   // device: each k point's sphere and projectors (batched CG)
+  double* sck = nullptr;      // device: 2 w_k / (Omega N_r) per k point (densities at q)
+  DevBuf w_dk;                // densities per (k point, perturbation) before the sum over k
 };
 
 // The k-point CG reads an iteration's active count this many iterations late:
@@ -99,6 +102,7 @@ void free_kpath(KPath* k) {
                     &k->w_eu, &k->w_ecol, &k->w_epcol, &k->w_dv, &k->w_dn, &k->w_in, &k->w_out, &k->w_in2})
     b->release();
   for (DevBuf& b : k->dav) b.release();
+  k->w_dk.release();
   for (hipEvent_t e : k->cg_ev)
     if (e) (void)hipEventDestroy(e);
   delete k;
@@ -649,15 +653,20 @@ static void cg_k_count(const int* rec, int last) {
   }
 }
 
-// Whether the CG of every k point's cols columns can run as one solve: the
-// fused iteration applies (small grid, at most 32 projector rows in the real
-// view) and the solve's vectors (seven of nks cols columns) fit the grid budget.
-static bool kbatch_ok(const KPath& K, int cols) {
+// Whether dfpt at q can take every k point at once for m perturbations: the
+// fused CG iteration applies (small grid, at most 32 projector rows in the
+// real view) and the solve's vectors (seven of nks m nbndval columns), the
+// states at k - q and density scratch grids and the per-k densities fit the
+// grid budget.
+static bool kbatch_ok(const KPath& K, int m) {
   const Context& c = ctx();
-  const long all = static_cast<long>(K.nks) * cols;
+  const long all = static_cast<long>(K.nks) * m * K.nbndval;
+  const long grids = static_cast<long>(K.nks) * (K.nbndval + m);   // states at k - q, density scratch
+  const long bytes = static_cast<long>(sizeof(z64)) * (7 * all * K.npwx + grids * K.fft.nnr() +
+                                                       static_cast<long>(K.nks) * m * K.npwqx);
   return K.nks > 1 && klds_enabled() && !c.l_kinetic_only && all <= (1L << 24) &&
          cg_iter_lds_available(K.fft.n1, K.fft.n2, K.fft.n3, 2 * (K.nbndval + K.nkb)) &&
-         7 * all * K.npwx * static_cast<long>(sizeof(z64)) <= static_cast<long>(K.grid_budget);
+         bytes <= static_cast<long>(K.grid_budget);
 }
 
 // dng (npwqx, m) = chi0(q) dvg for q point iq (0-based)  (dfpt_module.f90:163-505)
@@ -736,23 +745,41 @@ static int dfpt_q(KPath& K, int iq, int m, const z64* dvg, z64* dng, double tr2,
                                                                    nullptr, nullptr, 1, dng + p0 * ldq);
     WH_HIP(hipGetLastError());
   };
-  if (tr2 >= 0.0 && mp == m && kbatch_ok(K, m * nb)) {
-    // every k point's columns in one CG (350-361): the right-hand sides of
-    // all k points, one solve whose iterations take them all at once, then
-    // the densities (the states at k - q are transformed again)
+  if (tr2 >= 0.0 && mp == m && kbatch_ok(K, m)) {
+    // every k point at once (350-361): the states at k - q and the
+    // right-hand sides of all k points on LDS grids, -P_c and M^-1 per k
+    // point, one CG whose iterations take every k point's columns, then the
+    // densities on LDS grids and their sum over k in k order
+    const int n1 = K.fft.n1, n2 = K.fft.n2, n3 = K.fft.n3;
     const int cols = m * nb;
     const size_t all = static_cast<size_t>(K.nks) * cols;
+    z64* psik = K.g_b.as<z64>(static_cast<size_t>(K.nks) * nb * nnr);
     z64* dvpsi = K.w_dv.as<z64>(all * ld);
     z64* dpsi = K.w_x.as<z64>(all * ld);
     double* d_e = K.w_ecol.as<double>(all);
     double* d_ep = K.w_epcol.as<double>(all);
+    WH_TIMED("kgrid_lds", states_kmq_lds(n1, n2, n3, K.nks, nb, ld, K.ktab, Q.d_kq, psik, s));
+    WH_TIMED("kgrid_lds", rhs_lds(n1, n2, n3, K.nks, nb, m, ld, K.ktab, psik, aux, dvpsi, s));
     for (int ik = 0; ik < K.nks; ++ik) {
+      const KPointDev& P = K.k[ik];
+      const KPointDev& PQ = K.k[Q.ikq[ik]];
       const size_t o = static_cast<size_t>(ik) * cols;
-      rhs(ik, 0, m, states_kmq(ik), dvpsi + o * ld, dpsi + o * ld, d_e + o, d_ep + o);
+      k_band_cols<<<blocks(cols), NT, 0, s>>>(cols, nb, PQ.d_et, P.d_eprec, d_e + o, d_ep + o);
+      minus_pc_k(K, ik, dvpsi + o * ld, cols);
+      k_precond<<<blocks(static_cast<long>(cols) * ld), NT, 0, s>>>(cols, P.npw, ld, P.g2kin, d_ep + o,
+                                                                     dvpsi + o * ld, dpsi + o * ld);
+      WH_HIP(hipGetLastError());
     }
     launched.push_back(linsolve_k(K, 0, K.nks, cols, dvpsi, dpsi, d_e, d_ep, tr2, nb, m, recs, d_nonconv));
-    for (int ik = 0; ik < K.nks; ++ik)
-      density(ik, 0, m, states_kmq(ik), dpsi + static_cast<size_t>(ik) * cols * ld);
+    if (!K.sck) {
+      std::vector<double> sk(K.nks);
+      for (int ik = 0; ik < K.nks; ++ik) sk[ik] = 2.0 * K.wk[ik] / K.omega / static_cast<double>(nnr);
+      K.sck = upload(K, sk.data(), sk.size());
+    }
+    z64* dsc = K.g_c.as<z64>(static_cast<size_t>(K.nks) * m * nnr);
+    z64* dk = K.w_dk.as<z64>(static_cast<size_t>(K.nks) * m * ldq);
+    WH_TIMED("kgrid_lds", density_lds(n1, n2, n3, K.nks, nb, m, ld, K.ktab, psik, dpsi, dsc, Q.npwq, ldq, Q.nl,
+                                      K.sck, dk, dng, s));
   } else {
     for (int ik = 0; ik < K.nks; ++ik) {
       const z64* psik = states_kmq(ik);
@@ -1102,7 +1129,7 @@ int wh_set_qpoint(int iq, int npwq, const int* nl, const double* gq2kin, const i
   WH_HIP(hipStreamSynchronize(ctx().stream));
   QPointDev& Q = K.q[iq - 1];
   for (void* p : {static_cast<void*>(Q.nl), static_cast<void*>(Q.gq2), static_cast<void*>(Q.ig),
-                  static_cast<void*>(Q.sqvc)})
+                  static_cast<void*>(Q.sqvc), static_cast<void*>(Q.d_kq)})
     release(K, p);
   Q.sqvc = nullptr;
   const long nnr = K.fft.nnr();
@@ -1130,6 +1157,9 @@ int wh_set_qpoint(int iq, int npwq, const int* nl, const double* gq2kin, const i
   Q.nl = upload(K, hnl.data(), hnl.size());
   Q.gq2 = upload(K, hg2.data(), hg2.size());
   Q.ig = upload(K, hig.data(), hig.size());
+  std::vector<KlQpoint> kq(K.nks);
+  for (int k = 0; k < K.nks; ++k) kq[k] = KlQpoint{Q.ikq[k], Q.g0[3 * k], Q.g0[3 * k + 1], Q.g0[3 * k + 2]};
+  Q.d_kq = upload(K, kq.data(), kq.size());
   if (sqvc) {
     std::vector<double> hs(npwq);
     WH_HIP(hipMemcpy(hs.data(), sqvc, sizeof(double) * npwq, hipMemcpyDefault));
