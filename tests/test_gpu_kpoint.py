@@ -7,6 +7,8 @@ g0 = -b1; on the 3x1x1 grid k and q are not their own inverses, so the
 states are genuinely complex).  The
 runs use a system of their own, destroyed at the end.  Each case prints its
 name before it runs."""
+import os
+
 import numpy as np
 import pytest
 
@@ -103,6 +105,17 @@ def test_dfpt_and_apply_operator_at_q(gpu, ksys):
         assert _rel(out, ref) < 1e-9
         if qp.is_gamma:
             assert np.all(out[:, 0] == 0)
+        # the k points one after another (full-grid transforms around a
+        # solve per k point) instead of all at once on LDS grids
+        print(f"case kpoint dfpt iq={iq + 1} tight, k points in turn", flush=True)
+        os.environ["WH_KBATCH"] = "0"
+        try:
+            out1, ierr1 = gpu.dfpt(3, dvg, 1e-20, iq=iq + 1, return_ierr=True)
+        finally:
+            del os.environ["WH_KBATCH"]
+        assert not ierr1.any()
+        assert _rel(out1, ref) < 1e-9
+        assert _rel(out1, out) < 1e-12
         print(f"case kpoint apply_operator iq={iq + 1}", flush=True)
         sq = sqvc_init(qb, kgs.npwqx)
         assert _rel(gpu.apply_operator(3, dvg, 1e-20, iq=iq + 1), K.apply_operator_q(kgs, iq, sq, dvg, 1e-20)) < 1e-9

@@ -659,6 +659,8 @@ static void cg_k_count(const int* rec, int last) {
 // states at k - q and density scratch grids and the per-k densities fit the
 // grid budget.
 static bool kbatch_ok(const KPath& K, int m) {
+  const char* ev = std::getenv("WH_KBATCH");   // read per call: 0 keeps the k points one after another
+  if (ev && ev[0] == '0') return false;
   const Context& c = ctx();
   const long all = static_cast<long>(K.nks) * m * K.nbndval;
   const long grids = static_cast<long>(K.nks) * (K.nbndval + m);   // states at k - q, density scratch
