@@ -177,7 +177,9 @@ struct KlKpoint {
   int npw;
   const int* nl;
   const double* g2;
-  const double* q2;
+  const double* q2;      // real view: evc, i evc, vkb, i vkb (2 ld rows each)
+  const double* et;      // nbndval energies
+  const double* eprec;   // nbndval preconditioner energies
 };
 bool cg_iter_lds_available(int n1, int n2, int n3, int nq2);
 // The grids of dfpt at q on LDS grids for every k point in one launch each
@@ -191,6 +193,18 @@ struct KlQpoint {
 };
 void states_kmq_lds(int n1, int n2, int n3, int nks, int nb, long ld, const KlKpoint* ktab, const KlQpoint* qtab,
                     z64* psik, hipStream_t s);
+// For the nks m nb right-hand side columns f (k point k = col / (m nb), band
+// v = col % nb; 2 nb <= 32): e[col] = et(k - q)[v], ep[col] = eprec(k)[v],
+// f <- -f + evc (evc^H f) on the k sphere (apply_alpha_pc with alpha = -1,
+// dfpt_module.f90:346) and x = f / max(1, |k+G|^2 / ep) (348)
+void pc_precond_all(int nks, int nb, int m, long ld, const KlKpoint* ktab, const KlQpoint* qtab, z64* f, z64* x,
+                    double* e, double* ep, hipStream_t s);
+// The CG's first step for m = nks cols columns of cg_iter_lds's kind
+// (linsolve_sternheimer_m_wfcts.f90:127-205): g = (H_k - e + alpha P_v) x - b
+// with the local term in LDS, h = g / max(1, |k+G|^2 / eprec), rho[col] = Re <h|g>
+void cg_first_lds(int n1, int n2, int n3, long ld, const double* vrs, int nq2, const double* qs, int m,
+                  const double* e, const double* ep, const z64* x, const z64* b, z64* g, z64* h, double* rho,
+                  const KlKpoint* ktab, int cols_per_k, hipStream_t s);
 void rhs_lds(int n1, int n2, int n3, int nks, int nb, int m, long ld, const KlKpoint* ktab, const z64* psik,
              const z64* aux, z64* out, hipStream_t s);
 void density_lds(int n1, int n2, int n3, int nks, int nb, int m, long ld, const KlKpoint* ktab, const z64* psik,

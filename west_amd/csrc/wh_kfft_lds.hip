@@ -224,8 +224,13 @@ struct KlCgArgs {
   double* rho;
   const KlKpoint* ktab;
   int cols_per_k;
+  const z64* b;   // FIRST: the right-hand sides
 };
 
+// FIRST: the CG's first step instead (127-205), every launched column in
+// order (col = lb, eu = e by column): the operator acts on x, then
+// g = A x - b, h = g / max(1, g2kin/eprec), rho[col] = Re <h|g>.
+template <bool FIRST>
 __global__ __launch_bounds__(KL_NT) void k_cg_iter_lds(KlCgArgs a) {
   extern __shared__ char smem_raw[];
   __shared__ double red[KL_NQ2][KL_NT / 64];
@@ -234,8 +239,8 @@ __global__ __launch_bounds__(KL_NT) void k_cg_iter_lds(KlCgArgs a) {
   z64* grid = reinterpret_cast<z64*>(smem_raw);
   const int tid = threadIdx.x;
   const int lb = blockIdx.x;
-  if (lb >= *a.nc) return;   // whole workgroup
-  const int col = a.l2i[lb];
+  if (!FIRST && lb >= *a.nc) return;   // whole workgroup
+  const int col = FIRST ? lb : a.l2i[lb];
   const long ld = a.ld;
   int npw = a.npw;
   const int* nl = a.nl;
@@ -251,23 +256,28 @@ __global__ __launch_bounds__(KL_NT) void k_cg_iter_lds(KlCgArgs a) {
   const int nq2 = a.nq2;
   const long ldr = 2 * ld;   // real view
   z64* hc = a.h + col * ld;
-  z64* ho = a.hold + col * ld;
   z64* xc = a.x + col * ld;
   z64* gc = a.g + col * ld;
-  z64* bb = a.bc + lb * ld;
-  z64* tt = a.tc + lb * ld;
+  z64* ho = FIRST ? nullptr : a.hold + col * ld;
+  z64* bb = FIRST ? nullptr : a.bc + lb * ld;
+  z64* tt = FIRST ? nullptr : a.tc + lb * ld;
   const int np = a.p1 * a.n2 * a.n3;
   for (int i = tid; i < np; i += KL_NT) grid[i] = mk(0.0, 0.0);
   __syncthreads();
-  // new direction, scattered into the grid; bra sums of the projectors
-  const double bt = a.beta[lb];
+  // new direction (FIRST: x), scattered into the grid; bra sums of the projectors
+  const double bt = FIRST ? 0.0 : a.beta[lb];
   double acc[KL_NQ2];
 #pragma unroll
   for (int i = 0; i < KL_NQ2; ++i) acc[i] = 0.0;
   for (int ig = tid; ig < ld; ig += KL_NT) {
-    const z64 v = cscale(hc[ig], -1.0) + cscale(ho[ig], bt);
-    hc[ig] = v;
-    bb[ig] = v;
+    z64 v;
+    if constexpr (FIRST) {
+      v = xc[ig];
+    } else {
+      v = cscale(hc[ig], -1.0) + cscale(ho[ig], bt);
+      hc[ig] = v;
+      bb[ig] = v;
+    }
     if (ig < npw) {
       const int r = nl[ig];
       grid[r % a.n1 + a.p1 * (r / a.n1)] = v;
@@ -283,6 +293,37 @@ __global__ __launch_bounds__(KL_NT) void k_cg_iter_lds(KlCgArgs a) {
   // t: local + kinetic + projectors; dots of the new direction
   const double sc = 1.0 / (static_cast<double>(a.n1) * a.n2 * a.n3);
   const double eb = a.eu[lb];
+  if constexpr (FIRST) {
+    const z64* bcol = a.b + col * ld;
+    const double e = a.ep[col];
+    double rr[1] = {0.0};
+    for (int ig = tid; ig < ld; ig += KL_NT) {
+      if (ig >= npw) {
+        gc[ig] = mk(0.0, 0.0) - bcol[ig];
+        hc[ig] = mk(0.0, 0.0);
+        continue;
+      }
+      const int r = nl[ig];
+      const z64 v = xc[ig];
+      z64 t = cscale(grid[r % a.n1 + a.p1 * (r / a.n1)], sc) + cscale(v, g2[ig] - eb);
+      double kr = 0.0, ki = 0.0;
+#pragma unroll
+      for (int i = 0; i < KL_NQ2; ++i)
+        if (i < nq2) {
+          kr += q2[i * ldr + 2 * ig] * c2s[i];
+          ki += q2[i * ldr + 2 * ig + 1] * c2s[i];
+        }
+      t = t + mk(kr, ki);
+      const z64 gn = t - bcol[ig];
+      gc[ig] = gn;
+      const z64 hn = cscale(gn, 1.0 / fmax(1.0, g2[ig] / e));
+      hc[ig] = hn;
+      rr[0] += hn.x * gn.x + hn.y * gn.y;
+    }
+    kl_block_sums(rr, 1, red, sums);
+    if (tid == 0) a.rho[col] = sums[0];
+    return;
+  } else {
   double dots[2] = {0.0, 0.0};
   for (int ig = tid; ig < ld; ig += KL_NT) {
     if (ig >= npw) {
@@ -324,6 +365,58 @@ __global__ __launch_bounds__(KL_NT) void k_cg_iter_lds(KlCgArgs a) {
   }
   kl_block_sums(rr, 1, red, sums);
   if (tid == 0) a.rho[col] = sums[0];
+  }
+}
+
+// -P_c and M^-1 on the right-hand side columns of every k point (no grid),
+// one workgroup per column j (k point j / cols, band j % nb):
+//   f <- evc (evc^H f) - f on the k sphere, x = f / max(1, g2kin/eprec), 0 beyond npw
+__global__ __launch_bounds__(KL_NT) void k_pc_precond(int nb, int cols, long ld, const KlKpoint* __restrict__ ktab,
+                                                    const KlQpoint* __restrict__ qtab, z64* __restrict__ f,
+                                                    z64* __restrict__ x, double* __restrict__ e,
+                                                    double* __restrict__ ep) {
+  __shared__ double red[KL_NQ2][KL_NT / 64];
+  __shared__ double sums[KL_NQ2];
+  const int tid = threadIdx.x;
+  const int j = blockIdx.x;
+  const int kq = j / cols;
+  const int v = (j - kq * cols) % nb;
+  const KlKpoint pk = ktab[kq];
+  const double epj = pk.eprec[v];
+  if (tid == 0) {
+    e[j] = ktab[qtab[kq].ikq].et[v];   // energies at k - q (dfpt_module.f90:222)
+    ep[j] = epj;
+  }
+  const int nr = 2 * nb;   // evc and i evc in the real view
+  const long ldr = 2 * ld;
+  z64* fc = f + static_cast<long>(j) * ld;
+  z64* xc = x + static_cast<long>(j) * ld;
+  double acc[KL_NQ2];
+#pragma unroll
+  for (int i = 0; i < KL_NQ2; ++i) acc[i] = 0.0;
+  for (int ig = tid; ig < pk.npw; ig += KL_NT) {
+    const z64 fv = fc[ig];
+#pragma unroll
+    for (int i = 0; i < KL_NQ2; ++i)
+      if (i < nr) acc[i] += pk.q2[i * ldr + 2 * ig] * fv.x + pk.q2[i * ldr + 2 * ig + 1] * fv.y;
+  }
+  kl_block_sums(acc, nr, red, sums);
+  for (int ig = tid; ig < ld; ig += KL_NT) {
+    if (ig >= pk.npw) {
+      xc[ig] = mk(0.0, 0.0);
+      continue;
+    }
+    double kr = 0.0, ki = 0.0;
+#pragma unroll
+    for (int i = 0; i < KL_NQ2; ++i)
+      if (i < nr) {
+        kr += pk.q2[i * ldr + 2 * ig] * sums[i];
+        ki += pk.q2[i * ldr + 2 * ig + 1] * sums[i];
+      }
+    const z64 fn = mk(kr, ki) - fc[ig];
+    fc[ig] = fn;
+    xc[ig] = cscale(fn, 1.0 / fmax(1.0, pk.g2[ig] / epj));
+  }
 }
 
 // States at k - q in real space with the phase exp(-i g0.r) for every k
@@ -521,15 +614,42 @@ void cg_iter_lds(int n1, int n2, int n3, int npw, long ld, const int* nl, const 
   if (cap <= 0) return;
   const size_t dyn = kl_smem(n1, n2, n3);
   static const bool attr_set = [] {   // once per process (thread-safe static initialisation)
-    WH_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_cg_iter_lds),
+    WH_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_cg_iter_lds<false>),
                                hipFuncAttributeMaxDynamicSharedMemorySize,
                                static_cast<int>(kKlMaxSmem - kKlCgStatic)));
     return true;
   }();
   (void)attr_set;
   const KlCgArgs a{n1, n2, n3, n1 | 1, npw, ld, nl, vrs, g2, nq2, q2, qs, l2i, nc, beta, eu, ep,
-                   h, hold, x, g, bc, tc, rho, ktab, cols_per_k};
-  k_cg_iter_lds<<<cap, KL_NT, dyn, s>>>(a);
+                   h, hold, x, g, bc, tc, rho, ktab, cols_per_k, nullptr};
+  k_cg_iter_lds<false><<<cap, KL_NT, dyn, s>>>(a);
+  WH_HIP(hipGetLastError());
+}
+
+void cg_first_lds(int n1, int n2, int n3, long ld, const double* vrs, int nq2, const double* qs, int m,
+                  const double* e, const double* ep, const z64* x, const z64* b, z64* g, z64* h, double* rho,
+                  const KlKpoint* ktab, int cols_per_k, hipStream_t s) {
+  WH_REQUIRE(cg_iter_lds_available(n1, n2, n3, nq2), "cg_first_lds: grid or projector count not supported");
+  WH_REQUIRE(ktab && cols_per_k > 0, "cg_first_lds: a k-point table and cols_per_k > 0 are required");
+  if (m <= 0) return;
+  static const bool attr_set = [] {   // once per process (thread-safe static initialisation)
+    WH_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_cg_iter_lds<true>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               static_cast<int>(kKlMaxSmem - kKlCgStatic)));
+    return true;
+  }();
+  (void)attr_set;
+  const KlCgArgs a{n1, n2, n3, n1 | 1, 0, ld, nullptr, vrs, nullptr, nq2, nullptr, qs, nullptr, nullptr, nullptr,
+                   e, ep, h, nullptr, const_cast<z64*>(x), g, nullptr, nullptr, rho, ktab, cols_per_k, b};
+  k_cg_iter_lds<true><<<m, KL_NT, kl_smem(n1, n2, n3), s>>>(a);
+  WH_HIP(hipGetLastError());
+}
+
+void pc_precond_all(int nks, int nb, int m, long ld, const KlKpoint* ktab, const KlQpoint* qtab, z64* f, z64* x,
+                    double* e, double* ep, hipStream_t s) {
+  WH_REQUIRE(nb > 0 && 2 * nb <= KL_NQ2, "pc_precond_all: 2 nbndval must be <= 32");
+  if (nks <= 0 || m <= 0) return;
+  k_pc_precond<<<nks * m * nb, KL_NT, 0, s>>>(nb, m * nb, ld, ktab, qtab, f, x, e, ep);
   WH_HIP(hipGetLastError());
 }
 

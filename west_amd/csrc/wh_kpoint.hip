@@ -584,20 +584,6 @@ static int linsolve_k(KPath& K, int ik, int nkb, int cols, z64* b, z64* x, const
   const int itmax = c.n_dfpt_maxiter;
   WH_HIP(hipMemsetAsync(hold, 0, sizeof(z64) * vec, s));
   WH_HIP(hipMemsetAsync(d_conv, 0, sizeof(int) * (static_cast<size_t>(m) + 4), s));
-  // step 1 (127-138): g = A x - b, and h = M^-1 g, rho of every band
-  iota(d_l2i, m, s);
-  for (int kb = 0; kb < nkb; ++kb) {
-    const int kk = nkb == 1 ? ik : kb;
-    const long o = static_cast<long>(kb) * cols;
-    stern_k(K, kk, x + o * ld, g + o * ld, d_e + o, cols, K.alpha_pv);
-  }
-  count_calls(m, nullptr, group);
-  k_sub<<<blocks(static_cast<long>(vec)), NT, 0, s>>>(static_cast<long>(vec), b, g);
-  for (int kb = 0; kb < nkb; ++kb) {
-    const KPointDev& Pk = K.k[nkb == 1 ? ik : kb];
-    k_cg_prec_rho<<<cols, NT, 0, s>>>(d_l2i + kb * cols, nullptr, Pk.npw, ld, Pk.g2kin, d_ep, g, h, d_rhob);
-  }
-  WH_HIP(hipGetLastError());
   // small grids with few projector rows: the iteration after cg_select is
   // one workgroup per column (cg_iter_lds) instead of five launches
   const int nq2 = 2 * (K.nbndval + K.nkb);
@@ -605,6 +591,19 @@ static int linsolve_k(KPath& K, int ik, int nkb, int cols, z64* b, z64* x, const
                      cg_iter_lds_available(K.fft.n1, K.fft.n2, K.fft.n3, nq2);
   WH_REQUIRE(fused || nkb == 1, "linsolve_k: several k points need the fused iteration");
   if (fused) set_qscale2(K, K.alpha_pv);
+  // step 1 (127-138): g = A x - b, and h = M^-1 g, rho of every band
+  if (nkb > 1) {   // every k point's columns in one launch
+    WH_TIMED("cg_iter_lds", cg_first_lds(K.fft.n1, K.fft.n2, K.fft.n3, ld, K.vrs, nq2, K.qscale2, m, d_e, d_ep, x,
+                                         b, g, h, d_rhob, K.ktab, cols, s));
+    c.nstern += m;
+  } else {
+    iota(d_l2i, m, s);
+    stern_k(K, ik, x, g, d_e, m, K.alpha_pv);
+    k_sub<<<blocks(static_cast<long>(vec)), NT, 0, s>>>(static_cast<long>(vec), b, g);
+    k_cg_prec_rho<<<m, NT, 0, s>>>(d_l2i, nullptr, P.npw, ld, P.g2kin, d_ep, g, h, d_rhob);
+    WH_HIP(hipGetLastError());
+  }
+  count_calls(m, nullptr, group);
   int last = 0;   // iterations launched
   int cap = m;
   for (int it = 1; it <= itmax; ++it) {
@@ -762,16 +761,7 @@ static int dfpt_q(KPath& K, int iq, int m, const z64* dvg, z64* dng, double tr2,
     double* d_ep = K.w_epcol.as<double>(all);
     WH_TIMED("kgrid_lds", states_kmq_lds(n1, n2, n3, K.nks, nb, ld, K.ktab, Q.d_kq, psik, s));
     WH_TIMED("kgrid_lds", rhs_lds(n1, n2, n3, K.nks, nb, m, ld, K.ktab, psik, aux, dvpsi, s));
-    for (int ik = 0; ik < K.nks; ++ik) {
-      const KPointDev& P = K.k[ik];
-      const KPointDev& PQ = K.k[Q.ikq[ik]];
-      const size_t o = static_cast<size_t>(ik) * cols;
-      k_band_cols<<<blocks(cols), NT, 0, s>>>(cols, nb, PQ.d_et, P.d_eprec, d_e + o, d_ep + o);
-      minus_pc_k(K, ik, dvpsi + o * ld, cols);
-      k_precond<<<blocks(static_cast<long>(cols) * ld), NT, 0, s>>>(cols, P.npw, ld, P.g2kin, d_ep + o,
-                                                                     dvpsi + o * ld, dpsi + o * ld);
-      WH_HIP(hipGetLastError());
-    }
+    pc_precond_all(K.nks, nb, m, ld, K.ktab, Q.d_kq, dvpsi, dpsi, d_e, d_ep, s);
     launched.push_back(linsolve_k(K, 0, K.nks, cols, dvpsi, dpsi, d_e, d_ep, tr2, nb, m, recs, d_nonconv));
     if (!K.sck) {
       std::vector<double> sk(K.nks);
@@ -1111,10 +1101,10 @@ int wh_set_kpoint(int ik, int npw, const int* nl, const double* g2kin, const dou
   P.d_et = upload(K, P.et.data(), P.et.size());
   P.d_eprec = upload(K, P.eprec.data(), P.eprec.size());
   if (!K.ktab) {
-    const std::vector<KlKpoint> none(K.nks, KlKpoint{0, nullptr, nullptr, nullptr});
+    const std::vector<KlKpoint> none(K.nks, KlKpoint{0, nullptr, nullptr, nullptr, nullptr, nullptr});
     K.ktab = upload(K, none.data(), none.size());
   }
-  const KlKpoint kt{npw, P.nl, P.g2kin, reinterpret_cast<const double*>(P.q2)};
+  const KlKpoint kt{npw, P.nl, P.g2kin, reinterpret_cast<const double*>(P.q2), P.d_et, P.d_eprec};
   WH_HIP(hipMemcpy(K.ktab + (ik - 1), &kt, sizeof(kt), hipMemcpyHostToDevice));
   P.set = true;
   return 0;
