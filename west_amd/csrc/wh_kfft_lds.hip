@@ -1,13 +1,19 @@
-// Local-potential term of the Sternheimer operator at a k point on a small
-// full grid, in one workgroup per band with the whole grid in LDS:
-//   t = (1/N_r) gather(F^-1 V(r) F scatter(x)) + (|k+G|^2 - e) x
-// (the same result as k_scatter, the two GridFft transforms, k_mul_real and
-// k_gather of the q-point path, which make six launches and six HBM round
-// trips of the grid per band).  Each 1D pass gives one line to one thread,
-// transformed in registers with a compile-time radix plan for its length;
-// the a3 pass applies V(r) between its inverse and forward transforms.  The
-// LDS grid has an odd row pitch, so lines read in lock step by neighbouring
-// threads fall on distinct banks.
+// The q-point path on small full grids, one workgroup per band, column or
+// perturbation with the whole grid in LDS:
+// * the local-potential term of the Sternheimer operator at a k point,
+//     t = (1/N_r) gather(F^-1 V(r) F scatter(x)) + (|k+G|^2 - e) x
+//   (the same result as k_scatter, the two GridFft transforms, k_mul_real and
+//   k_gather, which make six launches and six HBM round trips of the grid);
+// * one CG iteration of the Sternheimer solve per column (direction, the
+//   operator with its projectors, CG step, next preconditioned residual), and
+//   its first step, over the columns of one k point or of every k point;
+// * the states at k - q, the right-hand sides and the response densities of
+//   dfpt at q for every k point, and -P_c with M^-1 on the right-hand sides.
+// Each 1D pass gives one line to one thread, transformed in registers with a
+// compile-time radix plan for its length; the operator's a3 pass applies V(r)
+// between its inverse and forward transforms.  The LDS grid has an odd row
+// pitch, so lines read in lock step by neighbouring threads fall on distinct
+// banks.
 #include <algorithm>
 
 #include "wh_fft.h"
